@@ -1,0 +1,129 @@
+"""ctypes binding of the CPU oracle (fmm3d_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product path never loads
+it.
+
+Array conventions (see the C file header): fields (nnr, nnx, nnz) phi
+fastest; veln (nnr+3, nnx+3, nnz+3) on nodes -1..n+1; vertex arrays
+(nvz+2, nvx+2, nvr+2) r fastest; receivers (nrc, 3) height km, colat rad,
+lon rad.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+
+class OrGrid(C.Structure):
+    _fields_ = [("nvr", C.c_int), ("nvx", C.c_int), ("nvz", C.c_int),
+                ("gor", C.c_double), ("gox", C.c_double), ("goz", C.c_double),
+                ("dvr", C.c_double), ("dvx", C.c_double), ("dvz", C.c_double),
+                ("nnr", C.c_int), ("nnx", C.c_int), ("nnz", C.c_int),
+                ("rgor", C.c_double), ("rgox", C.c_double), ("rgoz", C.c_double),
+                ("dnr", C.c_double), ("dnx", C.c_double), ("dnz", C.c_double),
+                ("earth", C.c_double)]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        lp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        up = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+        L.or_gridder.argtypes = [C.POINTER(OrGrid), dp, dp]
+        L.or_gridder.restype = None
+        L.or_fmm_tele.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, dp]
+        L.or_fmm_tele.restype = C.c_long
+        L.or_fmm_tele_part.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, dp, C.c_int]
+        L.or_fmm_tele_part.restype = C.c_int
+        L.or_fim_sweep.argtypes = [C.POINTER(OrGrid), dp, up, C.c_int, dp, dp]
+        L.or_fim_sweep.restype = C.c_double
+        L.or_srtimes.argtypes = [C.POINTER(OrGrid), dp, C.c_int, dp, fp]
+        L.or_srtimes.restype = C.c_int
+        L.or_rpaths.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, dp, lp, ip, fp, C.c_long, ip, ip]
+        L.or_rpaths.restype = C.c_long
+        L.or_octant_solve.argtypes = [ip, dp, dp, dp, C.c_double]
+        L.or_octant_solve.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def or_grid(g) -> OrGrid:
+    """GridDesc (degrees) -> OrGrid (radians, reference conversion)."""
+    return OrGrid(g.nvr, g.nvx, g.nvz, g.gor, g.gox, g.goz, g.dvr, g.dvx, g.dvz,
+                  g.nnr, g.nnx, g.nnz, g.rgor, g.rgox, g.rgoz, g.dnr, g.dnx, g.dnz, g.earth)
+
+
+def gridder(g, velv: np.ndarray) -> np.ndarray:
+    out = np.zeros((g.nnr + 3, g.nnx + 3, g.nnz + 3))
+    lib().or_gridder(C.byref(or_grid(g)), np.ascontiguousarray(velv, dtype=np.float64), out)
+    return out
+
+
+def fmm_tele(g, veln: np.ndarray, face: np.ndarray, fom: int = 1) -> np.ndarray:
+    ttn = np.zeros((g.nnr, g.nnx, g.nnz))
+    lib().or_fmm_tele(C.byref(or_grid(g)), veln, np.ascontiguousarray(face, dtype=np.float64), fom, ttn)
+    return ttn
+
+
+def fmm_tele_part(g, velv: np.ndarray, face: np.ndarray, fom: int = 1, max_pass: int = 1000):
+    """The reference's 2x2x2-partitioned solve; returns (ttn, passes)."""
+    ttn = np.zeros((g.nnr, g.nnx, g.nnz))
+    n = lib().or_fmm_tele_part(C.byref(or_grid(g)), np.ascontiguousarray(velv, dtype=np.float64),
+                               np.ascontiguousarray(face, dtype=np.float64), fom, ttn, max_pass)
+    return ttn, n
+
+
+def fim_sweep(g, veln, fixed, t_in, fom=1):
+    out = np.empty_like(t_in)
+    d = lib().or_fim_sweep(C.byref(or_grid(g)), veln, np.ascontiguousarray(fixed, dtype=np.uint8),
+                           fom, np.ascontiguousarray(t_in), out)
+    return out, d
+
+
+def srtimes(g, ttn: np.ndarray, rcv: np.ndarray) -> np.ndarray:
+    out = np.zeros(len(rcv), dtype=np.float32)
+    lib().or_srtimes(C.byref(or_grid(g)), np.ascontiguousarray(ttn), len(rcv),
+                     np.ascontiguousarray(rcv, dtype=np.float64), out)
+    return out
+
+
+def rpaths(g, veln: np.ndarray, ttn: np.ndarray, rcv: np.ndarray, cap: int = 0):
+    """Returns (row_ptr int64 [nrc+1], cols int32, vals float32, ok int32, steps int32)."""
+    nrc = len(rcv)
+    if cap <= 0:
+        cap = nrc * g.nvert if hasattr(g, "nvert") else nrc * 100000
+    rp = np.zeros(nrc + 1, dtype=np.int64)
+    cols = np.zeros(cap, dtype=np.int32)
+    vals = np.zeros(cap, dtype=np.float32)
+    ok = np.zeros(nrc, dtype=np.int32)
+    steps = np.zeros(nrc, dtype=np.int32)
+    n = lib().or_rpaths(C.byref(or_grid(g)), veln, np.ascontiguousarray(ttn), nrc,
+                        np.ascontiguousarray(rcv, dtype=np.float64), rp, cols, vals, cap, ok, steps)
+    if n < 0:
+        raise RuntimeError("or_rpaths: capacity too small")
+    return rp, cols[:n].copy(), vals[:n].copy(), ok, steps
+
+
+def octant_solve(ord3, t1, t2, h, slown) -> float:
+    return lib().or_octant_solve(np.asarray(ord3, dtype=np.int32), np.asarray(t1, dtype=np.float64),
+                                 np.asarray(t2, dtype=np.float64), np.asarray(h, dtype=np.float64), slown)
