@@ -1,0 +1,124 @@
+/*
+ * fmm3d.h -- C ABI of libfmm3d_hip.so, the MI355X forward solver for the
+ * fm3dt_parallel file contract (3-D spherical eikonal solve, B-spline
+ * gridder, receiver times, rays with Frechet rows).
+ *
+ * The reference has no library interface: its boundary is a process and a
+ * set of files (sub.sh:24,33 runs `mpirun -np 9 ./fm3dt_parallel` in a
+ * directory holding fm3dt_parallel.in, the .vtx model, sources.dat,
+ * receivers.dat and itimes.out; it writes rtravel.out and frechet.out).
+ * Each entry point below replaces one stage of that program; the comment
+ * names the reference code it stands for.  Plain pointers and sizes only;
+ * host buffers are caller-owned; device memory is owned by the context.
+ *
+ * Conventions
+ *   - Grid descriptors carry the six .vtx header lines verbatim (degrees);
+ *     the library converts with the reference's float32-rounded pi
+ *     (fm3dt_parallel.f90:61, 6110-6117).
+ *   - Host field arrays are (nnr, nnx, nnz) with phi fastest: the memory
+ *     order of the reference's ttn(iz,ix,ir).
+ *   - Vertex arrays are (nvz+2, nvx+2, nvr+2) with r fastest: the .vtx
+ *     record order.  Frechet columns are the 1-based r-fastest vertex index.
+ *   - Receivers are (nrc, 3) rows of (height km, colatitude rad, longitude
+ *     rad), converted as fm3dt_parallel.f90:6310-6311.
+ *   - Every call returns FMM3D_OK (0) or a negative FMM3D_E_* code and never
+ *     aborts; fmm3d_last_error() gives the message.  A context is not
+ *     re-entrant: use one per host thread.
+ */
+#ifndef FMM3D_H
+#define FMM3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMM3D_OK            0
+#define FMM3D_E_ARG        -1   /* bad argument or inconsistent sizes       */
+#define FMM3D_E_DEVICE     -2   /* HIP runtime / device error               */
+#define FMM3D_E_NOMEM      -3   /* device allocation failed                 */
+#define FMM3D_E_NOCONV     -4   /* solver hit its iteration cap             */
+#define FMM3D_E_STATE      -5   /* call out of order (e.g. no model yet)    */
+#define FMM3D_E_CAPACITY   -6   /* caller's output buffer too small         */
+
+/* solver options */
+#define FMM3D_ORDER_FIRST   0   /* first-order stencils (.in line 9 = 0)    */
+#define FMM3D_ORDER_MIXED   1   /* mixed first/second order (.in line 9 = 1) */
+#define FMM3D_PREC_FP64     0   /* float64 node updates                     */
+#define FMM3D_PREC_FP32     1   /* float32 updates on times offset by the source's boundary minimum */
+
+typedef struct {
+    int nvr, nvx, nvz;          /* vertex counts, border excluded          */
+    double gor, goxd, gozd;     /* vertex origin: height km, lat deg (N edge), lon deg (W edge) */
+    double dvr, dvxd, dvzd;     /* vertex spacing: km, deg, deg            */
+    int nnr, nnx, nnz;          /* propagation nodes                       */
+    double rgor, rgoxd, rgozd;  /* node origin (same units)                */
+    double dnr, dnxd, dnzd;     /* node spacing                            */
+    double earth;               /* earth radius km (.in line 8)            */
+} fmm3d_grid_desc;
+
+typedef struct {
+    int64_t node_updates;       /* node updates in the last solve call     */
+    int32_t sweeps;             /* relaxation sweeps (max over sources)    */
+    int32_t tile_r, tile_x, tile_z;  /* tile shape used                    */
+    double solve_ms;            /* device time of the last solve (events)  */
+    double kernel_ms;           /* device time inside the relaxation kernel */
+    int64_t kernel_launches;    /* relaxation-kernel launches              */
+} fmm3d_stats;
+
+typedef struct fmm3d_ctx fmm3d_ctx;
+
+/* Create a context on HIP device `device` for one grid.  Replaces the
+ * grid read and decomposition of fm3dt_parallel.f90:6103-6171. */
+int fmm3d_create(const fmm3d_grid_desc *grid, int device, fmm3d_ctx **out);
+
+/* Release the context and all device memory it owns. */
+int fmm3d_destroy(fmm3d_ctx *ctx);
+
+/* Message of the last failed call on `ctx` (or of the last failed create
+ * when ctx is NULL).  Valid until the next call on the same context. */
+const char *fmm3d_last_error(const fmm3d_ctx *ctx);
+
+/* Upload a vertex model and run the cubic B-spline gridder on the device
+ * (gridder, fm3dt_parallel.f90:6989-7038).  velv: (nvz+2)(nvx+2)(nvr+2)
+ * float64, r fastest. */
+int fmm3d_set_model_vtx(fmm3d_ctx *ctx, const double *velv);
+
+/* Teleseismic solves, one per source, with bottom-face injection: node
+ * values > 0 are fixed, the rest unknown (start_fmm + local_fmm + the
+ * outer relaxation, fm3dt_parallel.f90:6371-6448, 6894-6961, 78-155).
+ * faces: nsrc records of nnx*nnz float64, phi fastest (itimes.out).
+ * order: FMM3D_ORDER_*; precision: FMM3D_PREC_*.  The nsrc fields stay
+ * resident in the context (replacing those of the previous solve) for the
+ * receiver and Frechet calls.  flags: 0. */
+int fmm3d_solve_tele(fmm3d_ctx *ctx, int nsrc, const double *faces, int order,
+                     int precision, int flags);
+
+/* Receiver times for every resident source (srtimes,
+ * fm3dt_parallel.f90:7040-7248): t_out is nsrc*nrc float32, source-major.
+ * A receiver outside the grid yields NaN. */
+int fmm3d_receivers(fmm3d_ctx *ctx, int nrc, const double *rcv, float *t_out);
+
+/* Rays and Frechet rows for every resident source and receiver
+ * (parallel_rpaths + the writer's row assembly, fm3dt_parallel.f90:
+ * 7250-7777, 6503-6599).  Rows are source-major then receiver order;
+ * row_ptr has nsrc*nrc+1 entries; a receiver outside the grid gets an
+ * empty row.  Entries with |value| >= 1e-6 are kept, columns ascending.
+ * Returns FMM3D_E_CAPACITY (with row_ptr[nsrc*nrc] = entries needed) if
+ * cap is too small. */
+int fmm3d_frechet(fmm3d_ctx *ctx, int nrc, const double *rcv, int64_t *row_ptr,
+                  int32_t *cols, float *vals, int64_t cap);
+
+/* Copy resident field `src` to the host as float64 (nnr, nnx, nnz), phi
+ * fastest (the reference's ttn after the final exchange, F:6439-6448). */
+int fmm3d_get_field(fmm3d_ctx *ctx, int src, double *ttn);
+
+/* Statistics of the last solve. */
+int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMM3D_H */

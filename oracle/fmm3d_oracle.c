@@ -816,7 +816,9 @@ double or_fim_sweep_pred(const or_grid *g, const double *veln, const unsigned ch
                                 1.0 / veln[((size_t)(ir + 1) * MX + (ix + 1)) * MZ + (iz + 1)]);
                             if (T < best) best = T;
                         }
-                double nv = mode == 1 ? (best < INFINITY ? best : tc) : ((best <= tc) ? best : tc);
+                double nv;
+                if (mode == 1 && best == INFINITY && tc < INFINITY) nv = INFINITY;  /* unsupported value: drop */
+                else nv = mode == 1 ? (best < INFINITY ? best : tc) : ((best <= tc) ? best : tc);
                 t_out[k] = nv;
                 if (tc < INFINITY) { double dd = fabs(nv - tc); if (dd > maxd) maxd = dd; }
             }

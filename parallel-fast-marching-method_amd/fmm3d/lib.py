@@ -1,0 +1,171 @@
+"""ctypes binding of libfmm3d_hip.so (the C ABI declared in include/fmm3d.h).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+from .synth import GridDesc
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libfmm3d_hip.so")
+
+OK = 0
+E_ARG, E_DEVICE, E_NOMEM, E_NOCONV, E_STATE, E_CAPACITY = -1, -2, -3, -4, -5, -6
+ORDER_FIRST, ORDER_MIXED = 0, 1
+PREC_FP64, PREC_FP32 = 0, 1
+
+EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model_vtx",
+           "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_frechet", "fmm3d_get_field",
+           "fmm3d_get_stats")
+
+
+class GridDescC(C.Structure):
+    _fields_ = [("nvr", C.c_int), ("nvx", C.c_int), ("nvz", C.c_int),
+                ("gor", C.c_double), ("goxd", C.c_double), ("gozd", C.c_double),
+                ("dvr", C.c_double), ("dvxd", C.c_double), ("dvzd", C.c_double),
+                ("nnr", C.c_int), ("nnx", C.c_int), ("nnz", C.c_int),
+                ("rgor", C.c_double), ("rgoxd", C.c_double), ("rgozd", C.c_double),
+                ("dnr", C.c_double), ("dnxd", C.c_double), ("dnzd", C.c_double),
+                ("earth", C.c_double)]
+
+
+class StatsC(C.Structure):
+    _fields_ = [("node_updates", C.c_int64), ("sweeps", C.c_int32), ("tile_r", C.c_int32),
+                ("tile_x", C.c_int32), ("tile_z", C.c_int32), ("solve_ms", C.c_double),
+                ("kernel_ms", C.c_double), ("kernel_launches", C.c_int64)]
+
+
+class FmmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: Optional[str] = None):
+    """Load (once) and prototype the library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError(f"{p} is not built; run fmm3d.build.build_hip() (or __graft_entry__.build())")
+    L = C.CDLL(p)
+    vp = C.c_void_p
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+    lp = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+    L.fmm3d_create.argtypes = [C.POINTER(GridDescC), C.c_int, C.POINTER(vp)]
+    L.fmm3d_destroy.argtypes = [vp]
+    L.fmm3d_last_error.argtypes = [vp]
+    L.fmm3d_last_error.restype = C.c_char_p
+    L.fmm3d_set_model_vtx.argtypes = [vp, dp]
+    L.fmm3d_solve_tele.argtypes = [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int]
+    L.fmm3d_receivers.argtypes = [vp, C.c_int, dp, fp]
+    L.fmm3d_frechet.argtypes = [vp, C.c_int, dp, lp, ip, fp, C.c_int64]
+    L.fmm3d_get_field.argtypes = [vp, C.c_int, dp]
+    L.fmm3d_get_stats.argtypes = [vp, C.POINTER(StatsC)]
+    for f in EXPORTS:
+        if f not in ("fmm3d_last_error",):
+            getattr(L, f).restype = C.c_int
+    _lib = L
+    return L
+
+
+def grid_c(g: GridDesc) -> GridDescC:
+    return GridDescC(g.nvr, g.nvx, g.nvz, g.gor, g.goxd, g.gozd, g.dvr, g.dvxd, g.dvzd,
+                     g.nnr, g.nnx, g.nnz, g.rgor, g.rgoxd, g.rgozd, g.dnr, g.dnxd, g.dnzd, g.earth)
+
+
+class Context:
+    """One fmm3d_ctx: a grid, a velocity model and the resident fields of the
+    last solve on one device.  Mirrors the stages of PROGRAM fmmin3d
+    (fm3dt_parallel.f90:6015-6662): gridder, per-source solve, srtimes,
+    rays/Frechet rows."""
+
+    def __init__(self, g: GridDesc, device: int = 0):
+        self.L = load()
+        self.g = g
+        self._h = C.c_void_p()
+        rc = self.L.fmm3d_create(C.byref(grid_c(g)), device, C.byref(self._h))
+        if rc != OK:
+            raise FmmError(rc, self.L.fmm3d_last_error(None).decode())
+        self.nsrc = 0
+
+    def _check(self, rc: int):
+        if rc != OK:
+            raise FmmError(rc, self.L.fmm3d_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self.L.fmm3d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_model(self, velv: np.ndarray) -> None:
+        self._check(self.L.fmm3d_set_model_vtx(self._h, np.ascontiguousarray(velv, dtype=np.float64)))
+
+    def solve_tele(self, faces: np.ndarray, order: int = ORDER_MIXED, precision: int = PREC_FP64) -> None:
+        faces = np.ascontiguousarray(faces, dtype=np.float64)
+        if faces.ndim == 2:
+            faces = faces[None]
+        if faces.shape[1:] != (self.g.nnx, self.g.nnz):
+            raise ValueError(f"faces must be (nsrc, {self.g.nnx}, {self.g.nnz}), got {faces.shape}")
+        self._check(self.L.fmm3d_solve_tele(self._h, faces.shape[0], faces, order, precision, 0))
+        self.nsrc = faces.shape[0]
+
+    def field(self, src: int) -> np.ndarray:
+        out = np.zeros((self.g.nnr, self.g.nnx, self.g.nnz))
+        self._check(self.L.fmm3d_get_field(self._h, src, out))
+        return out
+
+    def receivers(self, rcv: np.ndarray) -> np.ndarray:
+        rcv = np.ascontiguousarray(rcv, dtype=np.float64)
+        out = np.zeros(self.nsrc * len(rcv), dtype=np.float32)
+        self._check(self.L.fmm3d_receivers(self._h, len(rcv), rcv, out))
+        return out.reshape(self.nsrc, len(rcv))
+
+    def frechet(self, rcv: np.ndarray, cap: int = 0) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(row_ptr int64 [nsrc*nrc+1], cols int32, vals float32)."""
+        rcv = np.ascontiguousarray(rcv, dtype=np.float64)
+        nrows = self.nsrc * len(rcv)
+        if cap <= 0:
+            cap = max(1024, nrows * 512)
+        for _ in range(2):
+            rp = np.zeros(nrows + 1, dtype=np.int64)
+            cols = np.zeros(cap, dtype=np.int32)
+            vals = np.zeros(cap, dtype=np.float32)
+            rc = self.L.fmm3d_frechet(self._h, len(rcv), rcv, rp, cols, vals, cap)
+            if rc == E_CAPACITY:
+                cap = int(rp[-1])
+                continue
+            self._check(rc)
+            n = int(rp[-1])
+            return rp, cols[:n].copy(), vals[:n].copy()
+        raise FmmError(E_CAPACITY, "frechet capacity retry failed")
+
+    def stats(self) -> dict:
+        s = StatsC()
+        self._check(self.L.fmm3d_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in StatsC._fields_}

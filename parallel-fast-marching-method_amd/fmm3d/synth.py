@@ -305,6 +305,13 @@ def receivers(g: GridDesc, n: int, seed: int, max_per_quadrant: Optional[int] = 
     return np.stack([np.zeros(n), lat, lon], axis=1)
 
 
+def receivers_rad(rcv_deg: np.ndarray) -> np.ndarray:
+    """(height km, lat deg, lon deg) -> (height km, colat rad, lon rad), the
+    reference's conversion (fm3dt_parallel.f90:6310-6311)."""
+    r = np.asarray(rcv_deg, dtype=np.float64)
+    return np.stack([r[:, 0], (90.0 - r[:, 1]) * PI_REF / 180.0, r[:, 2] * PI_REF / 180.0], axis=1)
+
+
 # ---------------------------------------------------------------- file contract
 
 def write_vtx(path: str, g: GridDesc, velv: np.ndarray) -> None:
