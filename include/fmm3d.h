@@ -65,6 +65,8 @@ typedef struct {
     double solve_ms;            /* device time of the last solve (events)  */
     double kernel_ms;           /* device time inside the relaxation kernel */
     int64_t kernel_launches;    /* relaxation-kernel launches              */
+    int32_t phase_sweeps[6];    /* sweeps per phase: A, B, C, C, C, -      */
+    int64_t brick_visits;       /* bricks relaxed (active bricks summed over sweeps) */
 } fmm3d_stats;
 
 typedef struct fmm3d_ctx fmm3d_ctx;

@@ -500,7 +500,8 @@ static void face_update(dom_t *D, int k, int ir, int ix, int iz)
             double v1 = D->t[DIDX(D, p1[0], p1[1], p1[2])];
             if (!((v1 < tc || tc == 0.0) && v1 != 0.0)) continue;
             ord[a][sd] = 1; t1[a][sd] = v1;
-            if (D->fom && p2[a] >= 1 && p2[a] <= D->n[a]) {
+            /* the face updates are mixed order whatever .in line 9 says (F:197-383) */
+            if (p2[a] >= 1 && p2[a] <= D->n[a]) {
                 double v2 = D->t[DIDX(D, p2[0], p2[1], p2[2])];
                 if (v2 != 0.0 && v1 > v2) { ord[a][sd] = 2; t2[a][sd] = v2; }
             }
@@ -826,6 +827,43 @@ double or_fim_sweep_pred(const or_grid *g, const double *veln, const unsigned ch
 #undef TP
     free(sth); free(ri);
     return maxd;
+}
+
+/* Teleseismic solve by relaxation to the fixed point, the three phases the
+ * HIP solver runs: first order with the value-based predicate until no
+ * value moves; then mixed order with the stencil choice frozen from the
+ * previous field, twice (or until a pass moves nothing).  In-place sweeps,
+ * exact convergence.  With a fully known bottom face this equals
+ * or_fmm_tele bit for bit; where the face has unknown nodes it can be
+ * earlier, because the heap march reaches such nodes only from above (the
+ * face nodes other than the seed never leave the alive set through the
+ * heap, F:6930-6958).  Returns the total number of sweeps or -1. */
+long or_relax_tele(const or_grid *g, const double *veln, const double *face, int fom, double *ttn)
+{
+    const size_t NL = (size_t)g->nnx * g->nnz, N = NL * g->nnr;
+    unsigned char *fixed = calloc(N, 1);
+    double *P = malloc(N * sizeof(double)), *prev = malloc(N * sizeof(double));
+    for (size_t i = 0; i < N; ++i) ttn[i] = INFINITY;
+    for (size_t f = 0; f < NL; ++f)
+        if (face[f] > 0.0) { ttn[(size_t)(g->nnr - 1) * NL + f] = face[f]; fixed[(size_t)(g->nnr - 1) * NL + f] = 1; }
+    long sweeps = 0;
+    int phases = fom ? 4 : 1;
+    for (int ph = 0; ph < phases; ++ph) {
+        memcpy(P, ttn, N * sizeof(double));
+        int moved_any = 0;
+        for (int it = 0; it < 100000; ++it) {
+            memcpy(prev, ttn, N * sizeof(double));
+            or_fim_sweep_pred(g, veln, fixed, ph == 0 ? 0 : fom, ttn, ph == 0 ? ttn : P, ttn, 1);
+            sweeps++;
+            if (!memcmp(prev, ttn, N * sizeof(double))) break;
+            moved_any = 1;
+            if (it == 99999) { sweeps = -1; goto out; }
+        }
+        if (ph >= 2 && !moved_any) break;
+    }
+out:
+    free(fixed); free(P); free(prev);
+    return sweeps;
 }
 
 /* ------------------------------------------------------------- receivers */

@@ -18,9 +18,11 @@
 // iteration k iff it or a face neighbour changed in iteration k-1, and a
 // phase ends at the first iteration with no change.
 //
-// Layouts on the device: fields and slowness are r-fastest,
-// idx = (iz*nnx + ix)*nnr + ir (0-based); the node velocity used by rays
-// keeps the reference gridder's halo (-1..n+1 per axis), r-fastest.
+// Layouts on the device: fields and slowness are phi-fastest, one radial
+// layer after another, idx = (ir*nnx + ix)*nnz + iz (0-based) -- the
+// reference's ttn(iz,ix,ir) order -- so a layer is contiguous and lanes map
+// to phi; the node velocity used by rays keeps the reference gridder's halo
+// (-1..n+1 per axis) in the same order.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -63,9 +65,9 @@ __global__ void k_gridder(Grid g, const double* __restrict__ velv, double* __res
     const int MR = g.nnr + 3, MX = g.nnx + 3, MZ = g.nnz + 3;
     const long total = (long)MR * MX * MZ;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
-        int kr = (int)(i % MR) - 1;
-        int kx = (int)((i / MR) % MX) - 1;
-        int kz = (int)(i / ((long)MR * MX)) - 1;
+        int kz = (int)(i % MZ) - 1;
+        int kx = (int)((i / MZ) % MX) - 1;
+        int kr = (int)(i / ((long)MZ * MX)) - 1;
         const int NR = g.nvr + 2, NX = g.nvx + 2, NZ = g.nvz + 2;
         double h = g.rgor - (kr - 1) * g.dnr;
         int ri = (int)floor((g.gor - g.rgor + (kr - 1) * g.dnr) / g.dvr) + 1;
@@ -92,7 +94,7 @@ __global__ void k_gridder(Grid g, const double* __restrict__ velv, double* __res
         }
         veln[i] = acc_z;
         if (kr >= 1 && kr <= g.nnr && kx >= 1 && kx <= g.nnx && kz >= 1 && kz <= g.nnz) {
-            size_t s = ((size_t)(kz - 1) * g.nnx + (kx - 1)) * g.nnr + (kr - 1);
+            size_t s = ((size_t)(kr - 1) * g.nnx + (kx - 1)) * g.nnz + (kz - 1);
             double sl = 1.0 / acc_z;
             slo64[s] = sl;
             slo32[s] = (float)sl;
@@ -107,119 +109,226 @@ template <typename R> struct SweepArgs {
     const R* P;             // frozen ordering fields (phases B/C)
     const R* slo;           // slowness, N
     const R* hx;            // per ir: ri*dnx
-    const R* hz;            // per (ix, ir): ri*sin(theta)*dnz, index ix*nnr + ir
+    const R* hz;            // per (ir, ix): ri*sin(theta)*dnz, index ir*nnx + ix
     R hr;                   // dnr
     const uint32_t* fixbits;// per slot: N bits, fixed (Dirichlet) nodes
-    int* stamp;             // per slot per tile: last iteration with a change
-    int* lastchg;           // per slot: last iteration with a change
+    int* stamp;             // per slot per brick: last sweep with a change
+    int* lastchg;           // per slot: last sweep with a change
     const int* slot_on;     // per slot: 1 while iterating
     int nnr, nnx, nnz;
-    int ntr, ntx, ntz;      // tiles per axis
+    int nbr, nbx, nbz;      // bricks per axis
+    int sl;                 // layers per brick
+    int kin;                // lateral relaxations per layer
     int nslots;
-    int iter;               // iteration number k (>= 1)
+    int iter;               // sweep number k (>= 1)
+    int dir;                // +1: layers bottom to top, -1: top to bottom
     int phase;              // 0: A (first order, value-based); 1: B/C (frozen ordering)
     int fom;                // second order allowed in phase 1
-    int64_t* updates;       // node updates counter (per launch accumulation)
+    unsigned long long* visits;  // active bricks counter
+    double tol;             // relative change below which a brick counts as settled
+    int* valchg;            // per slot, 1 if a sweep changed a value by more than tol
+    int dirmode;            // 0: sweep direction alternates; 1: per brick, along increasing time
 };
 
+// One node update from gathered stencil values.  Sides are ordered
+// (r-, r+, x-, x+, z-, z+); in1/in2 say whether the first/second neighbour
+// exists.  Phase 0 counts a neighbour iff its value is below the node's
+// (float32: not above, so a value rounded onto its neighbour keeps its
+// support); phase 1 counts by the frozen ordering field and goes second
+// order iff the second neighbour is ordered before the first.
 template <typename R, bool SHIFT>
-__device__ __forceinline__ R solve_node(const SweepArgs<R>& a, const R* __restrict__ T, const R* __restrict__ P,
-                                        int ir, int ix, int iz, size_t c, R tc)
+__device__ __forceinline__ R node_update(int phase, int fom, R tc, R pc, const R v1[6], const R v2[6],
+                                         const R q1[6], const R q2[6], const bool in1[6], const bool in2[6],
+                                         R hr, R hx, R hz, R slo)
 {
-    const int n[3] = { a.nnr, a.nnx, a.nnz };
-    const int pos[3] = { ir, ix, iz };
-    const size_t stride[3] = { 1, (size_t)a.nnr, (size_t)a.nnr * a.nnx };
-    Side<R> s[3][2];
     const R inf = Lim<R>::inf();
-    const R pc = a.phase ? P[c] : R(0);
+    Side<R> sd[3][2];
     R tb = inf;
 #pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-#pragma unroll
-        for (int sd = 0; sd < 2; ++sd) {
-            const int d = sd ? 1 : -1;
-            Side<R>& S = s[ax][sd];
-            S.ord = 0; S.t1 = R(0); S.t2 = R(0);
-            const int p1 = pos[ax] + d;
-            if (p1 < 0 || p1 >= n[ax]) { S.ord = -1; continue; }
-            const size_t i1 = d > 0 ? c + stride[ax] : c - stride[ax];
-            const R v1 = T[i1];
-            if (a.phase == 0) {
-                if (!(v1 < tc)) continue;
-                S.ord = 1; S.t1 = v1;
-            } else {
-                const R q1 = P[i1];
-                if (!(q1 < pc) || !(v1 < inf)) continue;
-                S.ord = 1; S.t1 = v1;
-                const int p2 = pos[ax] + 2 * d;
-                if (a.fom && p2 >= 0 && p2 < n[ax]) {
-                    const size_t i2 = d > 0 ? i1 + stride[ax] : i1 - stride[ax];
-                    const R q2 = P[i2], v2 = T[i2];
-                    if (q2 < inf && v2 < inf && q1 > q2) { S.ord = 2; S.t2 = v2; }
-                }
-            }
-            if (SHIFT && S.t1 < tb) tb = S.t1;
+    for (int q = 0; q < 6; ++q) {
+        Side<R>& S = sd[q / 2][q % 2];
+        S.ord = 0; S.t1 = R(0); S.t2 = R(0);
+        if (!in1[q]) { S.ord = -1; continue; }
+        if (phase == 0) {
+            const bool cnt = SHIFT ? (v1[q] <= tc && v1[q] < inf) : (v1[q] < tc);
+            if (!cnt) continue;
+            S.ord = 1; S.t1 = v1[q];
+        } else {
+            if (!(q1[q] < pc) || !(v1[q] < inf)) continue;
+            S.ord = 1; S.t1 = v1[q];
+            if (fom && in2[q] && q2[q] < inf && v2[q] < inf && q1[q] > q2[q]) { S.ord = 2; S.t2 = v2[q]; }
         }
+        if (SHIFT && S.t1 < tb) tb = S.t1;
     }
     if (SHIFT) {
         if (!(tb < inf)) return inf;
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax)
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int sd = 0; sd < 2; ++sd) { s[ax][sd].t1 -= tb; s[ax][sd].t2 -= tb; }
+            for (int b = 0; b < 2; ++b) { sd[a][b].t1 -= tb; sd[a][b].t2 -= tb; }
     }
-    const R hx = a.hx[ir];
-    const R hz = a.hz[(size_t)ix * a.nnr + ir];
-    R best = octant_min<R>(s, a.hr, hx, hz, a.slo[c]);
+    R best = octant_min<R>(sd, hr, hx, hz, slo);
     if (SHIFT && best < inf) best = tb + best;
     return best;
 }
 
-template <typename R, int TR, int TX, int TZ, bool SHIFT>
-__global__ void __launch_bounds__(TR * TX * TZ)
-k_sweep(SweepArgs<R> a)
+// Brick sweep.  A block owns a CX x CZ column patch of one brick of `sl`
+// layers and walks the layers in the sweep direction; each layer is staged
+// in LDS with a two-node lateral halo and relaxed `kin` times, the radial
+// neighbours come from HBM (the layer just finished by this block is
+// visible after the barrier).  Updates are in place; a brick runs in sweep
+// k iff it or a face neighbour changed in sweep k-1.
+template <typename R, int CX, int CZ, bool SHIFT>
+__global__ void __launch_bounds__(CX * CZ)
+k_brick(SweepArgs<R> a)
 {
-    const int ntiles = a.ntr * a.ntx * a.ntz;
-    const int slot = blockIdx.x / ntiles;
-    const int tile = blockIdx.x - slot * ntiles;
-    if (slot >= a.nslots || !a.slot_on[slot]) return;
-    const int tr = tile % a.ntr, tx = (tile / a.ntr) % a.ntx, tz = tile / (a.ntr * a.ntx);
-    int* st = a.stamp + (size_t)slot * ntiles;
-    // activity: this tile or a face neighbour changed in the previous iteration
+    constexpr int LX = CX + 4, LZ = CZ + 4, NT = CX * CZ;
+    __shared__ R cur[LX * LZ];
+    __shared__ R pcu[LX * LZ];
     __shared__ int s_act, s_chg;
+    const int nb = a.nbr * a.nbx * a.nbz;
+    const int slot = blockIdx.x / nb;
+    const int b = blockIdx.x - slot * nb;
+    if (slot >= a.nslots || !a.slot_on[slot]) return;
+    const int br = b / (a.nbx * a.nbz), bx = (b / a.nbz) % a.nbx, bz = b % a.nbz;
+    int* st = a.stamp + (size_t)slot * nb;
     if (threadIdx.x == 0) {
-        int m = st[tile];
-        if (tr > 0) m = max(m, st[tile - 1]);
-        if (tr < a.ntr - 1) m = max(m, st[tile + 1]);
-        if (tx > 0) m = max(m, st[tile - a.ntr]);
-        if (tx < a.ntx - 1) m = max(m, st[tile + a.ntr]);
-        if (tz > 0) m = max(m, st[tile - a.ntr * a.ntx]);
-        if (tz < a.ntz - 1) m = max(m, st[tile + a.ntr * a.ntx]);
+        int m = st[b];
+        if (br > 0) m = max(m, st[b - a.nbx * a.nbz]);
+        if (br < a.nbr - 1) m = max(m, st[b + a.nbx * a.nbz]);
+        if (bx > 0) m = max(m, st[b - a.nbz]);
+        if (bx < a.nbx - 1) m = max(m, st[b + a.nbz]);
+        if (bz > 0) m = max(m, st[b - 1]);
+        if (bz < a.nbz - 1) m = max(m, st[b + 1]);
         s_act = (m >= a.iter - 1);
         s_chg = 0;
     }
     __syncthreads();
     if (!s_act) return;
-    const size_t N = (size_t)a.nnr * a.nnx * a.nnz;
+    if (threadIdx.x == 0) atomicAdd(a.visits, 1ull);
+    const size_t NL = (size_t)a.nnx * a.nnz;
+    const size_t N = NL * a.nnr;
     R* T = a.T + (size_t)slot * N;
     const R* P = a.phase ? a.P + (size_t)slot * N : nullptr;
     const uint32_t* fb = a.fixbits + (size_t)slot * ((N + 31) / 32);
-    const int lr = threadIdx.x % TR, lx = (threadIdx.x / TR) % TX, lz = threadIdx.x / (TR * TX);
-    const int ir = tr * TR + lr, ix = tx * TX + lx, iz = tz * TZ + lz;
-    int changed = 0;
-    if (ir < a.nnr && ix < a.nnx && iz < a.nnz) {
-        const size_t c = ((size_t)iz * a.nnx + ix) * a.nnr + ir;
-        if (!((fb[c >> 5] >> (c & 31)) & 1u)) {
-            const R tc = T[c];
-            const R nv = solve_node<R, SHIFT>(a, T, P, ir, ix, iz, c, tc);
-            if (nv != tc) { T[c] = nv; changed = 1; }
+    const R inf = Lim<R>::inf();
+    const int lx = threadIdx.x / CZ, lz = threadIdx.x % CZ;
+    const int ix = bx * CX + lx, iz = bz * CZ + lz;
+    const bool valid = ix < a.nnx && iz < a.nnz;
+    const int own = (lx + 2) * LZ + (lz + 2);
+    const int r_lo = br * a.sl, r_hi = min(r_lo + a.sl, a.nnr) - 1;
+    const int nl = r_hi - r_lo + 1;
+    // Sweep direction of this brick: along increasing time, judged on the
+    // patch's middle column (the ordering field in phases B/C, the current
+    // field in phase A); the sweep's alternating default where unknown.
+    // Relaxing second-order updates against the dependency direction only
+    // feeds them stale inputs, which they amplify.
+    __shared__ int s_dir;
+    if (threadIdx.x == 0) {
+        const int mx = min(bx * CX + CX / 2, a.nnx - 1), mz = min(bz * CZ + CZ / 2, a.nnz - 1);
+        const R* D = a.phase ? P : T;
+        const R top = D[(size_t)r_lo * NL + (size_t)mx * a.nnz + mz];
+        const R bot = D[(size_t)r_hi * NL + (size_t)mx * a.nnz + mz];
+        int d = a.dir;
+        if (a.dirmode && top < inf && bot < inf && top != bot) d = top > bot ? 1 : -1;
+        s_dir = d;
+    }
+    __syncthreads();
+    const int dir = s_dir;
+    // radial side behind the sweep (already relaxed in this sweep) and ahead of it
+    const int qb = dir > 0 ? 1 : 0, qa = 1 - qb;
+    const long sb = dir > 0 ? (long)NL : -(long)NL;      // offset of the node behind
+    // the two nodes behind the first layer belong to the neighbouring brick
+    R beh1 = inf, beh2 = inf;
+    {
+        const int ir0 = dir > 0 ? r_hi : r_lo;
+        const int i1 = ir0 + (dir > 0 ? 1 : -1), i2 = ir0 + (dir > 0 ? 2 : -2);
+        if (valid) {
+            if (i1 >= 0 && i1 < a.nnr) beh1 = T[(size_t)i1 * NL + (size_t)ix * a.nnz + iz];
+            if (i2 >= 0 && i2 < a.nnr) beh2 = T[(size_t)i2 * NL + (size_t)ix * a.nnz + iz];
         }
     }
-    if (changed) s_chg = 1;   // benign race: all writers store 1
+    bool any = false;
+    for (int qq = 0; qq < nl; ++qq) {
+        const int ir = dir > 0 ? r_hi - qq : r_lo + qq;
+        const size_t lay = (size_t)ir * NL;
+        for (int e = threadIdx.x; e < LX * LZ; e += NT) {
+            const int hx_ = bx * CX + e / LZ - 2, hz_ = bz * CZ + e % LZ - 2;
+            const bool in = hx_ >= 0 && hx_ < a.nnx && hz_ >= 0 && hz_ < a.nnz;
+            const size_t gi = lay + (size_t)hx_ * a.nnz + hz_;
+            cur[e] = in ? T[gi] : inf;
+            if (a.phase) pcu[e] = in ? P[gi] : inf;
+        }
+        __syncthreads();
+        const size_t c = lay + (size_t)ix * a.nnz + iz;
+        const bool upd = valid && !((fb[c >> 5] >> (c & 31)) & 1u);
+        R v1[6], v2[6], q1[6], q2[6];
+        bool in1[6], in2[6];
+        R slo = R(0), hx = R(0), hz = R(0), pc = R(0);
+        if (upd) {
+            slo = a.slo[c];
+            hx = a.hx[ir];
+            hz = a.hz[(size_t)ir * a.nnx + ix];
+            pc = a.phase ? pcu[own] : R(0);
+            in1[0] = ir - 1 >= 0; in2[0] = ir - 2 >= 0;
+            in1[1] = ir + 1 < a.nnr; in2[1] = ir + 2 < a.nnr;
+            // behind: this thread's own column, relaxed in the previous steps
+            v1[qb] = in1[qb] ? beh1 : inf;
+            v2[qb] = in2[qb] ? beh2 : inf;
+            // ahead: not yet visited in this sweep
+            v1[qa] = in1[qa] ? T[c - sb] : inf;
+            v2[qa] = in2[qa] ? T[c - 2 * sb] : inf;
+            if (a.phase) {
+                q1[0] = in1[0] ? P[c - NL] : inf; q2[0] = in2[0] ? P[c - 2 * NL] : inf;
+                q1[1] = in1[1] ? P[c + NL] : inf; q2[1] = in2[1] ? P[c + 2 * NL] : inf;
+            }
+            in1[2] = ix - 1 >= 0; in2[2] = ix - 2 >= 0;
+            in1[3] = ix + 1 < a.nnx; in2[3] = ix + 2 < a.nnx;
+            in1[4] = iz - 1 >= 0; in2[4] = iz - 2 >= 0;
+            in1[5] = iz + 1 < a.nnz; in2[5] = iz + 2 < a.nnz;
+        }
+        for (int k = 0; k < a.kin; ++k) {
+            R nv = R(0), tc = R(0);
+            bool moved = false;
+            if (upd) {
+                tc = cur[own];
+                v1[2] = cur[own - LZ]; v2[2] = cur[own - 2 * LZ];
+                v1[3] = cur[own + LZ]; v2[3] = cur[own + 2 * LZ];
+                v1[4] = cur[own - 1]; v2[4] = cur[own - 2];
+                v1[5] = cur[own + 1]; v2[5] = cur[own + 2];
+                if (a.phase) {
+                    q1[2] = pcu[own - LZ]; q2[2] = pcu[own - 2 * LZ];
+                    q1[3] = pcu[own + LZ]; q2[3] = pcu[own + 2 * LZ];
+                    q1[4] = pcu[own - 1]; q2[4] = pcu[own - 2];
+                    q1[5] = pcu[own + 1]; q2[5] = pcu[own + 2];
+                }
+                nv = node_update<R, SHIFT>(a.phase, a.fom, tc, pc, v1, v2, q1, q2, in1, in2, a.hr, hx, hz, slo);
+            }
+            __syncthreads();
+            if (upd && nv != tc) {
+                cur[own] = nv;
+                T[c] = nv;
+                // a change of at most `tol` (relative to the larger value) does
+                // not wake the neighbourhood; a node gaining or losing a value does
+                const bool fin_n = nv < inf, fin_t = tc < inf;
+                if (fin_n != fin_t || !(fabs((double)nv - (double)tc) <= a.tol * fmax(fabs((double)nv), fabs((double)tc)))) {
+                    any = true;
+                    moved = true;
+                }
+            }
+            // relax the layer again only while something in the patch still moves
+            if (!__syncthreads_or(moved)) break;
+        }
+        __syncthreads();
+        if (valid) { beh2 = beh1; beh1 = cur[own]; }
+        __syncthreads();
+    }
+    if (any) s_chg = 1;
     __syncthreads();
     if (threadIdx.x == 0 && s_chg) {
-        st[tile] = a.iter;
+        st[b] = a.iter;
         atomicMax(a.lastchg + slot, a.iter);
+        atomicOr(a.valchg + slot, 1);
     }
 }
 
@@ -260,7 +369,7 @@ __global__ void k_receivers(Grid g, const R* __restrict__ T, const double* __res
             for (int l = 0; l < 2; ++l) {
                 double w = (1.0 - fabs((l * g.dnz - drz) / g.dnz)) * (1.0 - fabs((k * g.dnx - drx) / g.dnx));
                 w = w * (1.0 - fabs((j * g.dnr - drr) / g.dnr));
-                size_t idx = ((size_t)(irz - 1 + l) * g.nnx + (irx - 1 + k)) * g.nnr + (irr - 1 + j);
+                size_t idx = ((size_t)(irr - 1 + j) * g.nnx + (irx - 1 + k)) * g.nnz + (irz - 1 + l);
                 acc = (float)((double)acc + field_at<R>(F, to, idx) * w);
             }
     out[gid] = acc;
@@ -271,7 +380,7 @@ __global__ void k_receivers(Grid g, const R* __restrict__ T, const double* __res
 __device__ __forceinline__ double tri_vel(const Grid& g, const double* __restrict__ veln, int ipr, int ipx, int ipz,
                                           double drr, double drx, double drz)
 {
-    const int MR = g.nnr + 3, MX = g.nnx + 3;
+    const int MX = g.nnx + 3, MZ = g.nnz + 3;
     double vel = 0.0;
     for (int k = 0; k < 2; ++k)
         for (int l = 0; l < 2; ++l)
@@ -281,7 +390,7 @@ __device__ __forceinline__ double tri_vel(const Grid& g, const double* __restric
                 w = w * (1.0 - fabs((k * g.dnr - drr) / g.dnr));
                 int zr = ipr + k, zx = ipx + l, zz = ipz + m;
                 if (zz <= g.nnz + 1 && zx <= g.nnx + 1 && zr <= g.nnr + 1 && zr >= -1 && zx >= -1 && zz >= -1)
-                    vel += veln[((size_t)(zz + 1) * MX + (zx + 1)) * MR + (zr + 1)] * w;
+                    vel += veln[((size_t)(zr + 1) * MX + (zx + 1)) * MZ + (zz + 1)] * w;
             }
     return vel;
 }
@@ -316,7 +425,7 @@ k_rays(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const d
         const R* F = T + (size_t)s * N;
         const double to = toff[s];
         auto TT = [&](int r, int x, int z) -> double {   // 1-based node access
-            return field_at<R>(F, to, ((size_t)(z - 1) * g.nnx + (x - 1)) * g.nnr + (r - 1));
+            return field_at<R>(F, to, ((size_t)(r - 1) * g.nnx + (x - 1)) * g.nnz + (z - 1));
         };
         const int64_t orow = (int64_t)(rid - ray0) * out.cap;
         double r0 = rcv[3 * i], x0 = rcv[3 * i + 1], z0 = rcv[3 * i + 2];
@@ -532,6 +641,7 @@ struct Slots {
     int* lastchg = nullptr;
     int* on = nullptr;
     double* toff = nullptr;
+    int* valchg = nullptr;    // a sweep changed a value
 };
 
 }  // namespace
@@ -552,9 +662,15 @@ struct fmm3d_ctx {
     float* hz32 = nullptr;
     bool have_model = false;
     Slots sl;
-    int tile_r = 16, tile_x = 4, tile_z = 4;
+    int tile_x = 8, tile_z = 32;   // lateral patch of a brick (theta x phi)
+    int brick_sl = 8;              // layers per brick
+    int kin = 2;                   // lateral relaxations per layer (at most; stops when settled)
+    int dirmode = 0;               // sweep direction policy
+    double tol64 = 1e-14;          // settle tolerance (relative), float64 fields
+    double tol32 = 1e-6;           // settle tolerance (relative), float32 fields (~8 ulp)
     fmm3d_stats stats{};
     int64_t* d_updates = nullptr;
+    unsigned long long* d_visits = nullptr;
     float* ray_scratch = nullptr;
     size_t ray_scratch_workers = 0;
 };
@@ -642,6 +758,7 @@ extern "C" int fmm3d_create(const fmm3d_grid_desc* d, int device, fmm3d_ctx** ou
     CREATECHK(dalloc(&c->slo64, c->N * sizeof(double)));
     CREATECHK(dalloc(&c->slo32, c->N * sizeof(float)));
     CREATECHK(dalloc(&c->d_updates, sizeof(int64_t)));
+    CREATECHK(dalloc(&c->d_visits, 2 * sizeof(unsigned long long)));
     // metric tables in the reference's arithmetic (F:541-542)
     std::vector<double> hx(g.nnr), hz((size_t)g.nnx * g.nnr);
     std::vector<float> hx32(g.nnr), hz32((size_t)g.nnx * g.nnr);
@@ -651,8 +768,8 @@ extern "C" int fmm3d_create(const fmm3d_grid_desc* d, int device, fmm3d_ctx** ou
         hx32[ir] = (float)hx[ir];
         for (int ix = 0; ix < g.nnx; ++ix) {
             double sth = sin(g.rgox + (ix) * g.dnx);
-            hz[(size_t)ix * g.nnr + ir] = ri * sth * g.dnz;
-            hz32[(size_t)ix * g.nnr + ir] = (float)hz[(size_t)ix * g.nnr + ir];
+            hz[(size_t)ir * g.nnx + ix] = ri * sth * g.dnz;
+            hz32[(size_t)ir * g.nnx + ix] = (float)hz[(size_t)ir * g.nnx + ix];
         }
     }
     CREATECHK(dalloc(&c->hx64, hx.size() * sizeof(double)));
@@ -664,10 +781,15 @@ extern "C" int fmm3d_create(const fmm3d_grid_desc* d, int device, fmm3d_ctx** ou
     CREATECHK(hipMemcpy(c->hx32, hx32.data(), hx32.size() * sizeof(float), hipMemcpyHostToDevice));
     CREATECHK(hipMemcpy(c->hz32, hz32.data(), hz32.size() * sizeof(float), hipMemcpyHostToDevice));
 #undef CREATECHK
-    const char* ts = getenv("FMM3D_TILE");
+    const char* tl = getenv("FMM3D_TOL");    // "tol64[:tol32]"
+    if (tl) sscanf(tl, "%lf:%lf", &c->tol64, &c->tol32);
+    const char* dm = getenv("FMM3D_DIRMODE");
+    if (dm) c->dirmode = atoi(dm);
+    const char* ts = getenv("FMM3D_TILE");   // "SLxCXxCZ[:K]"
     if (ts) {
-        int a, b, e;
-        if (sscanf(ts, "%dx%dx%d", &a, &b, &e) == 3) { c->tile_r = a; c->tile_x = b; c->tile_z = e; }
+        int a, b, e, k = c->kin;
+        int n = sscanf(ts, "%dx%dx%d:%d", &a, &b, &e, &k);
+        if (n >= 3) { c->brick_sl = a; c->tile_x = b; c->tile_z = e; c->kin = k; }
     }
     *out = c;
     return FMM3D_OK;
@@ -677,6 +799,7 @@ static void free_slots(Slots& s)
 {
     (void)hipFree(s.T); (void)hipFree(s.P); (void)hipFree(s.fix); (void)hipFree(s.stamp);
     (void)hipFree(s.lastchg); (void)hipFree(s.on); (void)hipFree(s.toff);
+    (void)hipFree(s.valchg);
     s = Slots();
 }
 
@@ -688,7 +811,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     free_slots(c->sl);
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
     (void)hipFree(c->hx64); (void)hipFree(c->hz64); (void)hipFree(c->hx32); (void)hipFree(c->hz32);
-    (void)hipFree(c->d_updates); (void)hipFree(c->ray_scratch);
+    (void)hipFree(c->d_updates); (void)hipFree(c->d_visits); (void)hipFree(c->ray_scratch);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -721,6 +844,22 @@ extern "C" int fmm3d_set_model_vtx(fmm3d_ctx* c, const double* velv)
     return FMM3D_OK;
 }
 
+// brick geometry of the current configuration
+struct Bricks {
+    int nbr, nbx, nbz, sl;
+    int n() const { return nbr * nbx * nbz; }
+};
+
+static Bricks bricks_of(const fmm3d_ctx* c)
+{
+    Bricks b;
+    b.sl = c->brick_sl;
+    b.nbr = (c->g.nnr + b.sl - 1) / b.sl;
+    b.nbx = (c->g.nnx + c->tile_x - 1) / c->tile_x;
+    b.nbz = (c->g.nnz + c->tile_z - 1) / c->tile_z;
+    return b;
+}
+
 static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
 {
     Slots& s = c->sl;
@@ -728,16 +867,16 @@ static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
     free_slots(s);
     const size_t N = c->N;
     const size_t es = prec == FMM3D_PREC_FP32 ? sizeof(float) : sizeof(double);
-    // stamps for the largest tile count of any supported shape (4x4x4 is the smallest tile)
-    const size_t ntiles = (size_t)((c->g.nnr + 3) / 4) * ((c->g.nnx + 3) / 4) * ((c->g.nnz + 3) / 4);
+    // stamps for the largest brick count any supported shape gives (4-layer bricks, 4x8 patches)
+    const size_t nbmax = (size_t)((c->g.nnr + 3) / 4) * ((c->g.nnx + 3) / 4) * ((c->g.nnz + 7) / 8);
     HIPCHK(c, dalloc(&s.T, (size_t)nsrc * N * es));
     HIPCHK(c, dalloc(&s.P, (size_t)nsrc * N * es));
     HIPCHK(c, dalloc(&s.fix, (size_t)nsrc * ((N + 31) / 32) * sizeof(uint32_t)));
-    // tile stamps for the largest tile count any supported shape gives
-    HIPCHK(c, dalloc(&s.stamp, (size_t)nsrc * ntiles * sizeof(int)));
+    HIPCHK(c, dalloc(&s.stamp, (size_t)nsrc * nbmax * sizeof(int)));
     HIPCHK(c, dalloc(&s.lastchg, nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.on, nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.toff, nsrc * sizeof(double)));
+    HIPCHK(c, dalloc(&s.valchg, nsrc * sizeof(int)));
     s.cap = nsrc;
     s.prec = prec;
     return FMM3D_OK;
@@ -745,48 +884,28 @@ static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
 
 // ------------------------------------------------------------ solve driver
 
-__global__ void k_init_tele64(int nnr, int nnx, int nnz, int nslots, const double* faces, double* T,
-                              uint32_t* fix, double* toff)
+// Teleseismic initial state (start_fmm, F:6914-6939): bottom-face nodes with
+// a positive boundary time hold it and are fixed; every other node unknown.
+template <typename R>
+__global__ void k_init_tele(int nnr, int nnx, int nnz, int nslots, const double* __restrict__ faces, R* __restrict__ T,
+                            const double* __restrict__ toff)
 {
-    const size_t N = (size_t)nnr * nnx * nnz;
-    const size_t NW = (N + 31) / 32;
+    const size_t NL = (size_t)nnx * nnz, N = NL * nnr;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N * nslots; i += (size_t)gridDim.x * blockDim.x) {
         size_t s = i / N, c = i % N;
-        int ir = (int)(c % nnr);
-        size_t col = c / nnr;          // iz*nnx + ix
-        int ix = (int)(col % nnx), iz = (int)(col / nnx);
-        double v = __builtin_huge_val();
-        if (ir == nnr - 1) {
-            double f = faces[s * (size_t)nnx * nnz + (size_t)ix * nnz + iz];
-            if (f > 0.0) v = f;
-        }
-        T[i] = v;
-        (void)fix; (void)NW; (void)toff;
-    }
-}
-
-__global__ void k_init_tele32(int nnr, int nnx, int nnz, int nslots, const double* faces, float* T,
-                              const double* toff)
-{
-    const size_t N = (size_t)nnr * nnx * nnz;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N * nslots; i += (size_t)gridDim.x * blockDim.x) {
-        size_t s = i / N, c = i % N;
-        int ir = (int)(c % nnr);
-        size_t col = c / nnr;
-        int ix = (int)(col % nnx), iz = (int)(col / nnx);
-        float v = __builtin_huge_valf();
-        if (ir == nnr - 1) {
-            double f = faces[s * (size_t)nnx * nnz + (size_t)ix * nnz + iz];
-            if (f > 0.0) v = (float)(f - toff[s]);
+        size_t ir = c / NL, f = c % NL;
+        R v = Lim<R>::inf();
+        if (ir == (size_t)nnr - 1) {
+            double fv = faces[s * NL + f];
+            if (fv > 0.0) v = std::is_same<R, double>::value ? (R)fv : (R)(fv - toff[s]);
         }
         T[i] = v;
     }
 }
 
-// fixed-node bits: bottom-face nodes holding a positive boundary time
-__global__ void k_fixbits_tele(int nnr, int nnx, int nnz, int nslots, const double* faces, uint32_t* fix)
+__global__ void k_fixbits_tele(int nnr, int nnx, int nnz, int nslots, const double* __restrict__ faces, uint32_t* __restrict__ fix)
 {
-    const size_t N = (size_t)nnr * nnx * nnz;
+    const size_t NL = (size_t)nnx * nnz, N = NL * nnr;
     const size_t NW = (N + 31) / 32;
     for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < NW * nslots; w += (size_t)gridDim.x * blockDim.x) {
         size_t s = w / NW, w0 = w % NW;
@@ -794,23 +913,21 @@ __global__ void k_fixbits_tele(int nnr, int nnx, int nnz, int nslots, const doub
         for (int b = 0; b < 32; ++b) {
             size_t c = w0 * 32 + b;
             if (c >= N) break;
-            int ir = (int)(c % nnr);
-            if (ir != nnr - 1) continue;
-            size_t col = c / nnr;
-            int ix = (int)(col % nnx), iz = (int)(col / nnx);
-            if (faces[s * (size_t)nnx * nnz + (size_t)ix * nnz + iz] > 0.0) bits |= 1u << b;
+            if (c / NL != (size_t)nnr - 1) continue;
+            if (faces[s * NL + c % NL] > 0.0) bits |= 1u << b;
         }
         fix[w] = bits;
     }
 }
 
-// Phase-start tile stamps: -1 everywhere (phase A: tiles holding fixed
-// nodes 0), or 0 everywhere (phases B/C).
-__global__ void k_stamp_init(int nslots, int ntiles, int ntr, int all, int* stamp)
+// Phase-start brick stamps: phase A marks the bottom bricks (they hold the
+// fixed nodes) as changed at sweep 0 and the rest as never; phases B/C
+// mark every brick.
+__global__ void k_stamp_init(int nslots, int nb, int nbxz, int nbr, int all, int* stamp)
 {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)nslots * ntiles; i += (size_t)gridDim.x * blockDim.x) {
-        int t = (int)(i % ntiles);
-        stamp[i] = all ? 0 : (((t % ntr) == ntr - 1) ? 0 : -1);
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)nslots * nb; i += (size_t)gridDim.x * blockDim.x) {
+        int t = (int)(i % nb);
+        stamp[i] = all ? 0 : ((t / nbxz == nbr - 1) ? 0 : -1);
     }
 }
 
@@ -829,19 +946,21 @@ __global__ void k_copy(const R* __restrict__ a, R* __restrict__ b, size_t n)
 template <typename R, bool SHIFT>
 static hipError_t launch_sweep(fmm3d_ctx* c, const SweepArgs<R>& a, int blocks)
 {
-    const int tr = c->tile_r, tx = c->tile_x, tz = c->tile_z;
-#define FMM3D_TILE_CASE(A, B, C)                                                                 \
-    if (tr == A && tx == B && tz == C) {                                                         \
-        k_sweep<R, A, B, C, SHIFT><<<blocks, A * B * C, 0, c->stream>>>(a);                       \
-        return hipGetLastError();                                                                \
+    const int tx = c->tile_x, tz = c->tile_z;
+#define FMM3D_PATCH_CASE(A, B)                                                   \
+    if (tx == A && tz == B) {                                                    \
+        k_brick<R, A, B, SHIFT><<<blocks, A * B, 0, c->stream>>>(a);              \
+        return hipGetLastError();                                                \
     }
-    FMM3D_TILE_CASE(16, 4, 4)
-    FMM3D_TILE_CASE(8, 8, 4)
-    FMM3D_TILE_CASE(8, 8, 8)
-    FMM3D_TILE_CASE(4, 4, 4)
-    FMM3D_TILE_CASE(8, 4, 4)
-    FMM3D_TILE_CASE(32, 4, 2)
-#undef FMM3D_TILE_CASE
+    FMM3D_PATCH_CASE(8, 32)
+    FMM3D_PATCH_CASE(4, 32)
+    FMM3D_PATCH_CASE(4, 16)
+    FMM3D_PATCH_CASE(8, 16)
+    FMM3D_PATCH_CASE(16, 16)
+    FMM3D_PATCH_CASE(4, 64)
+    FMM3D_PATCH_CASE(2, 32)
+    FMM3D_PATCH_CASE(4, 8)
+#undef FMM3D_PATCH_CASE
     return hipErrorInvalidValue;
 }
 
@@ -850,10 +969,11 @@ static int run_phases(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
 {
     Slots& s = c->sl;
     const Grid& g = c->g;
-    const int ntr = (g.nnr + c->tile_r - 1) / c->tile_r;
-    const int ntx = (g.nnx + c->tile_x - 1) / c->tile_x;
-    const int ntz = (g.nnz + c->tile_z - 1) / c->tile_z;
-    const int ntiles = ntr * ntx * ntz;
+    const Bricks B = bricks_of(c);
+    const int nb = B.n();
+    const size_t nbmax = (size_t)((g.nnr + 3) / 4) * ((g.nnx + 3) / 4) * ((g.nnz + 7) / 8);
+    if (B.sl < 4 || (size_t)nb > nbmax)
+        return fail(c, FMM3D_E_ARG, "solver: brick shape %dx%dx%d not supported on this grid", B.sl, c->tile_x, c->tile_z);
     SweepArgs<R> a;
     a.T = (R*)s.T;
     a.P = (const R*)s.P;
@@ -866,25 +986,33 @@ static int run_phases(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     a.lastchg = s.lastchg;
     a.slot_on = s.on;
     a.nnr = g.nnr; a.nnx = g.nnx; a.nnz = g.nnz;
-    a.ntr = ntr; a.ntx = ntx; a.ntz = ntz;
+    a.nbr = B.nbr; a.nbx = B.nbx; a.nbz = B.nbz;
+    a.sl = B.sl;
+    a.kin = c->kin;
     a.nslots = nsrc;
     a.fom = order == FMM3D_ORDER_MIXED;
-    a.updates = c->d_updates;
-    const int blocks = nsrc * ntiles;
-    const int maxit = 40 * (g.nnr + g.nnx + g.nnz) + 1000;
-    const int CHECK = 16;
+    a.visits = c->d_visits;
+    a.tol = std::is_same<R, double>::value ? c->tol64 : c->tol32;
+    a.valchg = s.valchg;
+    a.dirmode = c->dirmode;
+    const int blocks = nsrc * nb;
+    const int maxit = 4 * (g.nnr + g.nnx + g.nnz) + 200;
+    const int CHECK = 4;
     std::vector<int> lc(nsrc);
     int total_sweeps = 0;
-    int nphases = order == FMM3D_ORDER_MIXED ? 5 : 1;   // A, B, C (+ up to 2 repeats of C)
+    // A, B, C (+ up to 2 repeats of C).  Float32 stops after B: the refreeze
+    // moves values by ~1e-5 s, below the float32 field's own resolution.
+    const int nphases = order != FMM3D_ORDER_MIXED ? 1 : (std::is_same<R, double>::value ? 5 : 2);
     for (int ph = 0; ph < nphases; ++ph) {
         a.phase = ph == 0 ? 0 : 1;
         if (ph >= 1) {
-            // freeze the ordering: P <- T
+            // freeze the ordering: P <- T; only fixed nodes start final
             size_t n = (size_t)nsrc * c->N;
             k_copy<R><<<4096, 256, 0, c->stream>>>((const R*)s.T, (R*)s.P, n);
+            k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.valchg, nsrc, 0);
             HIPCHK(c, hipGetLastError());
         }
-        k_stamp_init<<<std::min(4096, (blocks + 255) / 256), 256, 0, c->stream>>>(nsrc, ntiles, ntr, ph >= 1, s.stamp);
+        k_stamp_init<<<std::min(4096, (blocks + 255) / 256), 256, 0, c->stream>>>(nsrc, nb, B.nbx * B.nbz, B.nbr, ph >= 1, s.stamp);
         k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.lastchg, nsrc, 0);
         k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.on, nsrc, 1);
         HIPCHK(c, hipGetLastError());
@@ -894,6 +1022,7 @@ static int run_phases(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
         while (!done) {
             for (int j = 0; j < CHECK; ++j, ++k) {
                 a.iter = k;
+                a.dir = (k & 1) ? 1 : -1;
                 HIPCHK(c, (launch_sweep<R, SHIFT>(c, a, blocks)));
                 c->stats.kernel_launches++;
             }
@@ -902,12 +1031,20 @@ static int run_phases(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
             int mx = 0;
             for (int q = 0; q < nsrc; ++q) mx = std::max(mx, lc[q]);
             last_k_with_change = mx;
-            if (mx < k - 1) done = true;          // the last iteration changed nothing
+            if (mx < k - 1) done = true;          // the last sweep changed nothing
             if (!done && k > maxit)
                 return fail(c, FMM3D_E_NOCONV, "solver: phase %d did not converge in %d sweeps", ph, maxit);
         }
         total_sweeps += last_k_with_change + 1;
-        if (ph >= 2 && last_k_with_change == 0) break;   // C changed nothing: the ordering is consistent
+        c->stats.phase_sweeps[ph] = last_k_with_change + 1;
+        if (ph >= 1) {
+            std::vector<int> vc(nsrc);
+            HIPCHK(c, hipMemcpyAsync(vc.data(), s.valchg, nsrc * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            int anyv = 0;
+            for (int q = 0; q < nsrc; ++q) anyv |= vc[q];
+            if (ph >= 2 && !anyv) break;          // C changed nothing: the ordering is consistent
+        }
     }
     *sweeps_out = total_sweeps;
     return FMM3D_OK;
@@ -944,14 +1081,15 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     HIPCHK(c, hipMemcpyAsync(dfaces, faces, (size_t)nsrc * F * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.toff, toff.data(), nsrc * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (precision == FMM3D_PREC_FP64)
-        k_init_tele64<<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (double*)s.T, s.fix, s.toff);
+        k_init_tele<double><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (double*)s.T, s.toff);
     else
-        k_init_tele32<<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff);
+        k_init_tele<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff);
     k_fixbits_tele<<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, s.fix);
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) { (void)hipFree(dfaces); return fail(c, FMM3D_E_DEVICE, "init: %s", hipGetErrorString(le)); }
     c->stats = fmm3d_stats{};
-    c->stats.tile_r = c->tile_r; c->stats.tile_x = c->tile_x; c->stats.tile_z = c->tile_z;
+    c->stats.tile_r = c->brick_sl; c->stats.tile_x = c->tile_x; c->stats.tile_z = c->tile_z;
+    HIPCHK(c, hipMemsetAsync(c->d_visits, 0, sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int sweeps = 0;
     if (precision == FMM3D_PREC_FP64) rc = run_phases<double, false>(c, nsrc, order, &sweeps);
@@ -964,6 +1102,9 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
     c->stats.solve_ms = ms;
     c->stats.sweeps = sweeps;
+    unsigned long long vis = 0;
+    HIPCHK(c, hipMemcpy(&vis, c->d_visits, sizeof vis, hipMemcpyDeviceToHost));
+    c->stats.brick_visits = (int64_t)vis;
     s.n = nsrc;
     return FMM3D_OK;
 }
@@ -985,13 +1126,8 @@ extern "C" int fmm3d_get_field(fmm3d_ctx* c, int src, double* ttn)
         HIPCHK(c, hipMemcpy(&to, c->sl.toff + src, sizeof(double), hipMemcpyDeviceToHost));
         for (size_t i = 0; i < N; ++i) d[i] = to + (double)f[i];
     }
-    // device r-fastest -> host phi-fastest; unknown nodes come back as 0 (the reference's marker)
-    for (int iz = 0; iz < g.nnz; ++iz)
-        for (int ix = 0; ix < g.nnx; ++ix)
-            for (int ir = 0; ir < g.nnr; ++ir) {
-                double v = d[((size_t)iz * g.nnx + ix) * g.nnr + ir];
-                ttn[((size_t)ir * g.nnx + ix) * g.nnz + iz] = isfinite(v) ? v : 0.0;
-            }
+    // same phi-fastest order as the host; unknown nodes come back as 0 (the reference's marker)
+    for (size_t i = 0; i < N; ++i) ttn[i] = isfinite(d[i]) ? d[i] : 0.0;
     return FMM3D_OK;
 }
 

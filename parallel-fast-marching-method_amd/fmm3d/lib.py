@@ -39,7 +39,8 @@ class GridDescC(C.Structure):
 class StatsC(C.Structure):
     _fields_ = [("node_updates", C.c_int64), ("sweeps", C.c_int32), ("tile_r", C.c_int32),
                 ("tile_x", C.c_int32), ("tile_z", C.c_int32), ("solve_ms", C.c_double),
-                ("kernel_ms", C.c_double), ("kernel_launches", C.c_int64)]
+                ("kernel_ms", C.c_double), ("kernel_launches", C.c_int64),
+                ("phase_sweeps", C.c_int32 * 6), ("brick_visits", C.c_int64)]
 
 
 class FmmError(RuntimeError):
@@ -168,4 +169,6 @@ class Context:
     def stats(self) -> dict:
         s = StatsC()
         self._check(self.L.fmm3d_get_stats(self._h, C.byref(s)))
-        return {f: getattr(s, f) for f, _ in StatsC._fields_}
+        d = {f: getattr(s, f) for f, _ in StatsC._fields_}
+        d["phase_sweeps"] = list(d["phase_sweeps"])
+        return d
