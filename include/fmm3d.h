@@ -119,6 +119,18 @@ int fmm3d_get_field(fmm3d_ctx *ctx, int src, double *ttn);
 /* Statistics of the last solve. */
 int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
 
+/* Solver tuning knobs (no counterpart in the reference, which has none):
+ *   "brick_layers"  layers per brick (>= 4)
+ *   "patch_x", "patch_z"  lateral patch of a brick (supported pairs: 8x32,
+ *                   4x32, 4x16, 8x16, 16x16, 4x64, 2x32, 4x8)
+ *   "lateral_iters" lateral relaxations per layer
+ *   "tol_fp64", "tol_fp32"  relative change below which a sweep counts as
+ *                   settled
+ *   "direction"     0: sweep directions alternate; 1: per brick, along time
+ *   "timing"        1: time every relaxation launch with events (stats.kernel_ms)
+ * Returns FMM3D_E_ARG for an unknown key or a value out of range. */
+int fmm3d_set_option(fmm3d_ctx *ctx, const char *key, double value);
+
 #ifdef __cplusplus
 }
 #endif

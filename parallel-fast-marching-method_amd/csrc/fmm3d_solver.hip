@@ -128,6 +128,7 @@ template <typename R> struct SweepArgs {
     double tol;             // relative change below which a brick counts as settled
     int* valchg;            // per slot, 1 if a sweep changed a value by more than tol
     int dirmode;            // 0: sweep direction alternates; 1: per brick, along increasing time
+    unsigned long long* updates;  // node updates performed
 };
 
 // One node update from gathered stencil values.  Sides are ordered
@@ -249,6 +250,7 @@ k_brick(SweepArgs<R> a)
         }
     }
     bool any = false;
+    unsigned nupd = 0;
     for (int qq = 0; qq < nl; ++qq) {
         const int ir = dir > 0 ? r_hi - qq : r_lo + qq;
         const size_t lay = (size_t)ir * NL;
@@ -303,6 +305,7 @@ k_brick(SweepArgs<R> a)
                     q1[5] = pcu[own + 1]; q2[5] = pcu[own + 2];
                 }
                 nv = node_update<R, SHIFT>(a.phase, a.fom, tc, pc, v1, v2, q1, q2, in1, in2, a.hr, hx, hz, slo);
+                nupd++;
             }
             __syncthreads();
             if (upd && nv != tc) {
@@ -324,6 +327,12 @@ k_brick(SweepArgs<R> a)
         __syncthreads();
     }
     if (any) s_chg = 1;
+    // node updates of this block: wave-level sum, one atomic per wave
+    {
+        unsigned long long w = nupd;
+        for (int off = 32; off > 0; off >>= 1) w += __shfl_down(w, off, 64);
+        if ((threadIdx.x & 63) == 0 && w) atomicAdd(a.updates, w);
+    }
     __syncthreads();
     if (threadIdx.x == 0 && s_chg) {
         st[b] = a.iter;
@@ -666,8 +675,10 @@ struct fmm3d_ctx {
     int brick_sl = 8;              // layers per brick
     int kin = 2;                   // lateral relaxations per layer (at most; stops when settled)
     int dirmode = 0;               // sweep direction policy
+    int timing = 0;                // time every relaxation launch
+    hipEvent_t kev[2 * 8] = {};    // per-launch events (timing)
     double tol64 = 1e-14;          // settle tolerance (relative), float64 fields
-    double tol32 = 1e-6;           // settle tolerance (relative), float32 fields (~8 ulp)
+    double tol32 = 2.5e-7;         // settle tolerance (relative), float32 fields (~2 ulp)
     fmm3d_stats stats{};
     int64_t* d_updates = nullptr;
     unsigned long long* d_visits = nullptr;
@@ -753,6 +764,7 @@ extern "C" int fmm3d_create(const fmm3d_grid_desc* d, int device, fmm3d_ctx** ou
     CREATECHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     CREATECHK(hipEventCreate(&c->ev0));
     CREATECHK(hipEventCreate(&c->ev1));
+    for (auto& e : c->kev) CREATECHK(hipEventCreate(&e));
     size_t NH = (size_t)(g.nnr + 3) * (g.nnx + 3) * (g.nnz + 3);
     CREATECHK(dalloc(&c->veln, NH * sizeof(double)));
     CREATECHK(dalloc(&c->slo64, c->N * sizeof(double)));
@@ -814,6 +826,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->d_updates); (void)hipFree(c->d_visits); (void)hipFree(c->ray_scratch);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (auto& e : c->kev) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return FMM3D_OK;
@@ -843,6 +856,9 @@ extern "C" int fmm3d_set_model_vtx(fmm3d_ctx* c, const double* velv)
     c->have_model = true;
     return FMM3D_OK;
 }
+
+// relaxation launches between convergence checks
+#define FMM3D_CHECK 4
 
 // brick geometry of the current configuration
 struct Bricks {
@@ -995,14 +1011,14 @@ static int run_phases(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     a.tol = std::is_same<R, double>::value ? c->tol64 : c->tol32;
     a.valchg = s.valchg;
     a.dirmode = c->dirmode;
+    a.updates = c->d_visits + 1;
     const int blocks = nsrc * nb;
     const int maxit = 4 * (g.nnr + g.nnx + g.nnz) + 200;
-    const int CHECK = 4;
+    const int CHECK = FMM3D_CHECK;
     std::vector<int> lc(nsrc);
     int total_sweeps = 0;
-    // A, B, C (+ up to 2 repeats of C).  Float32 stops after B: the refreeze
-    // moves values by ~1e-5 s, below the float32 field's own resolution.
-    const int nphases = order != FMM3D_ORDER_MIXED ? 1 : (std::is_same<R, double>::value ? 5 : 2);
+    // A, B, C (+ up to 2 repeats of C for float64; float32 stops after one C)
+    const int nphases = order != FMM3D_ORDER_MIXED ? 1 : (std::is_same<R, double>::value ? 5 : 3);
     for (int ph = 0; ph < nphases; ++ph) {
         a.phase = ph == 0 ? 0 : 1;
         if (ph >= 1) {
@@ -1023,11 +1039,24 @@ static int run_phases(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
             for (int j = 0; j < CHECK; ++j, ++k) {
                 a.iter = k;
                 a.dir = (k & 1) ? 1 : -1;
-                HIPCHK(c, (launch_sweep<R, SHIFT>(c, a, blocks)));
+                if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j], c->stream));
+                {
+                    hipError_t le = launch_sweep<R, SHIFT>(c, a, blocks);
+                    if (le == hipErrorInvalidValue)
+                        return fail(c, FMM3D_E_ARG, "solver: unsupported patch %dx%d", c->tile_x, c->tile_z);
+                    HIPCHK(c, le);
+                }
+                if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j + 1], c->stream));
                 c->stats.kernel_launches++;
             }
             HIPCHK(c, hipMemcpyAsync(lc.data(), s.lastchg, nsrc * sizeof(int), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(c, hipStreamSynchronize(c->stream));
+            if (c->timing)
+                for (int j = 0; j < CHECK; ++j) {
+                    float ms = 0.0f;
+                    HIPCHK(c, hipEventElapsedTime(&ms, c->kev[2 * j], c->kev[2 * j + 1]));
+                    c->stats.kernel_ms += ms;
+                }
             int mx = 0;
             for (int q = 0; q < nsrc; ++q) mx = std::max(mx, lc[q]);
             last_k_with_change = mx;
@@ -1089,7 +1118,7 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     if (le != hipSuccess) { (void)hipFree(dfaces); return fail(c, FMM3D_E_DEVICE, "init: %s", hipGetErrorString(le)); }
     c->stats = fmm3d_stats{};
     c->stats.tile_r = c->brick_sl; c->stats.tile_x = c->tile_x; c->stats.tile_z = c->tile_z;
-    HIPCHK(c, hipMemsetAsync(c->d_visits, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_visits, 0, 2 * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int sweeps = 0;
     if (precision == FMM3D_PREC_FP64) rc = run_phases<double, false>(c, nsrc, order, &sweeps);
@@ -1102,9 +1131,10 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
     c->stats.solve_ms = ms;
     c->stats.sweeps = sweeps;
-    unsigned long long vis = 0;
-    HIPCHK(c, hipMemcpy(&vis, c->d_visits, sizeof vis, hipMemcpyDeviceToHost));
-    c->stats.brick_visits = (int64_t)vis;
+    unsigned long long vis[2] = { 0, 0 };
+    HIPCHK(c, hipMemcpy(vis, c->d_visits, sizeof vis, hipMemcpyDeviceToHost));
+    c->stats.brick_visits = (int64_t)vis[0];
+    c->stats.node_updates = (int64_t)vis[1];
     s.n = nsrc;
     return FMM3D_OK;
 }
@@ -1230,6 +1260,36 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
     if (e == hipErrorUnknown) return fail(c, FMM3D_E_DEVICE, "frechet: row staging overflow");
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "frechet: %s", hipGetErrorString(e));
     if (short_cap) return fail(c, FMM3D_E_CAPACITY, "fmm3d_frechet: capacity %lld < %lld entries", (long long)cap, (long long)row_ptr[nrays]);
+    return FMM3D_OK;
+}
+
+extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
+{
+    if (!c || !key) return fail(c, FMM3D_E_ARG, "fmm3d_set_option: null argument");
+    const int iv = (int)v;
+    if (!strcmp(key, "brick_layers")) {
+        if (iv < 4 || iv > 4096) return fail(c, FMM3D_E_ARG, "brick_layers %d out of range", iv);
+        c->brick_sl = iv;
+    } else if (!strcmp(key, "patch_x") || !strcmp(key, "patch_z")) {
+        int px = !strcmp(key, "patch_x") ? iv : c->tile_x, pz = !strcmp(key, "patch_z") ? iv : c->tile_z;
+        c->tile_x = px; c->tile_z = pz;   // validated at the next solve
+    } else if (!strcmp(key, "lateral_iters")) {
+        if (iv < 1 || iv > 256) return fail(c, FMM3D_E_ARG, "lateral_iters %d out of range", iv);
+        c->kin = iv;
+    } else if (!strcmp(key, "tol_fp64")) {
+        if (!(v >= 0.0 && v < 1e-3)) return fail(c, FMM3D_E_ARG, "tol_fp64 %g out of range", v);
+        c->tol64 = v;
+    } else if (!strcmp(key, "tol_fp32")) {
+        if (!(v >= 0.0 && v < 1e-2)) return fail(c, FMM3D_E_ARG, "tol_fp32 %g out of range", v);
+        c->tol32 = v;
+    } else if (!strcmp(key, "direction")) {
+        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "direction must be 0 or 1");
+        c->dirmode = iv;
+    } else if (!strcmp(key, "timing")) {
+        c->timing = iv != 0;
+    } else {
+        return fail(c, FMM3D_E_ARG, "fmm3d_set_option: unknown key '%s'", key);
+    }
     return FMM3D_OK;
 }
 

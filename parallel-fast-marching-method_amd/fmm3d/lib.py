@@ -23,7 +23,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 
 EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model_vtx",
            "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_frechet", "fmm3d_get_field",
-           "fmm3d_get_stats")
+           "fmm3d_get_stats", "fmm3d_set_option")
 
 
 class GridDescC(C.Structure):
@@ -76,6 +76,7 @@ def load(path: Optional[str] = None):
     L.fmm3d_frechet.argtypes = [vp, C.c_int, dp, lp, ip, fp, C.c_int64]
     L.fmm3d_get_field.argtypes = [vp, C.c_int, dp]
     L.fmm3d_get_stats.argtypes = [vp, C.POINTER(StatsC)]
+    L.fmm3d_set_option.argtypes = [vp, C.c_char_p, C.c_double]
     for f in EXPORTS:
         if f not in ("fmm3d_last_error",):
             getattr(L, f).restype = C.c_int
@@ -165,6 +166,9 @@ class Context:
             n = int(rp[-1])
             return rp, cols[:n].copy(), vals[:n].copy()
         raise FmmError(E_CAPACITY, "frechet capacity retry failed")
+
+    def set_option(self, key: str, value: float) -> None:
+        self._check(self.L.fmm3d_set_option(self._h, key.encode(), float(value)))
 
     def stats(self) -> dict:
         s = StatsC()

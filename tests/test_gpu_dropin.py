@@ -1,0 +1,51 @@
+"""The Fortran drop-in host (fm3dt_gpu) in a directory laid out like the
+reference's working directory: it must read the same files and write
+rtravel.out and frechet.out in the reference's formats, with the values the
+library computes."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import goldens as G
+from fmm3d import build, io, lib, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_dropin_writes_the_reference_file_contract(tmp_path):
+    if not os.path.exists(build.EXE):
+        pytest.skip("fm3dt_gpu not built")
+    g, velv, srcs, rcv = synth.tele_case((31, 33, 35), 400.0, (10.0, 10.0), (10, 11, 12), 0.05, 5, 12, 21)
+    info = synth.write_case(str(tmp_path), g, velv, srcs, rcv, fom=1, cfd=1)
+    env = dict(os.environ, FMM3D_BATCH="2")
+    r = subprocess.run([build.EXE], cwd=str(tmp_path), capture_output=True, text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr
+    tr = io.read_rtravel(str(tmp_path / "rtravel.out")).reshape(5, 12)
+    rows = io.read_frechet(str(tmp_path / "frechet.out"))
+    assert len(rows) == 5 * 12
+    rr = io.read_receivers(str(tmp_path / "receivers.dat"))[0]
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        c.solve_tele(info["faces"], lib.ORDER_MIXED, lib.PREC_FP64)
+        t = c.receivers(rr)
+        rp, cols, vals = c.frechet(rr)
+    assert np.array_equal(tr, t)
+    for k, (rid, rc, rv) in enumerate(rows):
+        assert rid == k % 12 + 1
+        assert np.array_equal(rc, cols[rp[k]:rp[k + 1]]) and np.array_equal(rv, vals[rp[k]:rp[k + 1]])
+
+
+def test_dropin_rejects_inconsistent_array_counts(tmp_path):
+    if not os.path.exists(build.EXE):
+        pytest.skip("fm3dt_gpu not built")
+    g, velv, srcs, rcv = synth.tele_case((11, 12, 13), 100.0, (2.0, 2.0), (6, 6, 7), 0.05, 2, 4, 3)
+    synth.write_case(str(tmp_path), g, velv, srcs, rcv)
+    with open(tmp_path / "receivers.dat") as f:
+        lines = f.read().split("\n")
+    lines[0] = "2"
+    with open(tmp_path / "receivers.dat", "w") as f:
+        f.write("\n".join(lines))
+    r = subprocess.run([build.EXE], cwd=str(tmp_path), capture_output=True, text=True, timeout=60)
+    assert "inconsistent" in r.stdout
