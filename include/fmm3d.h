@@ -66,7 +66,8 @@ typedef struct {
     double kernel_ms;           /* device time inside the relaxation kernel */
     int64_t kernel_launches;    /* relaxation-kernel launches              */
     int32_t phase_sweeps[6];    /* sweeps per phase: A, B, C, C, C, -      */
-    int64_t brick_visits;       /* bricks relaxed (active bricks summed over sweeps) */
+    int64_t brick_visits;       /* bricks / patch columns relaxed (summed over sweeps) */
+    int64_t node_visits;        /* column walk: free nodes staged (summed over sweeps) */
 } fmm3d_stats;
 
 typedef struct fmm3d_ctx fmm3d_ctx;
@@ -127,6 +128,13 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *   "tol_fp64", "tol_fp32"  relative change below which a sweep counts as
  *                   settled
  *   "direction"     0: sweep directions alternate; 1: per brick, along time
+ *   "strategy"      0: brick sweeps in three phases; 1: column walk (default)
+ *   "column_x", "column_z"  column-walk patch (8, 16, 32 or 64 each; supported
+ *                   pairs 8x8, 8x16, 16x8, 16x16, 8x32, 32x8, 16x32, 32x16,
+ *                   32x32, 16x64, 64x16, 8x64, 64x8)
+ *   "column_passes" line passes per layer at most (1..16, default 4)
+ *   "max_sweeps"    column walk: stop after this many sweeps (0 = at
+ *                   convergence); a diagnostic, the field is then approximate
  *   "timing"        1: time every relaxation launch with events (stats.kernel_ms)
  * Returns FMM3D_E_ARG for an unknown key or a value out of range. */
 int fmm3d_set_option(fmm3d_ctx *ctx, const char *key, double value);

@@ -766,78 +766,93 @@ double or_fim_sweep_mode(const or_grid *g, const double *veln, const unsigned ch
 /* The same sweep with the stencil choice taken from a separate ordering
  * field t_pred (neighbour counts iff t_pred(nb) < t_pred(c); second order
  * iff t_pred(nb2) < t_pred(nb)); values come from t_in. */
+/* The value-based node update shared by the relaxations: neighbours count
+ * by the ordering field t_pred (nb counts iff t_pred(nb) < t_pred(c) and
+ * its value is known; second order iff t_pred(nb2) < t_pred(nb)), values
+ * come from t_in; minimum over the valid octants (F:535-1004, F:1112-1150).
+ * Indices 1-based. */
+static double relax_node(const or_grid *g, const double *veln, const double *ri, const double *sth,
+                         int fom, const double *t_in, const double *t_pred, int ir, int ix, int iz)
+{
+    const int nr = g->nnr, nx = g->nnx, nz = g->nnz;
+    const int MX = nx + 3, MZ = nz + 3;
+#define TI(r, x, z) t_in[((size_t)((r) - 1) * nx + ((x) - 1)) * nz + ((z) - 1)]
+#define TP(r, x, z) t_pred[((size_t)((r) - 1) * nx + ((x) - 1)) * nz + ((z) - 1)]
+    const double pc = TP(ir, ix, iz);
+    const int n[3] = { nr, nx, nz };
+    const int c[3] = { ir, ix, iz };
+    double rr = ri[ir];
+    double h[3] = { g->dnr, rr * g->dnx, rr * sth[ix] * g->dnz };
+    int ord[3][2];
+    double t1[3][2], t2[3][2];
+    for (int a = 0; a < 3; ++a)
+        for (int sd = 0; sd < 2; ++sd) {
+            int d = sd ? 1 : -1;
+            int p1[3] = { c[0], c[1], c[2] }, p2[3] = { c[0], c[1], c[2] };
+            p1[a] += d; p2[a] += 2 * d;
+            ord[a][sd] = 0;
+            if (p1[a] < 1 || p1[a] > n[a]) { ord[a][sd] = -1; continue; }
+            double v1 = TI(p1[0], p1[1], p1[2]), q1 = TP(p1[0], p1[1], p1[2]);
+            if (!(q1 < pc) || !(v1 < INFINITY)) continue;   /* inf never counts */
+            ord[a][sd] = 1; t1[a][sd] = v1;
+            if (fom && p2[a] >= 1 && p2[a] <= n[a]) {
+                double v2 = TI(p2[0], p2[1], p2[2]), q2 = TP(p2[0], p2[1], p2[2]);
+                if (q2 < INFINITY && v2 < INFINITY && q1 > q2) { ord[a][sd] = 2; t2[a][sd] = v2; }
+            }
+        }
+#undef TI
+#undef TP
+    double best = INFINITY;
+    for (int sr = 0; sr < 2; ++sr)
+        for (int sx = 0; sx < 2; ++sx)
+            for (int sz = 0; sz < 2; ++sz) {
+                if (ord[0][sr] < 0 || ord[1][sx] < 0 || ord[2][sz] < 0) continue;
+                int o[3] = { ord[0][sr], ord[1][sx], ord[2][sz] };
+                if (!o[0] && !o[1] && !o[2]) continue;
+                double a1[3] = { t1[0][sr], t1[1][sx], t1[2][sz] };
+                double a2[3] = { t2[0][sr], t2[1][sx], t2[2][sz] };
+                double T = or_octant_solve(o, a1, a2, h,
+                    1.0 / veln[((size_t)(ir + 1) * MX + (ix + 1)) * MZ + (iz + 1)]);
+                if (T < best) best = T;
+            }
+    return best;
+}
+
 double or_fim_sweep_pred(const or_grid *g, const double *veln, const unsigned char *fixed,
                          int fom, const double *t_in, const double *t_pred, double *t_out, int mode)
 {
     const int nr = g->nnr, nx = g->nnx, nz = g->nnz;
-    const int MX = nx + 3, MZ = nz + 3;
     double *sth = malloc((nx + 2) * sizeof(double));
     double *ri = malloc((nr + 2) * sizeof(double));
     metric_tables(g, ri, sth);
     double maxd = 0.0;
-#define TI(r, x, z) t_in[((size_t)((r) - 1) * nx + ((x) - 1)) * nz + ((z) - 1)]
-#define TP(r, x, z) t_pred[((size_t)((r) - 1) * nx + ((x) - 1)) * nz + ((z) - 1)]
     for (int ir = 1; ir <= nr; ++ir)
         for (int ix = 1; ix <= nx; ++ix)
             for (int iz = 1; iz <= nz; ++iz) {
                 size_t k = ((size_t)(ir - 1) * nx + (ix - 1)) * nz + (iz - 1);
-                double tc = t_in[k], pc = t_pred[k];
+                double tc = t_in[k];
                 if (fixed[k]) { t_out[k] = tc; continue; }
-                const int n[3] = { nr, nx, nz };
-                const int c[3] = { ir, ix, iz };
-                double rr = ri[ir];
-                double h[3] = { g->dnr, rr * g->dnx, rr * sth[ix] * g->dnz };
-                int ord[3][2];
-                double t1[3][2], t2[3][2];
-                for (int a = 0; a < 3; ++a)
-                    for (int sd = 0; sd < 2; ++sd) {
-                        int d = sd ? 1 : -1;
-                        int p1[3] = { c[0], c[1], c[2] }, p2[3] = { c[0], c[1], c[2] };
-                        p1[a] += d; p2[a] += 2 * d;
-                        ord[a][sd] = 0;
-                        if (p1[a] < 1 || p1[a] > n[a]) { ord[a][sd] = -1; continue; }
-                        double v1 = TI(p1[0], p1[1], p1[2]), q1 = TP(p1[0], p1[1], p1[2]);
-                        if (!(q1 < pc) || !(v1 < INFINITY)) continue;   /* inf never counts */
-                        ord[a][sd] = 1; t1[a][sd] = v1;
-                        if (fom && p2[a] >= 1 && p2[a] <= n[a]) {
-                            double v2 = TI(p2[0], p2[1], p2[2]), q2 = TP(p2[0], p2[1], p2[2]);
-                            if (q2 < INFINITY && v2 < INFINITY && q1 > q2) { ord[a][sd] = 2; t2[a][sd] = v2; }
-                        }
-                    }
-                double best = INFINITY;
-                for (int sr = 0; sr < 2; ++sr)
-                    for (int sx = 0; sx < 2; ++sx)
-                        for (int sz = 0; sz < 2; ++sz) {
-                            if (ord[0][sr] < 0 || ord[1][sx] < 0 || ord[2][sz] < 0) continue;
-                            int o[3] = { ord[0][sr], ord[1][sx], ord[2][sz] };
-                            if (!o[0] && !o[1] && !o[2]) continue;
-                            double a1[3] = { t1[0][sr], t1[1][sx], t1[2][sz] };
-                            double a2[3] = { t2[0][sr], t2[1][sx], t2[2][sz] };
-                            double T = or_octant_solve(o, a1, a2, h,
-                                1.0 / veln[((size_t)(ir + 1) * MX + (ix + 1)) * MZ + (iz + 1)]);
-                            if (T < best) best = T;
-                        }
+                double best = relax_node(g, veln, ri, sth, fom, t_in, t_pred, ir, ix, iz);
                 double nv;
                 if (mode == 1 && best == INFINITY && tc < INFINITY) nv = INFINITY;  /* unsupported value: drop */
                 else nv = mode == 1 ? (best < INFINITY ? best : tc) : ((best <= tc) ? best : tc);
                 t_out[k] = nv;
                 if (tc < INFINITY) { double dd = fabs(nv - tc); if (dd > maxd) maxd = dd; }
             }
-#undef TI
-#undef TP
     free(sth); free(ri);
     return maxd;
 }
 
-/* Teleseismic solve by relaxation to the fixed point, the three phases the
- * HIP solver runs: first order with the value-based predicate until no
- * value moves; then mixed order with the stencil choice frozen from the
- * previous field, twice (or until a pass moves nothing).  In-place sweeps,
- * exact convergence.  With a fully known bottom face this equals
- * or_fmm_tele bit for bit; where the face has unknown nodes it can be
- * earlier, because the heap march reaches such nodes only from above (the
- * face nodes other than the seed never leave the alive set through the
- * heap, F:6930-6958).  Returns the total number of sweeps or -1. */
+/* Three-phase relaxation for a teleseismic bottom face: phase A relaxes the
+ * value-based first-order update (Gauss-Seidel, unsupported values dropped)
+ * to its fixed point; phases B, C, ... freeze the ordering from the previous
+ * phase and relax the mixed-order update with it, until a phase changes
+ * nothing.  Each phase iterates to exact convergence.  With a fully known
+ * bottom face this equals or_fmm_tele bit for bit; where the face has
+ * unknown nodes it can be earlier, because the heap march reaches such
+ * nodes only from above (the face nodes other than the seed never leave the
+ * alive set through the heap, F:6930-6958).  Returns the total number of
+ * sweeps or -1. */
 long or_relax_tele(const or_grid *g, const double *veln, const double *face, int fom, double *ttn)
 {
     const size_t NL = (size_t)g->nnx * g->nnz, N = NL * g->nnr;
@@ -863,6 +878,46 @@ long or_relax_tele(const or_grid *g, const double *veln, const double *face, int
     }
 out:
     free(fixed); free(P); free(prev);
+    return sweeps;
+}
+
+/* Single-phase relaxation in column-walk order for a teleseismic bottom
+ * face: the value-based mixed-order update (the node's own value orders its
+ * neighbours) with unsupported values dropped, applied in place layer by
+ * layer from the bottom to the top; each layer gets four line passes
+ * (phi ascending, phi descending, theta ascending, theta descending).
+ * Sweeps repeat until one changes nothing.  This is the discrete system the
+ * GPU column walk relaxes; its fixed point is the heap march's field where
+ * the bottom face is fully known.  Returns the number of sweeps or -1. */
+long or_relax1_tele(const or_grid *g, const double *veln, const double *face, int fom, double *ttn)
+{
+    const int nr = g->nnr, nx = g->nnx, nz = g->nnz;
+    const size_t NL = (size_t)nx * nz, N = NL * nr;
+    unsigned char *fixed = calloc(N, 1);
+    double *sth = malloc((nx + 2) * sizeof(double));
+    double *ri = malloc((nr + 2) * sizeof(double));
+    metric_tables(g, ri, sth);
+    for (size_t i = 0; i < N; ++i) ttn[i] = INFINITY;
+    for (size_t f = 0; f < NL; ++f)
+        if (face[f] > 0.0) { ttn[(size_t)(nr - 1) * NL + f] = face[f]; fixed[(size_t)(nr - 1) * NL + f] = 1; }
+    long sweeps = -1;
+    for (int it = 1; it <= 100000; ++it) {
+        int moved = 0;
+        for (int ir = nr; ir >= 1; --ir)
+            for (int pass = 0; pass < 4; ++pass)
+                for (int a = 0; a < (pass < 2 ? nz : nx); ++a)
+                    for (int b = 0; b < (pass < 2 ? nx : nz); ++b) {
+                        int ix, iz;
+                        if (pass < 2) { iz = 1 + (pass == 0 ? a : nz - 1 - a); ix = 1 + b; }
+                        else { ix = 1 + (pass == 2 ? a : nx - 1 - a); iz = 1 + b; }
+                        size_t k = ((size_t)(ir - 1) * nx + (ix - 1)) * nz + (iz - 1);
+                        if (fixed[k]) continue;
+                        double nv = relax_node(g, veln, ri, sth, fom, ttn, ttn, ir, ix, iz);
+                        if (nv != ttn[k]) { ttn[k] = nv; moved = 1; }
+                    }
+        if (!moved) { sweeps = it; break; }
+    }
+    free(fixed); free(sth); free(ri);
     return sweeps;
 }
 

@@ -57,6 +57,8 @@ def lib():
         L.or_fmm_tele_part.restype = C.c_int
         L.or_relax_tele.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, dp]
         L.or_relax_tele.restype = C.c_long
+        L.or_relax1_tele.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, dp]
+        L.or_relax1_tele.restype = C.c_long
         L.or_fim_sweep.argtypes = [C.POINTER(OrGrid), dp, up, C.c_int, dp, dp]
         L.or_fim_sweep.restype = C.c_double
         L.or_srtimes.argtypes = [C.POINTER(OrGrid), dp, C.c_int, dp, fp]
@@ -96,12 +98,22 @@ def fmm_tele_part(g, velv: np.ndarray, face: np.ndarray, fom: int = 1, max_pass:
 
 
 def relax_tele(g, veln: np.ndarray, face: np.ndarray, fom: int = 1) -> np.ndarray:
-    """Relaxation fixed point (the HIP solver's algorithm on the CPU); unknown = inf."""
+    """Three-phase relaxation fixed point (the brick-sweep strategy's system); unknown = inf."""
     ttn = np.zeros((g.nnr, g.nnx, g.nnz))
     n = lib().or_relax_tele(C.byref(or_grid(g)), veln, np.ascontiguousarray(face, dtype=np.float64), fom, ttn)
     if n < 0:
         raise RuntimeError("or_relax_tele: no convergence")
     return ttn
+
+
+def relax1_tele(g, veln: np.ndarray, face: np.ndarray, fom: int = 1):
+    """Single-phase value-based relaxation in column-walk order (the column
+    strategy's system) to its fixed point; returns (field, sweeps)."""
+    ttn = np.zeros((g.nnr, g.nnx, g.nnz))
+    n = lib().or_relax1_tele(C.byref(or_grid(g)), veln, np.ascontiguousarray(face, dtype=np.float64), fom, ttn)
+    if n < 0:
+        raise RuntimeError("or_relax1_tele: no convergence")
+    return ttn, n
 
 
 def fim_sweep(g, veln, fixed, t_in, fom=1):

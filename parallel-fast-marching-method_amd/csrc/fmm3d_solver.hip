@@ -341,6 +341,291 @@ k_brick(SweepArgs<R> a)
     }
 }
 
+// Column walk (single phase).  A one-wavefront block owns a PX x PZ patch of
+// every layer and walks the column from the bottom layer to the top.  Each
+// layer is staged in LDS (values with a two-node lateral halo, the two
+// layers above and the two just finished below, slowness with fixed nodes
+// marked negative) and relaxed by four ordered line passes: +phi, -phi,
+// +theta, -theta.  In a phi pass every lane owns an S-node segment of one
+// theta row (S = PX*PZ/64) and steps along it; in a theta pass a lane owns
+// an S-node segment of one phi column.  The lanes step together, so the
+// nodes of one step update from the values the previous steps left.  The
+// layer goes back to HBM once, after its passes.
+//
+// Node updates use the value-based mixed-order predicate on the node's own
+// value (a neighbour counts iff its value is below the node's -- float32:
+// not above; second order iff the second neighbour is known and below the
+// first) and drop a value no neighbour supports.  Processed in this
+// near-causal order the relaxation converges to the heap march's field
+// itself; the lateral halo is read from HBM when the layer is staged
+// (neighbouring patches walk the same layers concurrently), so a patch
+// column runs again in the next sweep while it or a lateral neighbour
+// changed.
+template <typename R> struct ColArgs {
+    R* T;
+    const R* slo;
+    const R* hx;
+    const R* hz;
+    R hr;
+    const uint32_t* fixbits;
+    int* stamp;             // per slot per patch column
+    int* lastchg;
+    int* valchg;
+    const int* slot_on;
+    int nnr, nnx, nnz;
+    int npx, npz;           // patches per axis
+    int nslots;
+    int iter;
+    int fom;
+    int maxpass;            // line passes per layer at most (cycle +phi, -phi, +theta, -theta)
+    double tol;
+    unsigned long long* visits;
+    unsigned long long* updates;
+    unsigned long long* nodes;   // free nodes staged
+};
+
+// float32 helpers for the axis-sum octant form: fused multiply-subtract,
+// hardware square root and reciprocal (about 1 ulp each).
+template <typename R> __device__ __forceinline__ R fused_mul_sub(R a, R b, R c) { return fma(a, b, -c); }
+template <typename R> __device__ __forceinline__ R fast_sqrt(R x) { return sqrt(x); }
+template <> __device__ __forceinline__ float fast_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <typename R> __device__ __forceinline__ R fast_rcp(R x) { return R(1) / x; }
+template <> __device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Octant minimum in the axis-sum form (FAST): for the counted sides of an
+// octant sum (c*T - D)^2 / H^2 = s^2 with c, D, H = (1, t1, h) first order
+// and (3, 4 t1 - t2, 2 h) second order, i.e. A T^2 - 2 B T + C = 0 with
+// A = sum (c/H)^2, B = sum (c/H)(D/H), C = sum (D/H)^2 - s^2; the larger
+// root with the discriminant clamped at zero.  This is the reference's
+// octant quadratic after the substitution T = (X + D_L)/c_L on its leading
+// axis (fm3dt_parallel.f90:535-1004), so the two agree up to rounding.
+// Octants through a missing side are skipped, octants with no counted side
+// give nothing, as in the reference.  Values arrive shifted by the smallest
+// counted neighbour.
+// One node update from the gathered stencil (sides r-, r+, x-, x+, z-, z+).
+// FAST selects the axis-sum octant form; otherwise the reference's
+// leading-axis form in the CPU oracle's operation order.
+template <typename R, bool FAST, bool SHIFT>
+__device__ __forceinline__ R col_update(int fom, R tc, const R v1[6], const R v2[6], const bool in1[6],
+                                        const bool in2[6], const R h[3], const R ih[3], R slown)
+{
+    const R inf = Lim<R>::inf();
+    bool cnt[6], sec[6];
+    R tb = inf;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        cnt[q] = in1[q] && (SHIFT ? (v1[q] <= tc && v1[q] < inf) : (v1[q] < tc));
+        sec[q] = cnt[q] && fom && in2[q] && v2[q] < inf && v1[q] > v2[q];
+        if (SHIFT && cnt[q] && v1[q] < tb) tb = v1[q];
+    }
+    if (SHIFT && !(tb < inf)) return inf;
+    const R sh = SHIFT ? tb : R(0);
+    R best;
+    if (FAST) {
+        // per side (a, b, c) = (alpha^2, alpha*beta, beta^2); a missing side
+        // carries a = NaN so every octant through it fails the A > 0 test
+        const R nan = __builtin_nan("");
+        R ta[6], tbb[6], tc2[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            // selects on plain values keep this branch-free
+            const R t1 = v1[q] - sh, t2 = v2[q] - sh;
+            const R num = sec[q] ? R(2) * t1 - R(0.5) * t2 : t1;
+            const R al = ih[q / 2] * (cnt[q] ? (sec[q] ? R(1.5) : R(1)) : R(0));
+            const R be = cnt[q] ? num * ih[q / 2] : R(0);
+            ta[q] = in1[q] ? al * al : nan;
+            tbb[q] = al * be;
+            tc2[q] = be * be;
+        }
+        const R s2 = slown * slown;
+        tc2[0] -= s2; tc2[1] -= s2;
+        best = inf;
+#pragma unroll
+        for (int j = 2; j < 4; ++j)
+#pragma unroll
+            for (int k = 4; k < 6; ++k) {
+                const R Axz = ta[j] + ta[k], Bxz = tbb[j] + tbb[k], Cxz = tc2[j] + tc2[k];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const R A = ta[i] + Axz, B = tbb[i] + Bxz, C = tc2[i] + Cxz;
+                    R disc = fused_mul_sub(B, B, A * C);
+                    disc = disc > R(0) ? disc : R(0);
+                    R T = (B + fast_sqrt(disc)) * fast_rcp(A);
+                    T = A > R(0) ? T : inf;
+                    best = T < best ? T : best;
+                }
+            }
+    } else {
+        Side<R> sd[3][2];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            Side<R>& S = sd[q / 2][q % 2];
+            S.ord = !in1[q] ? -1 : (sec[q] ? 2 : (cnt[q] ? 1 : 0));
+            S.t1 = cnt[q] ? v1[q] - sh : R(0);
+            S.t2 = sec[q] ? v2[q] - sh : R(0);
+        }
+        best = octant_min<R>(sd, h[0], h[1], h[2], slown);
+    }
+    if (SHIFT && best < inf) best = tb + best;
+    return best;
+}
+
+template <typename R, int PX, int PZ, bool FAST, bool SHIFT>
+__global__ void __launch_bounds__(64)
+k_column(ColArgs<R> a)
+{
+    constexpr int NP = PX * PZ, S = NP / 64;
+    static_assert(NP % 64 == 0 && PX % S == 0 && PZ % S == 0, "segments must tile the patch");
+    constexpr int LX = PX + 4, LZ = (PZ + 4) | 1;   // odd row stride: phi passes spread over banks
+    __shared__ R cur[LX * LZ];
+    __shared__ R b1[NP], b2[NP], u1[NP], u2[NP], sl[NP];
+    __shared__ R shz[PX], sihz[PX];
+    __shared__ int s_act;
+    const int np = a.npx * a.npz;
+    const int slot = blockIdx.x / np;
+    const int b = blockIdx.x - slot * np;
+    if (slot >= a.nslots || !a.slot_on[slot]) return;
+    const int bx = b / a.npz, bz = b % a.npz;
+    int* st = a.stamp + (size_t)slot * np;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        int m = st[b];
+        if (bx > 0) m = max(m, st[b - a.npz]);
+        if (bx < a.npx - 1) m = max(m, st[b + a.npz]);
+        if (bz > 0) m = max(m, st[b - 1]);
+        if (bz < a.npz - 1) m = max(m, st[b + 1]);
+        s_act = (m >= a.iter - 1);
+    }
+    __syncthreads();
+    if (!s_act) return;
+    if (t == 0) atomicAdd(a.visits, 1ull);
+    const size_t NL = (size_t)a.nnx * a.nnz;
+    const size_t N = NL * a.nnr;
+    R* T = a.T + (size_t)slot * N;
+    const uint32_t* fb = a.fixbits + (size_t)slot * ((N + 31) / 32);
+    const R inf = Lim<R>::inf();
+    const int x0 = bx * PX, z0 = bz * PZ;
+    const R hr = a.hr, ihr = R(1) / a.hr;
+    bool anyv = false;
+    unsigned nupd = 0, nvis = 0;
+    R orig[S];
+    for (int ir = a.nnr - 1; ir >= 0; --ir) {
+        const size_t lay = (size_t)ir * NL;
+        unsigned nfree = 0;
+        // stage: interior (own values kept for the write-back), above, slowness
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const int e = t + 64 * k;
+            const int lx = e / PZ, lz = e % PZ;
+            const int gx = x0 + lx, gz = z0 + lz;
+            const bool in = gx < a.nnx && gz < a.nnz;
+            const size_t c = lay + (size_t)gx * a.nnz + gz;
+            const R v = in ? T[c] : inf;
+            orig[k] = v;
+            cur[(lx + 2) * LZ + lz + 2] = v;
+            u1[e] = (in && ir >= 1) ? T[c - NL] : inf;
+            u2[e] = (in && ir >= 2) ? T[c - 2 * NL] : inf;
+            const bool fixed = !in || ((fb[c >> 5] >> (c & 31)) & 1u);
+            sl[e] = fixed ? R(-1) : a.slo[c];
+            nfree += fixed ? 0u : 1u;
+            if (ir == a.nnr - 1) { b1[e] = inf; b2[e] = inf; }
+        }
+        // lateral halo ring
+        for (int e = t; e < LX * (PZ + 4); e += 64) {
+            const int hx_ = e / (PZ + 4), hz_ = e % (PZ + 4);
+            if (hx_ >= 2 && hx_ < PX + 2 && hz_ >= 2 && hz_ < PZ + 2) continue;
+            const int gx = x0 + hx_ - 2, gz = z0 + hz_ - 2;
+            const bool in = gx >= 0 && gx < a.nnx && gz >= 0 && gz < a.nnz;
+            cur[hx_ * LZ + hz_] = in ? T[lay + (size_t)gx * a.nnz + gz] : inf;
+        }
+        for (int e = t; e < PX; e += 64) {
+            const int gx = min(x0 + e, a.nnx - 1);
+            const R h = a.hz[(size_t)ir * a.nnx + gx];
+            shz[e] = h;
+            sihz[e] = R(1) / h;
+        }
+        const R hxl = a.hx[ir], ihx = R(1) / hxl;
+        __syncthreads();
+        // the passes stop after one that changes nothing: the layer is then
+        // at a fixpoint of the update and later passes would not move it
+        int npass = 0;
+#pragma unroll 1
+        for (int pass = 0; pass < a.maxpass; ++pass) {
+            const bool zpass = (pass & 3) < 2, rev = pass & 1;
+            bool moved = false;
+            const int lane_l = zpass ? t / (PZ / S) : t % PZ;          // line (row or column) index
+            const int seg0 = (zpass ? t % (PZ / S) : t / PZ) * S;     // segment start along the line
+#pragma unroll 1
+            for (int s = 0; s < S; ++s) {
+                const int p = seg0 + (rev ? S - 1 - s : s);
+                const int lx = zpass ? lane_l : p, lz = zpass ? p : lane_l;
+                const int pl = lx * PZ + lz;
+                const R slown = sl[pl];
+                if (slown >= R(0)) {
+                    const int ix = x0 + lx, iz = z0 + lz;
+                    const int own = (lx + 2) * LZ + lz + 2;
+                    const R tc = cur[own];
+                    R v1[6], v2[6];
+                    bool in1[6], in2[6];
+                    in1[0] = ir >= 1; in2[0] = ir >= 2;
+                    in1[1] = ir + 1 < a.nnr; in2[1] = ir + 2 < a.nnr;
+                    in1[2] = ix >= 1; in2[2] = ix >= 2;
+                    in1[3] = ix + 1 < a.nnx; in2[3] = ix + 2 < a.nnx;
+                    in1[4] = iz >= 1; in2[4] = iz >= 2;
+                    in1[5] = iz + 1 < a.nnz; in2[5] = iz + 2 < a.nnz;
+                    v1[0] = u1[pl]; v2[0] = u2[pl];
+                    v1[1] = b1[pl]; v2[1] = b2[pl];
+                    v1[2] = cur[own - LZ]; v2[2] = cur[own - 2 * LZ];
+                    v1[3] = cur[own + LZ]; v2[3] = cur[own + 2 * LZ];
+                    v1[4] = cur[own - 1]; v2[4] = cur[own - 2];
+                    v1[5] = cur[own + 1]; v2[5] = cur[own + 2];
+                    const R h[3] = { hr, hxl, shz[lx] };
+                    const R ih[3] = { ihr, ihx, sihz[lx] };
+                    const R nv = col_update<R, FAST, SHIFT>(a.fom, tc, v1, v2, in1, in2, h, ih, slown);
+                    // every lane has read its stencil before any lane writes (one wavefront)
+                    __builtin_amdgcn_wave_barrier();
+                    if (nv != tc) { cur[own] = nv; moved = true; }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            ++npass;
+            if (!__ballot(moved)) break;
+        }
+        nupd += nfree * (unsigned)npass;
+        nvis += nfree;
+        __syncthreads();
+        // write back what changed; the layer becomes the one below
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const int e = t + 64 * k;
+            const int lx = e / PZ, lz = e % PZ;
+            const R nv = cur[(lx + 2) * LZ + lz + 2];
+            const R tc = orig[k];
+            if (nv != tc) {
+                T[lay + (size_t)(x0 + lx) * a.nnz + z0 + lz] = nv;
+                const bool fin_n = nv < inf, fin_t = tc < inf;
+                if (fin_n != fin_t || !(fabs((double)nv - (double)tc) <= a.tol * fmax(fabs((double)nv), fabs((double)tc))))
+                    anyv = true;
+            }
+            b2[e] = b1[e];
+            b1[e] = nv;
+        }
+        __syncthreads();
+    }
+    {
+        unsigned long long w = nupd, wv = nvis;
+        for (int off = 32; off > 0; off >>= 1) { w += __shfl_down(w, off, 64); wv += __shfl_down(wv, off, 64); }
+        if (t == 0 && w) atomicAdd(a.updates, w);
+        if (t == 0 && wv) atomicAdd(a.nodes, wv);
+    }
+    const bool blk = __syncthreads_or(anyv);
+    if (t == 0 && blk) {
+        st[b] = a.iter;
+        atomicMax(a.lastchg + slot, a.iter);
+        atomicOr(a.valchg + slot, 1);
+    }
+}
+
 // ------------------------------------------------------------ receivers
 
 template <typename R>
@@ -675,15 +960,31 @@ struct fmm3d_ctx {
     int brick_sl = 8;              // layers per brick
     int kin = 2;                   // lateral relaxations per layer (at most; stops when settled)
     int dirmode = 0;               // sweep direction policy
+    int strategy = 1;              // 0: brick sweeps in three phases; 1: column walk, one phase
+    int col_px = 8, col_pz = 8;    // column-walk patch (theta x phi)
+    int col_passes = 4;            // line passes per layer at most
+    int max_sweeps = 0;            // column walk: stop after this many sweeps (0: at convergence)
+    int trace = 0;                 // per-sweep counters to stderr (FMM3D_TRACE)
     int timing = 0;                // time every relaxation launch
     hipEvent_t kev[2 * 8] = {};    // per-launch events (timing)
     double tol64 = 1e-14;          // settle tolerance (relative), float64 fields
-    double tol32 = 2.5e-7;         // settle tolerance (relative), float32 fields (~2 ulp)
+    double tol32 = 1e-6;           // settle tolerance (relative), float32 fields (~8-16 ulp)
     fmm3d_stats stats{};
     int64_t* d_updates = nullptr;
     unsigned long long* d_visits = nullptr;
     float* ray_scratch = nullptr;
     size_t ray_scratch_workers = 0;
+    // Frechet row staging (per ray `fr_rcap` entries), counts, offsets and the packed rows
+    int32_t* fr_cols = nullptr;
+    float* fr_vals = nullptr;
+    size_t fr_stage = 0;           // entries the staging holds
+    int32_t* fr_count = nullptr;
+    int64_t* fr_off = nullptr;
+    size_t fr_rays = 0;            // rays the count/offset arrays hold
+    int32_t* fr_pcols = nullptr;
+    float* fr_pvals = nullptr;
+    size_t fr_packed = 0;          // entries the packed arrays hold
+    int64_t fr_rcap = 2048;        // staging entries per ray (grows on overflow)
 };
 
 static std::string g_create_err;
@@ -770,7 +1071,7 @@ extern "C" int fmm3d_create(const fmm3d_grid_desc* d, int device, fmm3d_ctx** ou
     CREATECHK(dalloc(&c->slo64, c->N * sizeof(double)));
     CREATECHK(dalloc(&c->slo32, c->N * sizeof(float)));
     CREATECHK(dalloc(&c->d_updates, sizeof(int64_t)));
-    CREATECHK(dalloc(&c->d_visits, 2 * sizeof(unsigned long long)));
+    CREATECHK(dalloc(&c->d_visits, 3 * sizeof(unsigned long long)));
     // metric tables in the reference's arithmetic (F:541-542)
     std::vector<double> hx(g.nnr), hz((size_t)g.nnx * g.nnr);
     std::vector<float> hx32(g.nnr), hz32((size_t)g.nnx * g.nnr);
@@ -803,6 +1104,20 @@ extern "C" int fmm3d_create(const fmm3d_grid_desc* d, int device, fmm3d_ctx** ou
         int n = sscanf(ts, "%dx%dx%d:%d", &a, &b, &e, &k);
         if (n >= 3) { c->brick_sl = a; c->tile_x = b; c->tile_z = e; c->kin = k; }
     }
+    const char* tr = getenv("FMM3D_TRACE");
+    if (tr) c->trace = atoi(tr);
+    const char* sg = getenv("FMM3D_STRATEGY");   // "brick" | "column[:PXxPZ]"
+    if (sg) {
+        if (!strncmp(sg, "brick", 5)) c->strategy = 0;
+        else if (!strncmp(sg, "column", 6)) {
+            c->strategy = 1;
+            int a, b;
+            int np_ = 0;
+            if (sscanf(sg, "column:%dx%d:%d", &a, &b, &np_) == 3 && np_ >= 1 && np_ <= 16) c->col_passes = np_;
+            if (sscanf(sg, "column:%dx%d", &a, &b) == 2 && (a == 8 || a == 16 || a == 32 || a == 64) &&
+                (b == 8 || b == 16 || b == 32 || b == 64)) { c->col_px = a; c->col_pz = b; }
+        }
+    }
     *out = c;
     return FMM3D_OK;
 }
@@ -824,6 +1139,8 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
     (void)hipFree(c->hx64); (void)hipFree(c->hz64); (void)hipFree(c->hx32); (void)hipFree(c->hz32);
     (void)hipFree(c->d_updates); (void)hipFree(c->d_visits); (void)hipFree(c->ray_scratch);
+    (void)hipFree(c->fr_cols); (void)hipFree(c->fr_vals); (void)hipFree(c->fr_count); (void)hipFree(c->fr_off);
+    (void)hipFree(c->fr_pcols); (void)hipFree(c->fr_pvals);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (auto& e : c->kev) if (e) (void)hipEventDestroy(e);
@@ -944,6 +1261,23 @@ __global__ void k_stamp_init(int nslots, int nb, int nbxz, int nbr, int all, int
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)nslots * nb; i += (size_t)gridDim.x * blockDim.x) {
         int t = (int)(i % nb);
         stamp[i] = all ? 0 : ((t / nbxz == nbr - 1) ? 0 : -1);
+    }
+}
+
+// Frechet rows from the per-ray staging into one packed array: ray q's
+// count[q] entries move to off[q] (one block per ray, grid-stride).
+__global__ void k_pack_rows(int nr, int64_t rcap, const int32_t* __restrict__ count, const int64_t* __restrict__ off,
+                            const int32_t* __restrict__ scols, const float* __restrict__ svals,
+                            int32_t* __restrict__ pcols, float* __restrict__ pvals)
+{
+    for (int q = blockIdx.x; q < nr; q += gridDim.x) {
+        const int n = count[q];
+        if (n <= 0) continue;
+        const int64_t src = (int64_t)q * rcap, dst = off[q];
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            pcols[dst + i] = scols[src + i];
+            pvals[dst + i] = svals[src + i];
+        }
     }
 }
 
@@ -1079,6 +1413,112 @@ static int run_phases(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     return FMM3D_OK;
 }
 
+template <typename R, bool FAST, bool SHIFT>
+static hipError_t launch_column(fmm3d_ctx* c, const ColArgs<R>& a, int blocks)
+{
+    const int px = c->col_px, pz = c->col_pz;
+#define FMM3D_COL_CASE(A, B)                                                   \
+    if (px == A && pz == B) {                                                  \
+        k_column<R, A, B, FAST, SHIFT><<<blocks, 64, 0, c->stream>>>(a);        \
+        return hipGetLastError();                                              \
+    }
+    FMM3D_COL_CASE(16, 16)
+    FMM3D_COL_CASE(16, 32)
+    FMM3D_COL_CASE(32, 16)
+    FMM3D_COL_CASE(32, 32)
+    FMM3D_COL_CASE(16, 64)
+    FMM3D_COL_CASE(64, 16)
+    FMM3D_COL_CASE(8, 64)
+    FMM3D_COL_CASE(64, 8)
+    FMM3D_COL_CASE(8, 32)
+    FMM3D_COL_CASE(32, 8)
+    FMM3D_COL_CASE(8, 16)
+    FMM3D_COL_CASE(16, 8)
+    FMM3D_COL_CASE(8, 8)
+#undef FMM3D_COL_CASE
+    return hipErrorInvalidValue;
+}
+
+template <typename R, bool FAST, bool SHIFT>
+static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
+{
+    Slots& s = c->sl;
+    const Grid& g = c->g;
+    const int npx = (g.nnx + c->col_px - 1) / c->col_px, npz = (g.nnz + c->col_pz - 1) / c->col_pz;
+    const int np = npx * npz;
+    ColArgs<R> a;
+    a.T = (R*)s.T;
+    a.slo = std::is_same<R, double>::value ? (const R*)c->slo64 : (const R*)c->slo32;
+    a.hx = std::is_same<R, double>::value ? (const R*)c->hx64 : (const R*)c->hx32;
+    a.hz = std::is_same<R, double>::value ? (const R*)c->hz64 : (const R*)c->hz32;
+    a.hr = (R)g.dnr;
+    a.fixbits = s.fix;
+    a.stamp = s.stamp;
+    a.lastchg = s.lastchg;
+    a.valchg = s.valchg;
+    a.slot_on = s.on;
+    a.nnr = g.nnr; a.nnx = g.nnx; a.nnz = g.nnz;
+    a.npx = npx; a.npz = npz;
+    a.nslots = nsrc;
+    a.fom = order == FMM3D_ORDER_MIXED;
+    a.maxpass = c->col_passes;
+    a.tol = std::is_same<R, double>::value ? c->tol64 : c->tol32;
+    a.visits = c->d_visits;
+    a.updates = c->d_visits + 1;
+    a.nodes = c->d_visits + 2;
+    const int blocks = nsrc * np;
+    const int maxit = 4 * (g.nnr + g.nnx + g.nnz) + 200;
+    std::vector<int> lc(nsrc);
+    k_stamp_init<<<std::min(4096, (blocks + 255) / 256), 256, 0, c->stream>>>(nsrc, np, 1, 1, 1, s.stamp);
+    k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.lastchg, nsrc, 0);
+    k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.on, nsrc, 1);
+    k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.valchg, nsrc, 0);
+    HIPCHK(c, hipGetLastError());
+    int k = 1, last = 0;
+    bool done = false;
+    while (!done) {
+        const int chunk = c->max_sweeps > 0 ? std::min(FMM3D_CHECK, c->max_sweeps - k + 1) : FMM3D_CHECK;
+        for (int j = 0; j < chunk; ++j, ++k) {
+            a.iter = k;
+            if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j], c->stream));
+            hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks);
+            if (le == hipErrorInvalidValue)
+                return fail(c, FMM3D_E_ARG, "solver: unsupported column patch %dx%d", c->col_px, c->col_pz);
+            HIPCHK(c, le);
+            if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j + 1], c->stream));
+            c->stats.kernel_launches++;
+            if (c->trace) {
+                unsigned long long v[2];
+                HIPCHK(c, hipMemcpyAsync(v, c->d_visits, sizeof v, hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                fprintf(stderr, "fmm3d trace: sweep %d visits %llu updates %llu\n", k, v[0], v[1]);
+            }
+        }
+        HIPCHK(c, hipMemcpyAsync(lc.data(), s.lastchg, nsrc * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->timing)
+            for (int j = 0; j < chunk; ++j) {
+                float ms = 0.0f;
+                HIPCHK(c, hipEventElapsedTime(&ms, c->kev[2 * j], c->kev[2 * j + 1]));
+                c->stats.kernel_ms += ms;
+            }
+        int mx = 0;
+        for (int q = 0; q < nsrc; ++q) mx = std::max(mx, lc[q]);
+        if (c->trace) {
+            fprintf(stderr, "fmm3d trace: lastchg");
+            for (int q = 0; q < nsrc; ++q) fprintf(stderr, " %d", lc[q]);
+            fprintf(stderr, "\n");
+        }
+        last = mx;
+        if (mx < k - 1) done = true;
+        if (!done && c->max_sweeps > 0 && k > c->max_sweeps) break;
+        if (!done && k > maxit) return fail(c, FMM3D_E_NOCONV, "solver: column walk did not converge in %d sweeps", maxit);
+    }
+    c->stats.phase_sweeps[0] = last + 1;
+    *sweeps_out = last + 1;
+    return FMM3D_OK;
+}
+
 extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int order, int precision, int flags)
 {
     if (!c) return fail(c, FMM3D_E_ARG, "fmm3d_solve_tele: null context");
@@ -1118,11 +1558,16 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     if (le != hipSuccess) { (void)hipFree(dfaces); return fail(c, FMM3D_E_DEVICE, "init: %s", hipGetErrorString(le)); }
     c->stats = fmm3d_stats{};
     c->stats.tile_r = c->brick_sl; c->stats.tile_x = c->tile_x; c->stats.tile_z = c->tile_z;
-    HIPCHK(c, hipMemsetAsync(c->d_visits, 0, 2 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_visits, 0, 3 * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int sweeps = 0;
-    if (precision == FMM3D_PREC_FP64) rc = run_phases<double, false>(c, nsrc, order, &sweeps);
-    else rc = run_phases<float, true>(c, nsrc, order, &sweeps);
+    if (c->strategy == 1) {
+        if (precision == FMM3D_PREC_FP64) rc = run_column<double, false, false>(c, nsrc, order, &sweeps);
+        else rc = run_column<float, true, true>(c, nsrc, order, &sweeps);
+    } else {
+        if (precision == FMM3D_PREC_FP64) rc = run_phases<double, false>(c, nsrc, order, &sweeps);
+        else rc = run_phases<float, true>(c, nsrc, order, &sweeps);
+    }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(dfaces);
@@ -1131,10 +1576,11 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
     c->stats.solve_ms = ms;
     c->stats.sweeps = sweeps;
-    unsigned long long vis[2] = { 0, 0 };
+    unsigned long long vis[3] = { 0, 0, 0 };
     HIPCHK(c, hipMemcpy(vis, c->d_visits, sizeof vis, hipMemcpyDeviceToHost));
     c->stats.brick_visits = (int64_t)vis[0];
     c->stats.node_updates = (int64_t)vis[1];
+    c->stats.node_visits = (int64_t)vis[2];
     s.n = nsrc;
     return FMM3D_OK;
 }
@@ -1209,54 +1655,91 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
         HIPCHK(c, hipMemsetAsync(c->ray_scratch, 0, workers * NV * sizeof(float), c->stream));
         c->ray_scratch_workers = workers;
     }
-    // rays go in chunks so the staging (NV entries per ray) stays bounded
-    const int64_t rcap = (int64_t)NV;
-    const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(nrays, (int64_t)(256ll << 20) / (rcap * 8)));
+    // Rays go in chunks whose staging (fr_rcap entries per ray) fits a fixed
+    // budget.  Per chunk: rays -> counts to the host -> row offsets -> rows
+    // packed on the device -> two copies into the caller's arrays.  A chunk
+    // with a row longer than the staging runs again with room for the whole
+    // vertex grid.
+    const size_t budget = (size_t)512 << 20;   // bytes of staging
     double* drcv = nullptr;
-    int32_t* dcols = nullptr;
-    float* dvals = nullptr;
-    int32_t* dcount = nullptr;
     HIPCHK(c, dalloc(&drcv, (size_t)nrc * 3 * sizeof(double)));
-    HIPCHK(c, dalloc(&dcols, (size_t)chunk * rcap * sizeof(int32_t)));
-    HIPCHK(c, dalloc(&dvals, (size_t)chunk * rcap * sizeof(float)));
-    HIPCHK(c, dalloc(&dcount, (size_t)chunk * sizeof(int32_t)));
     hipError_t e = hipMemcpyAsync(drcv, rcv, (size_t)nrc * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    std::vector<int32_t> cnt(chunk);
-    std::vector<int32_t> hc;
-    std::vector<float> hv;
+    std::vector<int32_t> cnt;
+    std::vector<int64_t> off;
     int64_t total = 0;
     bool short_cap = false;
-    for (int r0 = 0; r0 < nrays && e == hipSuccess; r0 += chunk) {
-        int nr = std::min(chunk, nrays - r0);
-        RayOut o{ dcols, dvals, dcount, rcap };
+    for (int r0 = 0; r0 < nrays && e == hipSuccess;) {
+        const int64_t rcap = std::min<int64_t>(c->fr_rcap, (int64_t)NV);
+        const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(nrays - r0, (int64_t)(budget / (rcap * 8))));
+        if (c->fr_stage < (size_t)chunk * rcap) {
+            (void)hipFree(c->fr_cols); (void)hipFree(c->fr_vals);
+            c->fr_cols = nullptr; c->fr_vals = nullptr; c->fr_stage = 0;
+            e = dalloc(&c->fr_cols, (size_t)chunk * rcap * sizeof(int32_t));
+            if (e == hipSuccess) e = dalloc(&c->fr_vals, (size_t)chunk * rcap * sizeof(float));
+            if (e != hipSuccess) break;
+            c->fr_stage = (size_t)chunk * rcap;
+        }
+        if (c->fr_rays < (size_t)chunk) {
+            (void)hipFree(c->fr_count); (void)hipFree(c->fr_off);
+            c->fr_count = nullptr; c->fr_off = nullptr; c->fr_rays = 0;
+            e = dalloc(&c->fr_count, (size_t)chunk * sizeof(int32_t));
+            if (e == hipSuccess) e = dalloc(&c->fr_off, (size_t)chunk * sizeof(int64_t));
+            if (e != hipSuccess) break;
+            c->fr_rays = chunk;
+        }
+        const int nr = chunk;
+        RayOut o{ c->fr_cols, c->fr_vals, c->fr_count, rcap };
         int grid = (int)std::min<size_t>(workers, (size_t)nr);
         if (c->sl.prec == FMM3D_PREC_FP64)
             k_rays<double><<<grid, 64, 0, c->stream>>>(g, (const double*)c->sl.T, c->sl.toff, c->veln, ns, nrc, drcv, r0, nr, c->ray_scratch, o);
         else
             k_rays<float><<<grid, 64, 0, c->stream>>>(g, (const float*)c->sl.T, c->sl.toff, c->veln, ns, nrc, drcv, r0, nr, c->ray_scratch, o);
         e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), dcount, nr * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+        cnt.resize(nr);
+        if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), c->fr_count, nr * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) break;
-        int64_t need = 0;
-        for (int q = 0; q < nr; ++q) {
-            if (cnt[q] == -2) { e = hipErrorUnknown; break; }
-            need = std::max<int64_t>(need, cnt[q]);
+        bool over = false;
+        for (int q = 0; q < nr; ++q) over |= cnt[q] == -2;
+        if (over) {
+            if (rcap >= (int64_t)NV) { e = hipErrorUnknown; break; }   // cannot happen: a row has at most NV entries
+            c->fr_rcap = (int64_t)NV;
+            continue;
         }
-        if (e != hipSuccess) break;
-        hc.resize(need); hv.resize(need);
+        off.resize(nr);
+        int64_t ctot = 0;
         for (int q = 0; q < nr; ++q) {
-            int64_t n = std::max(cnt[q], 0);
+            const int64_t n = std::max(cnt[q], 0);
             row_ptr[r0 + q + 1] = row_ptr[r0 + q] + n;
-            if (n == 0) continue;
-            if (total + n > cap) { short_cap = true; total += n; continue; }
-            e = hipMemcpy(cols + total, dcols + (int64_t)q * rcap, n * sizeof(int32_t), hipMemcpyDeviceToHost);
-            if (e == hipSuccess) e = hipMemcpy(vals + total, dvals + (int64_t)q * rcap, n * sizeof(float), hipMemcpyDeviceToHost);
-            if (e != hipSuccess) break;
-            total += n;
+            off[q] = ctot;
+            ctot += n;
         }
+        if (short_cap || total + ctot > cap) {
+            short_cap = true;
+        } else if (ctot > 0) {
+            if (c->fr_packed < (size_t)ctot) {
+                (void)hipFree(c->fr_pcols); (void)hipFree(c->fr_pvals);
+                c->fr_pcols = nullptr; c->fr_pvals = nullptr; c->fr_packed = 0;
+                e = dalloc(&c->fr_pcols, (size_t)ctot * sizeof(int32_t));
+                if (e == hipSuccess) e = dalloc(&c->fr_pvals, (size_t)ctot * sizeof(float));
+                if (e != hipSuccess) break;
+                c->fr_packed = ctot;
+            }
+            e = hipMemcpyAsync(c->fr_off, off.data(), nr * sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess) {
+                k_pack_rows<<<std::min(nr, 8192), 256, 0, c->stream>>>(nr, rcap, c->fr_count, c->fr_off, c->fr_cols, c->fr_vals,
+                                                                      c->fr_pcols, c->fr_pvals);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipMemcpyAsync(cols + total, c->fr_pcols, ctot * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(vals + total, c->fr_pvals, ctot * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) break;
+        }
+        total += ctot;
+        r0 += nr;
     }
-    (void)hipFree(drcv); (void)hipFree(dcols); (void)hipFree(dvals); (void)hipFree(dcount);
+    (void)hipFree(drcv);
     if (e == hipErrorUnknown) return fail(c, FMM3D_E_DEVICE, "frechet: row staging overflow");
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "frechet: %s", hipGetErrorString(e));
     if (short_cap) return fail(c, FMM3D_E_CAPACITY, "fmm3d_frechet: capacity %lld < %lld entries", (long long)cap, (long long)row_ptr[nrays]);
@@ -1285,6 +1768,18 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "direction")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "direction must be 0 or 1");
         c->dirmode = iv;
+    } else if (!strcmp(key, "strategy")) {
+        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "strategy must be 0 or 1");
+        c->strategy = iv;
+    } else if (!strcmp(key, "column_x") || !strcmp(key, "column_z")) {
+        if (iv != 8 && iv != 16 && iv != 32 && iv != 64) return fail(c, FMM3D_E_ARG, "%s must be 8, 16, 32 or 64", key);
+        (key[7] == 'x' ? c->col_px : c->col_pz) = iv;
+    } else if (!strcmp(key, "max_sweeps")) {
+        if (iv < 0) return fail(c, FMM3D_E_ARG, "max_sweeps must be >= 0");
+        c->max_sweeps = iv;
+    } else if (!strcmp(key, "column_passes")) {
+        if (iv < 1 || iv > 16) return fail(c, FMM3D_E_ARG, "column_passes must be in 1..16");
+        c->col_passes = iv;
     } else if (!strcmp(key, "timing")) {
         c->timing = iv != 0;
     } else {

@@ -40,7 +40,8 @@ class StatsC(C.Structure):
     _fields_ = [("node_updates", C.c_int64), ("sweeps", C.c_int32), ("tile_r", C.c_int32),
                 ("tile_x", C.c_int32), ("tile_z", C.c_int32), ("solve_ms", C.c_double),
                 ("kernel_ms", C.c_double), ("kernel_launches", C.c_int64),
-                ("phase_sweeps", C.c_int32 * 6), ("brick_visits", C.c_int64)]
+                ("phase_sweeps", C.c_int32 * 6), ("brick_visits", C.c_int64),
+                ("node_visits", C.c_int64)]
 
 
 class FmmError(RuntimeError):
