@@ -132,7 +132,11 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *   "column_x", "column_z"  column-walk patch (8, 16, 32 or 64 each; supported
  *                   pairs 8x8, 8x16, 16x8, 16x16, 8x32, 32x8, 16x32, 32x16,
  *                   32x32, 16x64, 64x16, 8x64, 64x8)
- *   "column_passes" line passes per layer at most (1..16, default 4)
+ *   "column_passes" line passes per layer at most (1..16, default 16)
+ *   "ordered"       column walk: 1 (default) patches start each layer after
+ *                   their upstream neighbours (along the source's lateral
+ *                   flow) finished it; 0 no ordering
+ *   "spin_limit"    polls before such a wait is given up (default 20000)
  *   "max_sweeps"    column walk: stop after this many sweeps (0 = at
  *                   convergence); a diagnostic, the field is then approximate
  *   "timing"        1: time every relaxation launch with events (stats.kernel_ms)

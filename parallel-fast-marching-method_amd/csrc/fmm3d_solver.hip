@@ -382,6 +382,9 @@ template <typename R> struct ColArgs {
     unsigned long long* visits;
     unsigned long long* updates;
     unsigned long long* nodes;   // free nodes staged
+    int* prog;              // per slot per patch: iter*(nnr+1) + layers finished, or null (no ordering)
+    const int* flow;        // per slot: lateral flow signs (fx, fz), or null
+    int spin_limit;         // polls before a wait on a neighbour is given up
 };
 
 // float32 helpers for the axis-sum octant form: fused multiply-subtract,
@@ -483,10 +486,17 @@ k_column(ColArgs<R> a)
     __shared__ int s_act;
     const int np = a.npx * a.npz;
     const int slot = blockIdx.x / np;
-    const int b = blockIdx.x - slot * np;
+    const int j = blockIdx.x - slot * np;
     if (slot >= a.nslots || !a.slot_on[slot]) return;
-    const int bx = b / a.npz, bz = b % a.npz;
+    // patches of a slot launch in the order of its lateral flow (fx, fz):
+    // a patch's upstream neighbours have lower block indices
+    const int fx = a.flow ? a.flow[2 * slot] : 1, fz = a.flow ? a.flow[2 * slot + 1] : 1;
+    const int bx = fx > 0 ? j / a.npz : a.npx - 1 - j / a.npz;
+    const int bz = fz > 0 ? j % a.npz : a.npz - 1 - j % a.npz;
+    const int b = bx * a.npz + bz;
     int* st = a.stamp + (size_t)slot * np;
+    int* pg = a.prog ? a.prog + (size_t)slot * np : nullptr;
+    const int pbase = a.iter * (a.nnr + 1);
     const int t = threadIdx.x;
     if (t == 0) {
         int m = st[b];
@@ -495,10 +505,16 @@ k_column(ColArgs<R> a)
         if (bz > 0) m = max(m, st[b - 1]);
         if (bz < a.npz - 1) m = max(m, st[b + 1]);
         s_act = (m >= a.iter - 1);
+        // an idle patch is finished with every layer of this sweep
+        if (!s_act && pg) __hip_atomic_store(pg + b, pbase + a.nnr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     if (!s_act) return;
     if (t == 0) atomicAdd(a.visits, 1ull);
+    // upstream neighbours along the flow, -1 where none
+    const int ux = (bx - fx >= 0 && bx - fx < a.npx) ? b - fx * a.npz : -1;
+    const int uz = (bz - fz >= 0 && bz - fz < a.npz) ? b - fz : -1;
+    bool wx = pg && ux >= 0, wz = pg && uz >= 0;   // still waiting on them (given up after a timeout)
     const size_t NL = (size_t)a.nnx * a.nnz;
     const size_t N = NL * a.nnr;
     R* T = a.T + (size_t)slot * N;
@@ -512,6 +528,25 @@ k_column(ColArgs<R> a)
     for (int ir = a.nnr - 1; ir >= 0; --ir) {
         const size_t lay = (size_t)ir * NL;
         unsigned nfree = 0;
+        // wait (bounded) until the upstream patches have finished this layer
+        if (wx || wz) {
+            if (t == 0) {
+                const int need = pbase + (a.nnr - ir);
+                int spins = 0;
+                while (wx && __hip_atomic_load(pg + ux, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    if (++spins > a.spin_limit) { wx = false; break; }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                spins = 0;
+                while (wz && __hip_atomic_load(pg + uz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    if (++spins > a.spin_limit) { wz = false; break; }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                s_act = (wx ? 1 : 0) | (wz ? 2 : 0);
+            }
+            __syncthreads();
+            wx = s_act & 1; wz = s_act & 2;
+        }
         // stage: interior (own values kept for the write-back), above, slowness
 #pragma unroll
         for (int k = 0; k < S; ++k) {
@@ -536,7 +571,8 @@ k_column(ColArgs<R> a)
             if (hx_ >= 2 && hx_ < PX + 2 && hz_ >= 2 && hz_ < PZ + 2) continue;
             const int gx = x0 + hx_ - 2, gz = z0 + hz_ - 2;
             const bool in = gx >= 0 && gx < a.nnx && gz >= 0 && gz < a.nnz;
-            cur[hx_ * LZ + hz_] = in ? T[lay + (size_t)gx * a.nnz + gz] : inf;
+            // neighbours' values: device-coherent loads (another XCD may have written them this sweep)
+            cur[hx_ * LZ + hz_] = in ? __hip_atomic_load(T + lay + (size_t)gx * a.nnz + gz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : inf;
         }
         for (int e = t; e < PX; e += 64) {
             const int gx = min(x0 + e, a.nnx - 1);
@@ -602,7 +638,7 @@ k_column(ColArgs<R> a)
             const R nv = cur[(lx + 2) * LZ + lz + 2];
             const R tc = orig[k];
             if (nv != tc) {
-                T[lay + (size_t)(x0 + lx) * a.nnz + z0 + lz] = nv;
+                __hip_atomic_store(T + lay + (size_t)(x0 + lx) * a.nnz + z0 + lz, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const bool fin_n = nv < inf, fin_t = tc < inf;
                 if (fin_n != fin_t || !(fabs((double)nv - (double)tc) <= a.tol * fmax(fabs((double)nv), fabs((double)tc))))
                     anyv = true;
@@ -610,7 +646,12 @@ k_column(ColArgs<R> a)
             b2[e] = b1[e];
             b1[e] = nv;
         }
+        // publish the layer once its stores have completed
         __syncthreads();
+        if (pg && t == 0) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __hip_atomic_store(pg + b, pbase + (a.nnr - ir), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
     {
         unsigned long long w = nupd, wv = nvis;
@@ -936,6 +977,9 @@ struct Slots {
     int* on = nullptr;
     double* toff = nullptr;
     int* valchg = nullptr;    // a sweep changed a value
+    int pcap = 0;             // slots P holds (allocated by the brick strategy only)
+    int* prog = nullptr;      // column walk: per slot per patch layer progress
+    int* flow = nullptr;      // column walk: per slot lateral flow signs
 };
 
 }  // namespace
@@ -962,7 +1006,9 @@ struct fmm3d_ctx {
     int dirmode = 0;               // sweep direction policy
     int strategy = 1;              // 0: brick sweeps in three phases; 1: column walk, one phase
     int col_px = 8, col_pz = 8;    // column-walk patch (theta x phi)
-    int col_passes = 4;            // line passes per layer at most
+    int col_passes = 16;           // line passes per layer at most
+    int ordered = 1;               // column walk: patches wait for their upstream neighbours
+    int spin_limit = 20000;        // polls (~64 cycles each) before such a wait is given up
     int max_sweeps = 0;            // column walk: stop after this many sweeps (0: at convergence)
     int trace = 0;                 // per-sweep counters to stderr (FMM3D_TRACE)
     int timing = 0;                // time every relaxation launch
@@ -1104,6 +1150,10 @@ extern "C" int fmm3d_create(const fmm3d_grid_desc* d, int device, fmm3d_ctx** ou
         int n = sscanf(ts, "%dx%dx%d:%d", &a, &b, &e, &k);
         if (n >= 3) { c->brick_sl = a; c->tile_x = b; c->tile_z = e; c->kin = k; }
     }
+    const char* od = getenv("FMM3D_ORDERED");
+    if (od) c->ordered = atoi(od) != 0;
+    const char* sp = getenv("FMM3D_SPIN");
+    if (sp) c->spin_limit = std::max(0, atoi(sp));
     const char* tr = getenv("FMM3D_TRACE");
     if (tr) c->trace = atoi(tr);
     const char* sg = getenv("FMM3D_STRATEGY");   // "brick" | "column[:PXxPZ]"
@@ -1126,7 +1176,7 @@ static void free_slots(Slots& s)
 {
     (void)hipFree(s.T); (void)hipFree(s.P); (void)hipFree(s.fix); (void)hipFree(s.stamp);
     (void)hipFree(s.lastchg); (void)hipFree(s.on); (void)hipFree(s.toff);
-    (void)hipFree(s.valchg);
+    (void)hipFree(s.valchg); (void)hipFree(s.prog); (void)hipFree(s.flow);
     s = Slots();
 }
 
@@ -1203,13 +1253,14 @@ static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
     // stamps for the largest brick count any supported shape gives (4-layer bricks, 4x8 patches)
     const size_t nbmax = (size_t)((c->g.nnr + 3) / 4) * ((c->g.nnx + 3) / 4) * ((c->g.nnz + 7) / 8);
     HIPCHK(c, dalloc(&s.T, (size_t)nsrc * N * es));
-    HIPCHK(c, dalloc(&s.P, (size_t)nsrc * N * es));
     HIPCHK(c, dalloc(&s.fix, (size_t)nsrc * ((N + 31) / 32) * sizeof(uint32_t)));
     HIPCHK(c, dalloc(&s.stamp, (size_t)nsrc * nbmax * sizeof(int)));
     HIPCHK(c, dalloc(&s.lastchg, nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.on, nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.toff, nsrc * sizeof(double)));
     HIPCHK(c, dalloc(&s.valchg, nsrc * sizeof(int)));
+    HIPCHK(c, dalloc(&s.prog, (size_t)nsrc * nbmax * sizeof(int)));
+    HIPCHK(c, dalloc(&s.flow, (size_t)nsrc * 2 * sizeof(int)));
     s.cap = nsrc;
     s.prec = prec;
     return FMM3D_OK;
@@ -1324,6 +1375,13 @@ static int run_phases(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     const size_t nbmax = (size_t)((g.nnr + 3) / 4) * ((g.nnx + 3) / 4) * ((g.nnz + 7) / 8);
     if (B.sl < 4 || (size_t)nb > nbmax)
         return fail(c, FMM3D_E_ARG, "solver: brick shape %dx%dx%d not supported on this grid", B.sl, c->tile_x, c->tile_z);
+    if (s.pcap < nsrc) {
+        (void)hipFree(s.P);
+        s.P = nullptr;
+        s.pcap = 0;
+        HIPCHK(c, dalloc(&s.P, (size_t)s.cap * c->N * sizeof(R)));
+        s.pcap = s.cap;
+    }
     SweepArgs<R> a;
     a.T = (R*)s.T;
     a.P = (const R*)s.P;
@@ -1446,6 +1504,9 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     const Grid& g = c->g;
     const int npx = (g.nnx + c->col_px - 1) / c->col_px, npz = (g.nnz + c->col_pz - 1) / c->col_pz;
     const int np = npx * npz;
+    const size_t nbmax = (size_t)((g.nnr + 3) / 4) * ((g.nnx + 3) / 4) * ((g.nnz + 7) / 8);
+    if ((size_t)np > nbmax)
+        return fail(c, FMM3D_E_ARG, "solver: %d column patches exceed the per-slot tables (%zu)", np, nbmax);
     ColArgs<R> a;
     a.T = (R*)s.T;
     a.slo = std::is_same<R, double>::value ? (const R*)c->slo64 : (const R*)c->slo32;
@@ -1466,7 +1527,14 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     a.visits = c->d_visits;
     a.updates = c->d_visits + 1;
     a.nodes = c->d_visits + 2;
+    a.prog = c->ordered ? s.prog : nullptr;
+    a.flow = s.flow;
+    a.spin_limit = c->spin_limit;
     const int blocks = nsrc * np;
+    if (c->ordered) {
+        k_fill_int<<<(nsrc * np + 255) / 256, 256, 0, c->stream>>>(s.prog, nsrc * np, 0);
+        HIPCHK(c, hipGetLastError());
+    }
     const int maxit = 4 * (g.nnr + g.nnx + g.nnz) + 200;
     std::vector<int> lc(nsrc);
     k_stamp_init<<<std::min(4096, (blocks + 255) / 256), 256, 0, c->stream>>>(nsrc, np, 1, 1, 1, s.stamp);
@@ -1519,6 +1587,33 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     return FMM3D_OK;
 }
 
+// Lateral flow direction of a teleseismic source for the column walk's
+// patch order: the signs of the (theta, phi) index gradient of a
+// least-squares plane through the bottom face's known times (+1 when the
+// times grow with the index, the wave then travels that way).  Only the
+// order in which the relaxation visits patches depends on this.
+static void lateral_flow(const Grid& g, const double* face, int out[2])
+{
+    const int nx = g.nnx, nz = g.nnz;
+    double n = 0, sx = 0, sz = 0, sxx = 0, szz = 0, sxz = 0, st = 0, sxt = 0, szt = 0;
+    for (int ix = 0; ix < nx; ++ix)
+        for (int iz = 0; iz < nz; ++iz) {
+            const double v = face[(size_t)ix * nz + iz];
+            if (!(v > 0.0)) continue;
+            n += 1; sx += ix; sz += iz; sxx += (double)ix * ix; szz += (double)iz * iz; sxz += (double)ix * iz;
+            st += v; sxt += ix * v; szt += iz * v;
+        }
+    double gx = 0.0, gz = 0.0;
+    if (n >= 3) {
+        const double cxx = sxx - sx * sx / n, czz = szz - sz * sz / n, cxz = sxz - sx * sz / n;
+        const double cxt = sxt - sx * st / n, czt = szt - sz * st / n;
+        const double det = cxx * czz - cxz * cxz;
+        if (det > 0.0) { gx = (cxt * czz - czt * cxz) / det; gz = (czt * cxx - cxt * cxz) / det; }
+    }
+    out[0] = gx >= 0.0 ? 1 : -1;
+    out[1] = gz >= 0.0 ? 1 : -1;
+}
+
 extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int order, int precision, int flags)
 {
     if (!c) return fail(c, FMM3D_E_ARG, "fmm3d_solve_tele: null context");
@@ -1554,6 +1649,11 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     else
         k_init_tele<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff);
     k_fixbits_tele<<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, s.fix);
+    if (c->strategy == 1) {
+        std::vector<int> flow((size_t)nsrc * 2);
+        for (int q = 0; q < nsrc; ++q) lateral_flow(g, faces + (size_t)q * F, &flow[2 * q]);
+        HIPCHK(c, hipMemcpyAsync(s.flow, flow.data(), flow.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) { (void)hipFree(dfaces); return fail(c, FMM3D_E_DEVICE, "init: %s", hipGetErrorString(le)); }
     c->stats = fmm3d_stats{};
@@ -1774,6 +1874,12 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "column_x") || !strcmp(key, "column_z")) {
         if (iv != 8 && iv != 16 && iv != 32 && iv != 64) return fail(c, FMM3D_E_ARG, "%s must be 8, 16, 32 or 64", key);
         (key[7] == 'x' ? c->col_px : c->col_pz) = iv;
+    } else if (!strcmp(key, "ordered")) {
+        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "ordered must be 0 or 1");
+        c->ordered = iv;
+    } else if (!strcmp(key, "spin_limit")) {
+        if (iv < 0 || iv > 10000000) return fail(c, FMM3D_E_ARG, "spin_limit out of range");
+        c->spin_limit = iv;
     } else if (!strcmp(key, "max_sweeps")) {
         if (iv < 0) return fail(c, FMM3D_E_ARG, "max_sweeps must be >= 0");
         c->max_sweeps = iv;
