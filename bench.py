@@ -47,7 +47,23 @@ def parse():
     p.add_argument("--plain", type=int, default=3, help="float64 single-source solves timed")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-sources", type=int, default=1)
+    p.add_argument("--copy-mib", type=int, default=0,
+                   help="first run one device-to-device copy of this many MiB (the reference kernel "
+                        "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return p.parse_args()
+
+
+def copy_calibration(mib):
+    """One plain device copy of a known size, untimed: under rocprofv3 --pmc its
+    FETCH_SIZE and WRITE_SIZE give the counters' scale."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
 
 
 def one_step(ctx, faces, rcv, prec):
@@ -80,6 +96,8 @@ def main():
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (sources shard across GPUs "
               f"with no exchange, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_mib > 0:
+        copy_calibration(a.copy_mib)
     g, velv, srcs, rcv_deg = synth.config_201(nsrc=a.batch, nrc=a.nrc, seed=2)
     rcv = synth.receivers_rad(rcv_deg)
     faces = np.stack([synth.tele_bottom_face(g, s) for s in srcs])
@@ -120,7 +138,7 @@ def main():
         s0 = time.perf_counter()
         one_step(ctx, faces[s:s + 1], rcv, lib.PREC_FP64)
         plain_t.append(time.perf_counter() - s0)
-    plain = 1.0 / float(np.mean(plain_t))
+    plain = 1.0 / float(np.mean(plain_t)) if plain_t else None
 
     avg_launch_ms = kms / max(launches, 1)
     bytes_per_launch = BYTES_PER_NODE_VISIT * nvis / max(launches, 1)
@@ -142,8 +160,8 @@ def main():
                                f"{a.batch} plane-wave sources per batch (bottom injection), {a.nrc} receivers, "
                                f"mixed-order stencils, receiver times + rays/Frechet rows",
                    "grid": [g.nnr, g.nnx, g.nnz], "batch": a.batch, "receivers": a.nrc},
-        "plain_fp64_source_solves_per_s": round(plain, 3),
-        "ratio_vs_plain_fp64": round(value / plain, 2),
+        "plain_fp64_source_solves_per_s": round(plain, 3) if plain else None,
+        "ratio_vs_plain_fp64": round(value / plain, 2) if plain else None,
         "max_receiver_err_s_vs_fp64": err,
         "latency_ms_per_batch": {"p50": round(float(np.percentile(lat, 50)), 2),
                                  "p99": round(float(np.percentile(lat, 99)), 2)},

@@ -485,14 +485,21 @@ k_column(ColArgs<R> a)
     __shared__ R shz[PX], sihz[PX];
     __shared__ int s_act;
     const int np = a.npx * a.npz;
-    const int slot = blockIdx.x / np;
-    const int j = blockIdx.x - slot * np;
-    if (slot >= a.nslots || !a.slot_on[slot]) return;
-    // patches of a slot launch in the order of its lateral flow (fx, fz):
-    // a patch's upstream neighbours have lower block indices
+    // Block -> patch.  A row of patches (one slot, one theta index, every phi
+    // index) runs on one XCD -- workgroups go round-robin over the 8 XCDs --
+    // so patches that share cache lines share an L2; rows go round-robin
+    // over the XCDs.  Inside a slot, rows and the patches of a row follow
+    // the slot's lateral flow (fx, fz), so a patch's upstream neighbours have
+    // lower block indices.
+    const int xcd = blockIdx.x & 7, k = blockIdx.x >> 3;
+    const int row = (k / a.npz) * 8 + xcd;
+    if (row >= a.nslots * a.npx) return;
+    const int slot = row / a.npx;
+    if (!a.slot_on[slot]) return;
     const int fx = a.flow ? a.flow[2 * slot] : 1, fz = a.flow ? a.flow[2 * slot + 1] : 1;
-    const int bx = fx > 0 ? j / a.npz : a.npx - 1 - j / a.npz;
-    const int bz = fz > 0 ? j % a.npz : a.npz - 1 - j % a.npz;
+    const int rx = row - slot * a.npx, rz = k % a.npz;
+    const int bx = fx > 0 ? rx : a.npx - 1 - rx;
+    const int bz = fz > 0 ? rz : a.npz - 1 - rz;
     const int b = bx * a.npz + bz;
     int* st = a.stamp + (size_t)slot * np;
     int* pg = a.prog ? a.prog + (size_t)slot * np : nullptr;
@@ -584,6 +591,8 @@ k_column(ColArgs<R> a)
         __syncthreads();
         // the passes stop after one that changes nothing: the layer is then
         // at a fixpoint of the update and later passes would not move it
+        // (stopping at a tolerance instead leaves second-order transients
+        // that keep neighbouring patches oscillating)
         int npass = 0;
 #pragma unroll 1
         for (int pass = 0; pass < a.maxpass; ++pass) {
@@ -1530,7 +1539,7 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     a.prog = c->ordered ? s.prog : nullptr;
     a.flow = s.flow;
     a.spin_limit = c->spin_limit;
-    const int blocks = nsrc * np;
+    const int blocks = 8 * ((nsrc * npx + 7) / 8) * npz;   // rows of patches round-robin over 8 XCDs
     if (c->ordered) {
         k_fill_int<<<(nsrc * np + 255) / 256, 256, 0, c->stream>>>(s.prog, nsrc * np, 0);
         HIPCHK(c, hipGetLastError());
@@ -1746,7 +1755,7 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
     if (nrays == 0) return FMM3D_OK;
     const size_t NV = (size_t)(g.nvr + 2) * (g.nvx + 2) * (g.nvz + 2);
     // ray workers: one wavefront each, with a dense float32 scratch of NV
-    const size_t workers = std::min<size_t>(4096, (size_t)nrays);
+    const size_t workers = std::min<size_t>(16384, (size_t)nrays);
     if (c->ray_scratch_workers < workers) {
         (void)hipFree(c->ray_scratch);
         c->ray_scratch = nullptr;
