@@ -47,6 +47,7 @@ extern "C" {
 #define FMM3D_ORDER_MIXED   1   /* mixed first/second order (.in line 9 = 1) */
 #define FMM3D_PREC_FP64     0   /* float64 node updates                     */
 #define FMM3D_PREC_FP32     1   /* float32 updates on times offset by the source's boundary minimum */
+#define FMM3D_WALK_ALTERNATE 1  /* flag: column walk alternates bottom-up / top-down (point sources) */
 
 typedef struct {
     int nvr, nvx, nvz;          /* vertex counts, border excluded          */
@@ -97,6 +98,30 @@ int fmm3d_set_model_vtx(fmm3d_ctx *ctx, const double *velv);
  * receiver and Frechet calls.  flags: 0. */
 int fmm3d_solve_tele(fmm3d_ctx *ctx, int nsrc, const double *faces, int order,
                      int precision, int flags);
+
+/* Solves from general fixed (Dirichlet) node sets, one per source: the
+ * generalisation of the bottom-face injection above to any nodes, e.g.
+ * bottom and side faces of a teleseismic plane wave or the refined
+ * neighbourhood of a point source.  No counterpart in the reference, which
+ * injects bottom faces only (fm3dt_parallel.f90:6894-6961); the node update
+ * and the relaxed discrete system are the same.  Source s owns entries
+ * offs[s] .. offs[s+1]-1 of node (0-based index (ir*nnx + ix)*nnz + iz)
+ * and time (seconds, finite, >= 0).  flags: 0, or FMM3D_WALK_ALTERNATE
+ * when waves travel downward too (point sources). */
+int fmm3d_solve_fixed(fmm3d_ctx *ctx, int nsrc, const int64_t *offs, const int64_t *node,
+                      const double *time, int order, int precision, int flags);
+
+/* Point (local) sources with a refined source neighbourhood; no
+ * counterpart in the reference (teleseismic only).  src: nsrc records of
+ * (height km, colatitude rad, longitude rad).  Around each source the
+ * coarse cell holding it, widened by `radius` nodes per side, is refined by
+ * `refine` per axis; the sub-grid gets B-spline velocities, straight-ray
+ * times on the 8 corners of the fine cell holding the source and a float64
+ * solve; the coarse nodes it covers then seed the coarse solve
+ * (FMM3D_WALK_ALTERNATE).  Returns FMM3D_E_ARG for a source outside the
+ * grid.  flags: 0. */
+int fmm3d_solve_point(fmm3d_ctx *ctx, int nsrc, const double *src, int refine, int radius,
+                      int order, int precision, int flags);
 
 /* Receiver times for every resident source (srtimes,
  * fm3dt_parallel.f90:7040-7248): t_out is nsrc*nrc float32, source-major.

@@ -16,6 +16,9 @@
  *     local update  mixed-order (fouds2_all) / first-order
  *                   (fouds1_all) octant quadratic            F:535-1004
  *   or_fim_sweep    one value-based (FIM-form) sweep          F:1100-1400
+ *   or_fmm_fixed, or_relax1_fixed, or_point_region
+ *                   general fixed sets and point sources with a
+ *                   refined neighbourhood (build-only, no reference)
  *   or_srtimes      trilinear receiver times, float32 sum    F:7040-7248
  *   or_rpaths       steepest-descent rays + Frechet rows     F:7250-7777
  *
@@ -921,6 +924,185 @@ long or_relax1_tele(const or_grid *g, const double *veln, const double *face, in
     return sweeps;
 }
 
+
+/* ------------------------------------------- general Dirichlet sets, point sources */
+
+/* Textbook narrow-band march from a fixed (Dirichlet) node set: every
+ * fixed node is alive with its given time and seeds the band through its
+ * six neighbours; then the march of F:78-155 with the update of
+ * F:535-1004.  (The reference injects only bottom faces and seeds at the
+ * face minimum, F:6894-6961; with a fully known face both give the same
+ * field.)  fixed/tfix: (nnr, nnx, nnz) phi fastest.  Returns alive nodes. */
+long or_fmm_fixed(const or_grid *g, const double *veln, const unsigned char *fixed, const double *tfix,
+                  int fom, double *ttn)
+{
+    dom_t D;
+    int n[3] = { g->nnr, g->nnx, g->nnz }, off[3] = { 1, 1, 1 };
+    dom_init(&D, g, n, off, fom, NULL, veln);
+    dom_reset(&D);
+    for (int ir = 1; ir <= n[0]; ++ir)
+        for (int ix = 1; ix <= n[1]; ++ix)
+            for (int iz = 1; iz <= n[2]; ++iz) {
+                size_t k = ((size_t)(ir - 1) * n[1] + (ix - 1)) * n[2] + (iz - 1);
+                if (!fixed[k]) continue;
+                size_t d = DIDX(&D, ir, ix, iz);
+                D.t[d] = tfix[k];
+                D.st[d] = 0;
+            }
+    for (int ir = 1; ir <= n[0]; ++ir)
+        for (int ix = 1; ix <= n[1]; ++ix)
+            for (int iz = 1; iz <= n[2]; ++iz) {
+                size_t k = ((size_t)(ir - 1) * n[1] + (ix - 1)) * n[2] + (iz - 1);
+                if (!fixed[k]) continue;
+                int c[3] = { ir, ix, iz };
+                for (int a = 0; a < 3; ++a)
+                    for (int dd = -1; dd <= 1; dd += 2) {
+                        int p[3] = { c[0], c[1], c[2] };
+                        p[a] += dd;
+                        if (p[a] < 1 || p[a] > n[a]) continue;
+                        int nb = (int)DIDX(&D, p[0], p[1], p[2]);
+                        if (D.st[nb] == 0) continue;
+                        double T = fmm_update(&D, p[0], p[1], p[2]);
+                        if (D.t[nb] == 0.0 || D.t[nb] > T) D.t[nb] = T;
+                        if (D.st[nb] < 0) heap_push(&D, nb);
+                        else heap_up(&D, D.st[nb]);
+                    }
+            }
+    march(&D);
+    dom_extract(&D, ttn);
+    long alive = 0;
+    for (int ir = 1; ir <= n[0]; ++ir)
+        for (int ix = 1; ix <= n[1]; ++ix)
+            for (int iz = 1; iz <= n[2]; ++iz) alive += D.st[DIDX(&D, ir, ix, iz)] == 0;
+    dom_free(&D);
+    return alive;
+}
+
+/* The single-phase column relaxation (see or_relax1_tele) for a general
+ * fixed set; with `alternate` the layer walk goes bottom to top on odd
+ * sweeps and top to bottom on even ones.  Unknown nodes start at +inf.
+ * Returns the number of sweeps or -1. */
+long or_relax1_fixed(const or_grid *g, const double *veln, const unsigned char *fixed, const double *tfix,
+                     int fom, int alternate, double *ttn)
+{
+    const int nr = g->nnr, nx = g->nnx, nz = g->nnz;
+    const size_t N = (size_t)nr * nx * nz;
+    double *sth = malloc((nx + 2) * sizeof(double));
+    double *ri = malloc((nr + 2) * sizeof(double));
+    metric_tables(g, ri, sth);
+    for (size_t i = 0; i < N; ++i) ttn[i] = fixed[i] ? tfix[i] : INFINITY;
+    long sweeps = -1;
+    for (int it = 1; it <= 100000; ++it) {
+        int moved = 0;
+        const int down = alternate && !(it & 1);
+        for (int q = 0; q < nr; ++q) {
+            const int ir = down ? 1 + q : nr - q;
+            for (int pass = 0; pass < 4; ++pass)
+                for (int a = 0; a < (pass < 2 ? nz : nx); ++a)
+                    for (int b = 0; b < (pass < 2 ? nx : nz); ++b) {
+                        int ix, iz;
+                        if (pass < 2) { iz = 1 + (pass == 0 ? a : nz - 1 - a); ix = 1 + b; }
+                        else { ix = 1 + (pass == 2 ? a : nx - 1 - a); iz = 1 + b; }
+                        size_t k = ((size_t)(ir - 1) * nx + (ix - 1)) * nz + (iz - 1);
+                        if (fixed[k]) continue;
+                        double nv = relax_node(g, veln, ri, sth, fom, ttn, ttn, ir, ix, iz);
+                        if (nv != ttn[k]) { ttn[k] = nv; moved = 1; }
+                    }
+        }
+        if (!moved) { sweeps = it; break; }
+    }
+    free(sth); free(ri);
+    return sweeps;
+}
+
+/* Cartesian position of (height km, colat, lon) */
+static void cart(const or_grid *g, double h, double th, double ph, double x[3])
+{
+    double r = g->earth + h;
+    x[0] = r * sin(th) * cos(ph);
+    x[1] = r * sin(th) * sin(ph);
+    x[2] = r * cos(th);
+}
+
+/* Refined source neighbourhood for a point source (build-only: the
+ * reference solves teleseismic injection only).  The coarse cell holding
+ * the source, widened by `radius` nodes per side (clipped to the grid), is
+ * refined by `refine` per axis.  On that sub-grid: node velocities by the
+ * B-spline gridder; the 8 corners of the fine cell holding the source get
+ * the straight-ray time |x - x_s| times the mean of the source and node
+ * slowness; a heap march (`solve` 0) or the column relaxation (`solve` 1,
+ * alternating walks) fills the sub-grid.  The coarse nodes it covers take
+ * the fine times and become the fixed set of the coarse solve.
+ * src: height km, colat rad, lon rad.  Outputs: lo/hi coarse node range of
+ * the refined block (0-based, inclusive) and the coarse fixed set/times;
+ * returns the number of fine nodes or -1 when the source lies outside. */
+long or_point_region(const or_grid *g, const double *velv, const double src[3], int refine, int radius,
+                     int fom, int solve, int lo[3], int hi[3], unsigned char *cfixed, double *ctfix)
+{
+    const int nn[3] = { g->nnr, g->nnx, g->nnz };
+    double u[3] = { (g->rgor - src[0]) / g->dnr, (src[1] - g->rgox) / g->dnx, (src[2] - g->rgoz) / g->dnz };
+    int c[3];
+    for (int a = 0; a < 3; ++a) {
+        if (!(u[a] >= 0.0 && u[a] <= nn[a] - 1)) return -1;
+        c[a] = (int)floor(u[a]);
+        if (c[a] > nn[a] - 2) c[a] = nn[a] - 2;
+        lo[a] = c[a] - radius < 0 ? 0 : c[a] - radius;
+        hi[a] = c[a] + 1 + radius > nn[a] - 1 ? nn[a] - 1 : c[a] + 1 + radius;
+    }
+    or_grid f = *g;
+    f.nnr = (hi[0] - lo[0]) * refine + 1;
+    f.nnx = (hi[1] - lo[1]) * refine + 1;
+    f.nnz = (hi[2] - lo[2]) * refine + 1;
+    f.rgor = g->rgor - lo[0] * g->dnr;
+    f.rgox = g->rgox + lo[1] * g->dnx;
+    f.rgoz = g->rgoz + lo[2] * g->dnz;
+    f.dnr = g->dnr / refine; f.dnx = g->dnx / refine; f.dnz = g->dnz / refine;
+    const size_t FN = (size_t)f.nnr * f.nnx * f.nnz;
+    double *fveln = malloc((size_t)(f.nnr + 3) * (f.nnx + 3) * (f.nnz + 3) * sizeof(double));
+    or_gridder(&f, velv, fveln);
+    unsigned char *ffix = calloc(FN, 1);
+    double *ftf = calloc(FN, sizeof(double)), *ft = malloc(FN * sizeof(double));
+    double fu[3] = { (f.rgor - src[0]) / f.dnr, (src[1] - f.rgox) / f.dnx, (src[2] - f.rgoz) / f.dnz };
+    const int fn[3] = { f.nnr, f.nnx, f.nnz };
+    int fc[3];
+    for (int a = 0; a < 3; ++a) {
+        fc[a] = (int)floor(fu[a]);
+        if (fc[a] < 0) fc[a] = 0;
+        if (fc[a] > fn[a] - 2) fc[a] = fn[a] - 2;
+    }
+    double xs[3];
+    cart(g, src[0], src[1], src[2], xs);
+    const double ss = 1.0 / node_velocity(g, velv, src[0], src[1], src[2], g->gor - src[0]);
+    const int MX = f.nnx + 3, MZ = f.nnz + 3;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 2; ++k) {
+                int r = fc[0] + i, x = fc[1] + j, z = fc[2] + k;
+                double xn[3];
+                cart(g, f.rgor - r * f.dnr, f.rgox + x * f.dnx, f.rgoz + z * f.dnz, xn);
+                double d = sqrt((xn[0] - xs[0]) * (xn[0] - xs[0]) + (xn[1] - xs[1]) * (xn[1] - xs[1]) +
+                                (xn[2] - xs[2]) * (xn[2] - xs[2]));
+                double sn = 1.0 / fveln[((size_t)(r + 2) * MX + (x + 2)) * MZ + (z + 2)];   /* node r: kr = r+1 */
+                size_t q = ((size_t)r * f.nnx + x) * f.nnz + z;
+                ffix[q] = 1;
+                ftf[q] = d * 0.5 * (ss + sn);
+            }
+    if (solve == 0) or_fmm_fixed(&f, fveln, ffix, ftf, fom, ft);
+    else (void)or_relax1_fixed(&f, fveln, ffix, ftf, fom, 1, ft);
+    const size_t N = (size_t)g->nnr * g->nnx * g->nnz;
+    memset(cfixed, 0, N);
+    for (int r = lo[0]; r <= hi[0]; ++r)
+        for (int x = lo[1]; x <= hi[1]; ++x)
+            for (int z = lo[2]; z <= hi[2]; ++z) {
+                size_t k = ((size_t)r * g->nnx + x) * g->nnz + z;
+                size_t q = ((size_t)(r - lo[0]) * refine * f.nnx + (size_t)(x - lo[1]) * refine) * f.nnz + (size_t)(z - lo[2]) * refine;
+                cfixed[k] = 1;
+                ctfix[k] = ft[q];
+            }
+    free(fveln); free(ffix); free(ftf); free(ft);
+    return (long)FN;
+}
+
 /* ------------------------------------------------------------- receivers */
 
 /* Trilinear receiver times (F:7063-7106).  rcv: (nrc, 3) = height km,
@@ -986,9 +1168,30 @@ static double tri_vel(const or_grid *g, const double *veln, int ipr, int ipx, in
  * vals; a receiver that starts outside the grid gets an empty row and
  * row_ok[i]=0.  cap bounds cols/vals.  Returns total entries, or -1 if cap
  * is too small.  steps_out (optional) receives the step count per ray. */
+static long rpaths(const or_grid *g, const double *veln, const double *ttn, int nrc,
+                   const double *rcv, long *row_ptr, int *cols, float *vals, long cap,
+                   int *row_ok, int *steps_out, const double *src);
+
 long or_rpaths(const or_grid *g, const double *veln, const double *ttn, int nrc,
                const double *rcv, long *row_ptr, int *cols, float *vals, long cap,
                int *row_ok, int *steps_out)
+{
+    return rpaths(g, veln, ttn, nrc, rcv, row_ptr, cols, vals, cap, row_ok, steps_out, NULL);
+}
+
+/* The same rays for a point source (build-only): a ray within one step of
+ * the source (height km, colat, lon) ends with a step to the source, whose
+ * Frechet increments use that step's true length. */
+long or_rpaths_point(const or_grid *g, const double *veln, const double *ttn, int nrc,
+                     const double *rcv, long *row_ptr, int *cols, float *vals, long cap,
+                     int *row_ok, int *steps_out, const double *src)
+{
+    return rpaths(g, veln, ttn, nrc, rcv, row_ptr, cols, vals, cap, row_ok, steps_out, src);
+}
+
+static long rpaths(const or_grid *g, const double *veln, const double *ttn, int nrc,
+                   const double *rcv, long *row_ptr, int *cols, float *vals, long cap,
+                   int *row_ok, int *steps_out, const double *src)
 {
     const int nx = g->nnx, nz = g->nnz;
     const int NVR = g->nvr + 2, NVX = g->nvx + 2, NVZ = g->nvz + 2;
@@ -1019,6 +1222,11 @@ long or_rpaths(const or_grid *g, const double *veln, const double *ttn, int nrc,
         row_ok[i] = !sw;
         if (sw) { row_ptr[i + 1] = nent; continue; }
         double pr = r0, px = x0, pz = z0;
+        double xsrc[3] = { 0.0, 0.0, 0.0 };
+        if (src) {
+            const double rr = g->earth + src[0];
+            xsrc[0] = rr * sin(src[1]) * cos(src[2]); xsrc[1] = rr * sin(src[1]) * sin(src[2]); xsrc[2] = rr * cos(src[1]);
+        }
         int step;
         for (step = 1; step <= maxrp; ++step) {
             /* gradient on the cell (F:7348-7378) */
@@ -1028,7 +1236,10 @@ long or_rpaths(const or_grid *g, const double *veln, const double *ttn, int nrc,
             for (int q = 0; q < 4; ++q) {
                 double lo = TT(ipr + 1, ipx + cx[q], ipz + cz[q]);
                 double hi = TT(ipr, ipx + cx[q], ipz + cz[q]);
-                if (lo < hi) { dtr = dtr + lo - hi; cnt++; }
+                /* teleseismic: only edges where t decreases downward (the
+                 * reference's rule); a point source: every edge, since rays
+                 * climb towards a source above them */
+                if (src || lo < hi) { dtr = dtr + lo - hi; cnt++; }
             }
             if (cnt) dtr = dtr / (cnt * g->dnr);
             double dtx = TT(ipr, ipx + 1, ipz) - TT(ipr, ipx, ipz);
@@ -1047,14 +1258,26 @@ long or_rpaths(const or_grid *g, const double *veln, const double *ttn, int nrc,
             double nr_ = pr + dpl * dtr / gn;
             double nx_ = px - dpl * dtx / (crad * gn);
             double nz_ = pz - dpl * dtz / (crad * sin(px) * gn);
+            double seg = dpl;
+            int done = 0;
+            if (src) {
+                const double sp = sin(px), cp = cos(px);
+                const double d0 = crad * sp * cos(pz) - xsrc[0], d1 = crad * sp * sin(pz) - xsrc[1], d2 = crad * cp - xsrc[2];
+                const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                if (d <= dpl || !(gn > 0.0)) { nr_ = src[0]; nx_ = src[1]; nz_ = src[2]; seg = d; done = 1; }
+            }
             int ipro = ipr, ipxo = ipx, ipzo = ipz;
             ipr = (int)floor((g->rgor - nr_) / g->dnr) + 1;
             ipx = (int)floor((nx_ - g->rgox) / g->dnx) + 1;
             ipz = (int)floor((nz_ - g->rgoz) / g->dnz) + 1;
-            int exr = (ipr < 1 || ipr >= g->nnr);
-            int exx = (ipx < 1 || ipx >= g->nnx);
-            int exz = (ipz < 1 || ipz >= g->nnz);
-            int done = 0;
+            if (done) {
+                ipr = clampi(ipr, 1, g->nnr - 1);
+                ipx = clampi(ipx, 1, g->nnx - 1);
+                ipz = clampi(ipz, 1, g->nnz - 1);
+            }
+            int exr = !done && (ipr < 1 || ipr >= g->nnr);
+            int exx = !done && (ipx < 1 || ipx >= g->nnx);
+            int exz = !done && (ipz < 1 || ipz >= g->nnz);
             if (exr || exx || exz) {
                 /* clip to the first face crossed (F:7463-7538) */
                 int which = 0;
@@ -1150,7 +1373,7 @@ long or_rpaths(const or_grid *g, const double *veln, const double *ttn, int nrc,
                 bspline4(((g->gor - sr_) - (ivrt - 1) * g->dvr) / g->dvr, ui);
                 bspline4(((sx_ - g->gox) - (ivxt - 1) * g->dvx) / g->dvx, vi);
                 bspline4(((sz_ - g->goz) - (ivzt - 1) * g->dvz) / g->dvz, wi);
-                double dinc = (k == 0) ? vrat[0] * dpl : (vrat[k] - vrat[k - 1]) * dpl;
+                double dinc = (k == 0) ? vrat[0] * seg : (vrat[k] - vrat[k - 1]) * seg;
                 /* trapezoid of w/v^2 at both sub-segment ends, added to the
                  * 64 taps of the current vertex cell (F:7717-7732) */
                 for (int n = 0; n < 4; ++n)

@@ -59,6 +59,13 @@ def lib():
         L.or_relax_tele.restype = C.c_long
         L.or_relax1_tele.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, dp]
         L.or_relax1_tele.restype = C.c_long
+        L.or_fmm_fixed.argtypes = [C.POINTER(OrGrid), dp, up, dp, C.c_int, dp]
+        L.or_fmm_fixed.restype = C.c_long
+        L.or_relax1_fixed.argtypes = [C.POINTER(OrGrid), dp, up, dp, C.c_int, C.c_int, dp]
+        L.or_relax1_fixed.restype = C.c_long
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.or_point_region.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, up, dp]
+        L.or_point_region.restype = C.c_long
         L.or_fim_sweep.argtypes = [C.POINTER(OrGrid), dp, up, C.c_int, dp, dp]
         L.or_fim_sweep.restype = C.c_double
         L.or_srtimes.argtypes = [C.POINTER(OrGrid), dp, C.c_int, dp, fp]
@@ -106,6 +113,46 @@ def relax_tele(g, veln: np.ndarray, face: np.ndarray, fom: int = 1) -> np.ndarra
     return ttn
 
 
+def fmm_fixed(g, veln, fixed, tfix, fom=1):
+    """Heap march from a general fixed node set (every fixed node alive)."""
+    ttn = np.zeros((g.nnr, g.nnx, g.nnz))
+    lib().or_fmm_fixed(C.byref(or_grid(g)), veln, np.ascontiguousarray(fixed, dtype=np.uint8),
+                       np.ascontiguousarray(tfix, dtype=np.float64), fom, ttn)
+    return ttn
+
+
+def relax1_fixed(g, veln, fixed, tfix, fom=1, alternate=True):
+    """Single-phase column relaxation from a general fixed set; (field, sweeps)."""
+    ttn = np.zeros((g.nnr, g.nnx, g.nnz))
+    n = lib().or_relax1_fixed(C.byref(or_grid(g)), veln, np.ascontiguousarray(fixed, dtype=np.uint8),
+                              np.ascontiguousarray(tfix, dtype=np.float64), fom, int(alternate), ttn)
+    if n < 0:
+        raise RuntimeError("or_relax1_fixed: no convergence")
+    return ttn, n
+
+
+def point_region(g, velv, src, refine=5, radius=10, fom=1, solve=0):
+    """Refined neighbourhood of a point source (height km, colat rad, lon rad):
+    returns (lo, hi, fixed, tfix) -- the coarse fixed set the refined solve
+    hands to the coarse grid.  solve 0: heap march, 1: column relaxation."""
+    lo = np.zeros(3, np.int32); hi = np.zeros(3, np.int32)
+    fixed = np.zeros((g.nnr, g.nnx, g.nnz), np.uint8)
+    tfix = np.zeros((g.nnr, g.nnx, g.nnz))
+    n = lib().or_point_region(C.byref(or_grid(g)), np.ascontiguousarray(velv, dtype=np.float64),
+                              np.ascontiguousarray(src, dtype=np.float64), refine, radius, fom, solve,
+                              lo, hi, fixed, tfix)
+    if n < 0:
+        raise ValueError("point source outside the grid")
+    return lo, hi, fixed, tfix
+
+
+def point_source(g, velv, veln, src, refine=5, radius=10, fom=1):
+    """Point-source field by the heap march: refined neighbourhood, then the
+    coarse grid from the transferred nodes."""
+    lo, hi, fixed, tfix = point_region(g, velv, src, refine, radius, fom, 0)
+    return fmm_fixed(g, veln, fixed, tfix, fom)
+
+
 def relax1_tele(g, veln: np.ndarray, face: np.ndarray, fom: int = 1):
     """Single-phase value-based relaxation in column-walk order (the column
     strategy's system) to its fixed point; returns (field, sweeps)."""
@@ -130,8 +177,9 @@ def srtimes(g, ttn: np.ndarray, rcv: np.ndarray) -> np.ndarray:
     return out
 
 
-def rpaths(g, veln: np.ndarray, ttn: np.ndarray, rcv: np.ndarray, cap: int = 0):
-    """Returns (row_ptr int64 [nrc+1], cols int32, vals float32, ok int32, steps int32)."""
+def rpaths(g, veln: np.ndarray, ttn: np.ndarray, rcv: np.ndarray, cap: int = 0, src=None):
+    """Returns (row_ptr int64 [nrc+1], cols int32, vals float32, ok int32, steps int32).
+    src: point-source position (height km, colat rad, lon rad) or None."""
     nrc = len(rcv)
     if cap <= 0:
         cap = nrc * g.nvert if hasattr(g, "nvert") else nrc * 100000
@@ -140,8 +188,16 @@ def rpaths(g, veln: np.ndarray, ttn: np.ndarray, rcv: np.ndarray, cap: int = 0):
     vals = np.zeros(cap, dtype=np.float32)
     ok = np.zeros(nrc, dtype=np.int32)
     steps = np.zeros(nrc, dtype=np.int32)
-    n = lib().or_rpaths(C.byref(or_grid(g)), veln, np.ascontiguousarray(ttn), nrc,
-                        np.ascontiguousarray(rcv, dtype=np.float64), rp, cols, vals, cap, ok, steps)
+    if src is None:
+        n = lib().or_rpaths(C.byref(or_grid(g)), veln, np.ascontiguousarray(ttn), nrc,
+                            np.ascontiguousarray(rcv, dtype=np.float64), rp, cols, vals, cap, ok, steps)
+    else:
+        L = lib()
+        L.or_rpaths_point.argtypes = list(L.or_rpaths.argtypes) + [np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
+        L.or_rpaths_point.restype = C.c_long
+        n = L.or_rpaths_point(C.byref(or_grid(g)), veln, np.ascontiguousarray(ttn), nrc,
+                              np.ascontiguousarray(rcv, dtype=np.float64), rp, cols, vals, cap, ok, steps,
+                              np.ascontiguousarray(src, dtype=np.float64))
     if n < 0:
         raise RuntimeError("or_rpaths: capacity too small")
     return rp, cols[:n].copy(), vals[:n].copy(), ok, steps

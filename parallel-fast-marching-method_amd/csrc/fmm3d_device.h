@@ -109,7 +109,7 @@ __device__ __forceinline__ R octant_min(const Side<R> s[3][2], R hr, R hx, R hz,
 }
 
 // uniform cubic B-spline basis at fraction t (fm3dt_parallel.f90:7000-7003)
-__device__ __forceinline__ void bspline4(double t, double w[4])
+__host__ __device__ __forceinline__ void bspline4(double t, double w[4])
 {
     double s = 1.0 - t, t2 = t * t, t3 = t2 * t;
     w[0] = s * s * s / 6.0;

@@ -385,6 +385,7 @@ template <typename R> struct ColArgs {
     int* prog;              // per slot per patch: iter*(nnr+1) + layers finished, or null (no ordering)
     const int* flow;        // per slot: lateral flow signs (fx, fz), or null
     int spin_limit;         // polls before a wait on a neighbour is given up
+    int down;               // 1: walk top to bottom
 };
 
 // float32 helpers for the axis-sum octant form: fused multiply-subtract,
@@ -532,13 +533,17 @@ k_column(ColArgs<R> a)
     bool anyv = false;
     unsigned nupd = 0, nvis = 0;
     R orig[S];
-    for (int ir = a.nnr - 1; ir >= 0; --ir) {
+    // walk: bottom to top, or top to bottom when a.down; "prev" layers are the
+    // two this block relaxed just before (b1, b2), "next" ones come from HBM (u1, u2)
+    const int dn = a.down ? 1 : -1;
+    for (int q = 0; q < a.nnr; ++q) {
+        const int ir = a.down ? q : a.nnr - 1 - q;
         const size_t lay = (size_t)ir * NL;
         unsigned nfree = 0;
         // wait (bounded) until the upstream patches have finished this layer
         if (wx || wz) {
             if (t == 0) {
-                const int need = pbase + (a.nnr - ir);
+                const int need = pbase + q + 1;
                 int spins = 0;
                 while (wx && __hip_atomic_load(pg + ux, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
                     if (++spins > a.spin_limit) { wx = false; break; }
@@ -565,12 +570,13 @@ k_column(ColArgs<R> a)
             const R v = in ? T[c] : inf;
             orig[k] = v;
             cur[(lx + 2) * LZ + lz + 2] = v;
-            u1[e] = (in && ir >= 1) ? T[c - NL] : inf;
-            u2[e] = (in && ir >= 2) ? T[c - 2 * NL] : inf;
+            const int n1 = ir + dn, n2 = ir + 2 * dn;
+            u1[e] = (in && n1 >= 0 && n1 < a.nnr) ? T[c + (ptrdiff_t)dn * (ptrdiff_t)NL] : inf;
+            u2[e] = (in && n2 >= 0 && n2 < a.nnr) ? T[c + (ptrdiff_t)(2 * dn) * (ptrdiff_t)NL] : inf;
             const bool fixed = !in || ((fb[c >> 5] >> (c & 31)) & 1u);
             sl[e] = fixed ? R(-1) : a.slo[c];
             nfree += fixed ? 0u : 1u;
-            if (ir == a.nnr - 1) { b1[e] = inf; b2[e] = inf; }
+            if (q == 0) { b1[e] = inf; b2[e] = inf; }
         }
         // lateral halo ring
         for (int e = t; e < LX * (PZ + 4); e += 64) {
@@ -618,8 +624,9 @@ k_column(ColArgs<R> a)
                     in1[3] = ix + 1 < a.nnx; in2[3] = ix + 2 < a.nnx;
                     in1[4] = iz >= 1; in2[4] = iz >= 2;
                     in1[5] = iz + 1 < a.nnz; in2[5] = iz + 2 < a.nnz;
-                    v1[0] = u1[pl]; v2[0] = u2[pl];
-                    v1[1] = b1[pl]; v2[1] = b2[pl];
+                    // side 0 is the layer above (ir-1), side 1 the one below (ir+1)
+                    v1[0] = a.down ? b1[pl] : u1[pl]; v2[0] = a.down ? b2[pl] : u2[pl];
+                    v1[1] = a.down ? u1[pl] : b1[pl]; v2[1] = a.down ? u2[pl] : b2[pl];
                     v1[2] = cur[own - LZ]; v2[2] = cur[own - 2 * LZ];
                     v1[3] = cur[own + LZ]; v2[3] = cur[own + 2 * LZ];
                     v1[4] = cur[own - 1]; v2[4] = cur[own - 2];
@@ -756,7 +763,7 @@ template <typename R>
 __global__ void __launch_bounds__(64)
 k_rays(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const double* __restrict__ veln,
        int nslots, int nrc, const double* __restrict__ rcv, int ray0, int nrays, float* __restrict__ scratch,
-       RayOut out)
+       RayOut out, const double* __restrict__ srcpos)
 {
     const int lane = threadIdx.x;
     const int NVR = g.nvr + 2, NVX = g.nvx + 2, NVZ = g.nvz + 2;
@@ -787,6 +794,14 @@ k_rays(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const d
         int bmin[3] = { 1 << 30, 1 << 30, 1 << 30 }, bmax[3] = { -1, -1, -1 };
         double pr = r0, px = x0, pz = z0;
         const double dpl = g.dnr;
+        // a point source ends the ray: within one step of it, the last step goes to the source
+        const double sr = srcpos[3 * s], sxs = srcpos[3 * s + 1], szs = srcpos[3 * s + 2];
+        const bool point = !isnan(sr);
+        double xsrc[3] = { 0.0, 0.0, 0.0 };
+        if (point) {
+            const double rr = g.earth + sr;
+            xsrc[0] = rr * sin(sxs) * cos(szs); xsrc[1] = rr * sin(sxs) * sin(szs); xsrc[2] = rr * cos(sxs);
+        }
         for (int step = 1; step <= 100000; ++step) {
             double dtr = 0.0;
             int cnt = 0;
@@ -794,7 +809,10 @@ k_rays(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const d
             for (int q = 0; q < 4; ++q) {
                 double lo = TT(ipr + 1, ipx + cx[q], ipz + cz[q]);
                 double hi = TT(ipr, ipx + cx[q], ipz + cz[q]);
-                if (lo < hi) { dtr = dtr + lo - hi; cnt++; }
+                // teleseismic: only edges where t decreases downward (the
+                // reference's rule); a point source: every edge, since rays
+                // climb towards a source above them
+                if (point || lo < hi) { dtr = dtr + lo - hi; cnt++; }
             }
             if (cnt) dtr = dtr / (cnt * g.dnr);
             double dtx = TT(ipr, ipx + 1, ipz) - TT(ipr, ipx, ipz);
@@ -814,14 +832,26 @@ k_rays(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const d
             double nr_ = pr + dpl * dtr / gn;
             double nx_ = px - dpl * dtx / (crad * gn);
             double nz_ = pz - dpl * dtz / (crad * sx * gn);
+            double seg = dpl;   // length the Frechet increments use (the reference's nominal step)
+            int done = 0;
+            if (point) {
+                const double sp = sin(px), cp = cos(px);
+                const double d0 = crad * sp * cos(pz) - xsrc[0], d1 = crad * sp * sin(pz) - xsrc[1], d2 = crad * cp - xsrc[2];
+                const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                if (d <= dpl || !(gn > 0.0)) { nr_ = sr; nx_ = sxs; nz_ = szs; seg = d; done = 1; }
+            }
             int ipro = ipr, ipxo = ipx, ipzo = ipz;
             ipr = (int)floor((g.rgor - nr_) / g.dnr) + 1;
             ipx = (int)floor((nx_ - g.rgox) / g.dnx) + 1;
             ipz = (int)floor((nz_ - g.rgoz) / g.dnz) + 1;
-            int exr = (ipr < 1 || ipr >= g.nnr);
-            int exx = (ipx < 1 || ipx >= g.nnx);
-            int exz = (ipz < 1 || ipz >= g.nnz);
-            int done = 0;
+            if (done) {   // the source cell, with the last-node rule of the start point
+                ipr = min(max(ipr, 1), g.nnr - 1);
+                ipx = min(max(ipx, 1), g.nnx - 1);
+                ipz = min(max(ipz, 1), g.nnz - 1);
+            }
+            int exr = !done && (ipr < 1 || ipr >= g.nnr);
+            int exx = !done && (ipx < 1 || ipx >= g.nnx);
+            int exz = !done && (ipz < 1 || ipz >= g.nnz);
             if (exr || exx || exz) {
                 // clip to the first face crossed (F:7463-7538)
                 int which = 0;
@@ -916,7 +946,7 @@ k_rays(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const d
                 bspline4(((g.gor - sr_) - (ivrt - 1) * g.dvr) / g.dvr, ui);
                 bspline4(((sx_ - g.gox) - (ivxt - 1) * g.dvx) / g.dvx, vi);
                 bspline4(((sz_ - g.goz) - (ivzt - 1) * g.dvz) / g.dvz, wi);
-                double dinc = (k == 0) ? vrat[0] * dpl : (vrat[k] - vrat[k - 1]) * dpl;
+                double dinc = (k == 0) ? vrat[0] * seg : (vrat[k] - vrat[k - 1]) * seg;
                 // this lane's tap: vertex (ivt-1+n) with n the lane's residue offset
                 int rr = ivrt - 1 + (((own_r - (ivrt - 1)) % 4 + 4) % 4);
                 int xx = ivxt - 1 + (((own_x - (ivxt - 1)) % 4 + 4) % 4);
@@ -989,6 +1019,7 @@ struct Slots {
     int pcap = 0;             // slots P holds (allocated by the brick strategy only)
     int* prog = nullptr;      // column walk: per slot per patch layer progress
     int* flow = nullptr;      // column walk: per slot lateral flow signs
+    double* srcpos = nullptr; // per slot point-source position (height, colat, lon), NaN if none
 };
 
 }  // namespace
@@ -1016,6 +1047,9 @@ struct fmm3d_ctx {
     int strategy = 1;              // 0: brick sweeps in three phases; 1: column walk, one phase
     int col_px = 8, col_pz = 8;    // column-walk patch (theta x phi)
     int col_passes = 16;           // line passes per layer at most
+    int walk_alt = 0;              // this solve: alternate the column walk direction per sweep
+    std::vector<double> velv_host; // the vertex model (for refined sub-grids)
+    fmm3d_ctx* fine = nullptr;     // refined source neighbourhood (point sources)
     int ordered = 1;               // column walk: patches wait for their upstream neighbours
     int spin_limit = 20000;        // polls (~64 cycles each) before such a wait is given up
     int max_sweeps = 0;            // column walk: stop after this many sweeps (0: at convergence)
@@ -1073,6 +1107,9 @@ extern "C" const char* fmm3d_last_error(const fmm3d_ctx* c)
     return c ? c->err.c_str() : g_create_err.c_str();
 }
 
+static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out);
+static int upload_metrics(fmm3d_ctx* c);
+
 extern "C" int fmm3d_create(const fmm3d_grid_desc* d, int device, fmm3d_ctx** out)
 {
     if (!d || !out) return fail(nullptr, FMM3D_E_ARG, "fmm3d_create: null argument");
@@ -1103,6 +1140,44 @@ extern "C" int fmm3d_create(const fmm3d_grid_desc* d, int device, fmm3d_ctx** ou
     g.rgox = (90.0 - d->rgoxd) * PI_REF / 180.0;
     g.rgoz = d->rgozd * PI_REF / 180.0;
     g.earth = d->earth;
+    return ctx_setup(c, device, out);
+}
+
+// Metric tables in the reference's arithmetic (F:541-542): hx[ir] = ri*dnx,
+// hz[ir][ix] = ri*sin(theta)*dnz, ri = rgor - ir*dnr + earth (0-based).
+static int upload_metrics(fmm3d_ctx* c)
+{
+    const Grid& g = c->g;
+    std::vector<double> hx(g.nnr), hz((size_t)g.nnx * g.nnr);
+    std::vector<float> hx32(g.nnr), hz32((size_t)g.nnx * g.nnr);
+    for (int ir = 0; ir < g.nnr; ++ir) {
+        double ri = g.rgor - (ir) * g.dnr + g.earth;
+        hx[ir] = ri * g.dnx;
+        hx32[ir] = (float)hx[ir];
+        for (int ix = 0; ix < g.nnx; ++ix) {
+            double sth = sin(g.rgox + (ix) * g.dnx);
+            hz[(size_t)ir * g.nnx + ix] = ri * sth * g.dnz;
+            hz32[(size_t)ir * g.nnx + ix] = (float)hz[(size_t)ir * g.nnx + ix];
+        }
+    }
+    if (!c->hx64) {
+        HIPCHK(c, dalloc(&c->hx64, hx.size() * sizeof(double)));
+        HIPCHK(c, dalloc(&c->hz64, hz.size() * sizeof(double)));
+        HIPCHK(c, dalloc(&c->hx32, hx32.size() * sizeof(float)));
+        HIPCHK(c, dalloc(&c->hz32, hz32.size() * sizeof(float)));
+    }
+    HIPCHK(c, hipMemcpy(c->hx64, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->hz64, hz.data(), hz.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->hx32, hx32.data(), hx32.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->hz32, hz32.data(), hz32.size() * sizeof(float), hipMemcpyHostToDevice));
+    return FMM3D_OK;
+}
+
+// Device resources of a context whose grid is set (also used for the
+// refined sub-grid of point sources).
+static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
+{
+    const Grid& g = c->g;
     c->device = device;
     c->N = (size_t)g.nnr * g.nnx * g.nnz;
     int rc;
@@ -1127,27 +1202,8 @@ extern "C" int fmm3d_create(const fmm3d_grid_desc* d, int device, fmm3d_ctx** ou
     CREATECHK(dalloc(&c->slo32, c->N * sizeof(float)));
     CREATECHK(dalloc(&c->d_updates, sizeof(int64_t)));
     CREATECHK(dalloc(&c->d_visits, 3 * sizeof(unsigned long long)));
-    // metric tables in the reference's arithmetic (F:541-542)
-    std::vector<double> hx(g.nnr), hz((size_t)g.nnx * g.nnr);
-    std::vector<float> hx32(g.nnr), hz32((size_t)g.nnx * g.nnr);
-    for (int ir = 0; ir < g.nnr; ++ir) {
-        double ri = g.rgor - (ir) * g.dnr + g.earth;
-        hx[ir] = ri * g.dnx;
-        hx32[ir] = (float)hx[ir];
-        for (int ix = 0; ix < g.nnx; ++ix) {
-            double sth = sin(g.rgox + (ix) * g.dnx);
-            hz[(size_t)ir * g.nnx + ix] = ri * sth * g.dnz;
-            hz32[(size_t)ir * g.nnx + ix] = (float)hz[(size_t)ir * g.nnx + ix];
-        }
-    }
-    CREATECHK(dalloc(&c->hx64, hx.size() * sizeof(double)));
-    CREATECHK(dalloc(&c->hz64, hz.size() * sizeof(double)));
-    CREATECHK(dalloc(&c->hx32, hx32.size() * sizeof(float)));
-    CREATECHK(dalloc(&c->hz32, hz32.size() * sizeof(float)));
-    CREATECHK(hipMemcpy(c->hx64, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice));
-    CREATECHK(hipMemcpy(c->hz64, hz.data(), hz.size() * sizeof(double), hipMemcpyHostToDevice));
-    CREATECHK(hipMemcpy(c->hx32, hx32.data(), hx32.size() * sizeof(float), hipMemcpyHostToDevice));
-    CREATECHK(hipMemcpy(c->hz32, hz32.data(), hz32.size() * sizeof(float), hipMemcpyHostToDevice));
+    rc = upload_metrics(c);
+    if (rc) { fmm3d_destroy(c); return rc; }
 #undef CREATECHK
     const char* tl = getenv("FMM3D_TOL");    // "tol64[:tol32]"
     if (tl) sscanf(tl, "%lf:%lf", &c->tol64, &c->tol32);
@@ -1185,7 +1241,7 @@ static void free_slots(Slots& s)
 {
     (void)hipFree(s.T); (void)hipFree(s.P); (void)hipFree(s.fix); (void)hipFree(s.stamp);
     (void)hipFree(s.lastchg); (void)hipFree(s.on); (void)hipFree(s.toff);
-    (void)hipFree(s.valchg); (void)hipFree(s.prog); (void)hipFree(s.flow);
+    (void)hipFree(s.valchg); (void)hipFree(s.prog); (void)hipFree(s.flow); (void)hipFree(s.srcpos);
     s = Slots();
 }
 
@@ -1194,6 +1250,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     if (!c) return FMM3D_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->fine) { fmm3d_destroy(c->fine); c->fine = nullptr; }
     free_slots(c->sl);
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
     (void)hipFree(c->hx64); (void)hipFree(c->hz64); (void)hipFree(c->hx32); (void)hipFree(c->hz32);
@@ -1217,6 +1274,7 @@ extern "C" int fmm3d_set_model_vtx(fmm3d_ctx* c, const double* velv)
     for (size_t i = 0; i < nv; ++i)
         if (!(velv[i] > 0.0) || !isfinite(velv[i]))
             return fail(c, FMM3D_E_ARG, "fmm3d_set_model_vtx: vertex %zu has non-positive velocity %g", i, velv[i]);
+    c->velv_host.assign(velv, velv + nv);
     double* d = nullptr;
     HIPCHK(c, dalloc(&d, nv * sizeof(double)));
     hipError_t e = hipMemcpyAsync(d, velv, nv * sizeof(double), hipMemcpyHostToDevice, c->stream);
@@ -1270,6 +1328,7 @@ static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
     HIPCHK(c, dalloc(&s.valchg, nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.prog, (size_t)nsrc * nbmax * sizeof(int)));
     HIPCHK(c, dalloc(&s.flow, (size_t)nsrc * 2 * sizeof(int)));
+    HIPCHK(c, dalloc(&s.srcpos, (size_t)nsrc * 3 * sizeof(double)));
     s.cap = nsrc;
     s.prec = prec;
     return FMM3D_OK;
@@ -1310,6 +1369,39 @@ __global__ void k_fixbits_tele(int nnr, int nnx, int nnz, int nslots, const doub
             if (faces[s * NL + c % NL] > 0.0) bits |= 1u << b;
         }
         fix[w] = bits;
+    }
+}
+
+// General fixed (Dirichlet) sets: every slot's field starts unknown, then
+// each listed node takes its time (float32: minus the slot's offset) and its
+// fixed bit.  Bits of one word can come from several entries, hence atomicOr.
+template <typename R>
+__global__ void k_fill_real(R* __restrict__ p, size_t n, R v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+template <typename R>
+__global__ void k_scatter_fixed(int nslots, size_t N, const int64_t* __restrict__ offs, const int64_t* __restrict__ node,
+                                const double* __restrict__ tv, const double* __restrict__ toff, R* __restrict__ T,
+                                uint32_t* __restrict__ fix)
+{
+    const size_t NW = (N + 31) / 32;
+    const int64_t total = offs[nslots];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = nslots - 1;            // slot of entry i: offs[s] <= i < offs[s+1]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (offs[mid] <= i) lo = mid; else hi = mid - 1;
+        }
+        const size_t k = (size_t)node[i];
+        T[(size_t)lo * N + k] = std::is_same<R, double>::value ? (R)tv[i] : (R)(tv[i] - toff[lo]);
+        atomicOr(fix + (size_t)lo * NW + (k >> 5), 1u << (k & 31));
     }
 }
 
@@ -1536,11 +1628,13 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     a.visits = c->d_visits;
     a.updates = c->d_visits + 1;
     a.nodes = c->d_visits + 2;
-    a.prog = c->ordered ? s.prog : nullptr;
+    const bool ordered = c->ordered && !c->walk_alt;
+    a.prog = ordered ? s.prog : nullptr;
     a.flow = s.flow;
     a.spin_limit = c->spin_limit;
     const int blocks = 8 * ((nsrc * npx + 7) / 8) * npz;   // rows of patches round-robin over 8 XCDs
-    if (c->ordered) {
+    a.down = 0;
+    if (ordered) {
         k_fill_int<<<(nsrc * np + 255) / 256, 256, 0, c->stream>>>(s.prog, nsrc * np, 0);
         HIPCHK(c, hipGetLastError());
     }
@@ -1557,6 +1651,7 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
         const int chunk = c->max_sweeps > 0 ? std::min(FMM3D_CHECK, c->max_sweeps - k + 1) : FMM3D_CHECK;
         for (int j = 0; j < chunk; ++j, ++k) {
             a.iter = k;
+            a.down = c->walk_alt && !(k & 1);
             if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j], c->stream));
             hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks);
             if (le == hipErrorInvalidValue)
@@ -1593,6 +1688,41 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     }
     c->stats.phase_sweeps[0] = last + 1;
     *sweeps_out = last + 1;
+    return FMM3D_OK;
+}
+
+// Relax the prepared slots (fields, fixed bits, offsets and flows are set)
+// and record the statistics.
+static int solve_run(fmm3d_ctx* c, int nsrc, int order, int precision)
+{
+    Slots& s = c->sl;
+    k_fill_real<double><<<(3 * nsrc + 255) / 256, 256, 0, c->stream>>>(s.srcpos, (size_t)3 * nsrc, __builtin_nan(""));
+    HIPCHK(c, hipGetLastError());
+    c->stats = fmm3d_stats{};
+    c->stats.tile_r = c->brick_sl; c->stats.tile_x = c->tile_x; c->stats.tile_z = c->tile_z;
+    HIPCHK(c, hipMemsetAsync(c->d_visits, 0, 3 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    int sweeps = 0, rc;
+    if (c->strategy == 1 || c->walk_alt) {
+        if (precision == FMM3D_PREC_FP64) rc = run_column<double, false, false>(c, nsrc, order, &sweeps);
+        else rc = run_column<float, true, true>(c, nsrc, order, &sweeps);
+    } else {
+        if (precision == FMM3D_PREC_FP64) rc = run_phases<double, false>(c, nsrc, order, &sweeps);
+        else rc = run_phases<float, true>(c, nsrc, order, &sweeps);
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (rc) return rc;
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->stats.solve_ms = ms;
+    c->stats.sweeps = sweeps;
+    unsigned long long vis[3] = { 0, 0, 0 };
+    HIPCHK(c, hipMemcpy(vis, c->d_visits, sizeof vis, hipMemcpyDeviceToHost));
+    c->stats.brick_visits = (int64_t)vis[0];
+    c->stats.node_updates = (int64_t)vis[1];
+    c->stats.node_visits = (int64_t)vis[2];
+    s.n = nsrc;
     return FMM3D_OK;
 }
 
@@ -1649,6 +1779,7 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     if (rc) return rc;
     Slots& s = c->sl;
     s.n = 0;
+    c->walk_alt = 0;
     double* dfaces = nullptr;
     HIPCHK(c, dalloc(&dfaces, (size_t)nsrc * F * sizeof(double)));
     HIPCHK(c, hipMemcpyAsync(dfaces, faces, (size_t)nsrc * F * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -1665,32 +1796,239 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     }
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) { (void)hipFree(dfaces); return fail(c, FMM3D_E_DEVICE, "init: %s", hipGetErrorString(le)); }
-    c->stats = fmm3d_stats{};
-    c->stats.tile_r = c->brick_sl; c->stats.tile_x = c->tile_x; c->stats.tile_z = c->tile_z;
-    HIPCHK(c, hipMemsetAsync(c->d_visits, 0, 3 * sizeof(unsigned long long), c->stream));
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    int sweeps = 0;
-    if (c->strategy == 1) {
-        if (precision == FMM3D_PREC_FP64) rc = run_column<double, false, false>(c, nsrc, order, &sweeps);
-        else rc = run_column<float, true, true>(c, nsrc, order, &sweeps);
-    } else {
-        if (precision == FMM3D_PREC_FP64) rc = run_phases<double, false>(c, nsrc, order, &sweeps);
-        else rc = run_phases<float, true>(c, nsrc, order, &sweeps);
-    }
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc2 = solve_run(c, nsrc, order, precision);
     (void)hipFree(dfaces);
+    return rc2;
+}
+
+// Lateral flow signs for a general fixed set: the (theta, phi) slope of a
+// least-squares plane through the fixed times over their node positions.
+static void fixed_flow(const Grid& g, const int64_t* node, const double* tv, int64_t n, int out[2])
+{
+    double m = 0, sx = 0, sz = 0, sxx = 0, szz = 0, sxz = 0, st = 0, sxt = 0, szt = 0;
+    const int64_t NL = (int64_t)g.nnx * g.nnz;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t f = node[i] % NL;
+        const double ix = (double)(f / g.nnz), iz = (double)(f % g.nnz), v = tv[i];
+        m += 1; sx += ix; sz += iz; sxx += ix * ix; szz += iz * iz; sxz += ix * iz;
+        st += v; sxt += ix * v; szt += iz * v;
+    }
+    double gx = 0.0, gz = 0.0;
+    if (m >= 3) {
+        const double cxx = sxx - sx * sx / m, czz = szz - sz * sz / m, cxz = sxz - sx * sz / m;
+        const double cxt = sxt - sx * st / m, czt = szt - sz * st / m;
+        const double det = cxx * czz - cxz * cxz;
+        if (det > 0.0) { gx = (cxt * czz - czt * cxz) / det; gz = (czt * cxx - cxt * cxz) / det; }
+    }
+    out[0] = gx >= 0.0 ? 1 : -1;
+    out[1] = gz >= 0.0 ? 1 : -1;
+}
+
+extern "C" int fmm3d_solve_fixed(fmm3d_ctx* c, int nsrc, const int64_t* offs, const int64_t* node, const double* tv,
+                                 int order, int precision, int flags)
+{
+    if (!c) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: null context");
+    if (nsrc <= 0 || !offs) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: need nsrc > 0 and offsets");
+    if (order != FMM3D_ORDER_FIRST && order != FMM3D_ORDER_MIXED) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: bad order %d", order);
+    if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: bad precision %d", precision);
+    if (flags & ~FMM3D_WALK_ALTERNATE) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: unknown flags 0x%x", flags);
+    if (!c->have_model) return fail(c, FMM3D_E_STATE, "fmm3d_solve_fixed: no velocity model (call fmm3d_set_model_vtx)");
+    if (offs[0] != 0) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: offs[0] must be 0");
+    const int64_t total = offs[nsrc];
+    if (total > 0 && (!node || !tv)) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: null node or time array");
+    std::vector<double> toff(nsrc, 0.0);
+    std::vector<int> flow((size_t)nsrc * 2, 1);
+    const Grid& g = c->g;
+    for (int q = 0; q < nsrc; ++q) {
+        if (offs[q + 1] < offs[q]) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: offsets decrease at source %d", q);
+        double m = __builtin_huge_val();
+        for (int64_t i = offs[q]; i < offs[q + 1]; ++i) {
+            if (node[i] < 0 || (size_t)node[i] >= c->N)
+                return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: node %lld out of range (source %d)", (long long)node[i], q);
+            if (!(tv[i] >= 0.0) || !isfinite(tv[i]))
+                return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: time %g at node %lld must be finite and >= 0", tv[i], (long long)node[i]);
+            m = std::min(m, tv[i]);
+        }
+        toff[q] = isfinite(m) ? m : 0.0;
+        fixed_flow(g, node + offs[q], tv + offs[q], offs[q + 1] - offs[q], &flow[2 * q]);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_slots(c, nsrc, precision);
     if (rc) return rc;
-    float ms = 0.0f;
-    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    c->stats.solve_ms = ms;
-    c->stats.sweeps = sweeps;
-    unsigned long long vis[3] = { 0, 0, 0 };
-    HIPCHK(c, hipMemcpy(vis, c->d_visits, sizeof vis, hipMemcpyDeviceToHost));
-    c->stats.brick_visits = (int64_t)vis[0];
-    c->stats.node_updates = (int64_t)vis[1];
-    c->stats.node_visits = (int64_t)vis[2];
-    s.n = nsrc;
+    Slots& s = c->sl;
+    s.n = 0;
+    c->walk_alt = (flags & FMM3D_WALK_ALTERNATE) ? 1 : 0;
+    int64_t* doffs = nullptr;
+    int64_t* dnode = nullptr;
+    double* dtv = nullptr;
+    HIPCHK(c, dalloc(&doffs, (size_t)(nsrc + 1) * sizeof(int64_t)));
+    hipError_t e = hipMemcpyAsync(doffs, offs, (size_t)(nsrc + 1) * sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && total > 0) e = dalloc(&dnode, (size_t)total * sizeof(int64_t));
+    if (e == hipSuccess && total > 0) e = dalloc(&dtv, (size_t)total * sizeof(double));
+    if (e == hipSuccess && total > 0) e = hipMemcpyAsync(dnode, node, (size_t)total * sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && total > 0) e = hipMemcpyAsync(dtv, tv, (size_t)total * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.toff, toff.data(), nsrc * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.flow, flow.data(), flow.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        const size_t n = (size_t)nsrc * c->N, nw = (size_t)nsrc * ((c->N + 31) / 32);
+        if (precision == FMM3D_PREC_FP64) k_fill_real<double><<<4096, 256, 0, c->stream>>>((double*)s.T, n, Lim<double>::inf());
+        else k_fill_real<float><<<4096, 256, 0, c->stream>>>((float*)s.T, n, Lim<float>::inf());
+        k_fill_u32<<<4096, 256, 0, c->stream>>>(s.fix, nw);
+        if (total > 0) {
+            const int nb = (int)std::min<int64_t>(4096, (total + 255) / 256);
+            if (precision == FMM3D_PREC_FP64)
+                k_scatter_fixed<double><<<nb, 256, 0, c->stream>>>(nsrc, c->N, doffs, dnode, dtv, s.toff, (double*)s.T, s.fix);
+            else
+                k_scatter_fixed<float><<<nb, 256, 0, c->stream>>>(nsrc, c->N, doffs, dnode, dtv, s.toff, (float*)s.T, s.fix);
+        }
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        (void)hipFree(doffs); (void)hipFree(dnode); (void)hipFree(dtv);
+        return fail(c, FMM3D_E_DEVICE, "fmm3d_solve_fixed: %s", hipGetErrorString(e));
+    }
+    rc = solve_run(c, nsrc, order, precision);
+    (void)hipFree(doffs); (void)hipFree(dnode); (void)hipFree(dtv);
+    return rc;
+}
+
+// B-spline velocity at one point, in the gridder's arithmetic (k_gridder);
+// hr is its (gor - h) expression, spelled as the caller's grid spells it.
+static double host_velocity(const Grid& g, const double* velv, double h, double hr, double th, double ph)
+{
+    const int NR = g.nvr + 2, NX = g.nvx + 2, NZ = g.nvz + 2;
+    const int ri = (int)floor(hr / g.dvr) + 1;
+    double wr[4], wx[4], wz[4];
+    bspline4(((g.gor - (ri - 1) * g.dvr) - h) / g.dvr, wr);
+    const int xi = (int)floor((th - g.gox) / g.dvx) + 1;
+    bspline4((th - g.gox - (xi - 1) * g.dvx) / g.dvx, wx);
+    const int zi = (int)floor((ph - g.goz) / g.dvz) + 1;
+    bspline4((ph - g.goz - (zi - 1) * g.dvz) / g.dvz, wz);
+    double acc_z = 0.0;
+    for (int a = 0; a < 4; ++a) {
+        const int iz = std::min(std::max(zi - 1 + a, 0), NZ - 1);
+        double acc_x = 0.0;
+        for (int b = 0; b < 4; ++b) {
+            const int ix = std::min(std::max(xi - 1 + b, 0), NX - 1);
+            const double* col = velv + ((size_t)iz * NX + ix) * NR;
+            double acc_r = 0.0;
+            for (int cc = 0; cc < 4; ++cc) acc_r += wr[cc] * col[std::min(std::max(ri - 1 + cc, 0), NR - 1)];
+            acc_x += wx[b] * acc_r;
+        }
+        acc_z += wz[a] * acc_x;
+    }
+    return acc_z;
+}
+
+static void cart3(const Grid& g, double h, double th, double ph, double x[3])
+{
+    const double r = g.earth + h;
+    x[0] = r * sin(th) * cos(ph);
+    x[1] = r * sin(th) * sin(ph);
+    x[2] = r * cos(th);
+}
+
+extern "C" int fmm3d_solve_point(fmm3d_ctx* c, int nsrc, const double* src, int refine, int radius,
+                                 int order, int precision, int flags)
+{
+    if (!c) return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: null context");
+    if (nsrc <= 0 || !src) return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: need nsrc > 0 and sources");
+    if (refine < 1 || refine > 16 || radius < 1 || radius > 64)
+        return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: refine must be 1..16 and radius 1..64");
+    if (order != FMM3D_ORDER_FIRST && order != FMM3D_ORDER_MIXED) return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: bad order %d", order);
+    if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: bad precision %d", precision);
+    if (flags != 0) return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: flags must be 0");
+    if (!c->have_model) return fail(c, FMM3D_E_STATE, "fmm3d_solve_point: no velocity model (call fmm3d_set_model_vtx)");
+    const Grid& g = c->g;
+    const int nn[3] = { g.nnr, g.nnx, g.nnz };
+    std::vector<int64_t> offs(nsrc + 1, 0), node;
+    std::vector<double> tv, fine;
+    const double* velv = c->velv_host.data();
+    for (int q = 0; q < nsrc; ++q) {
+        const double hs = src[3 * q], ths = src[3 * q + 1], phs = src[3 * q + 2];
+        const double u[3] = { (g.rgor - hs) / g.dnr, (ths - g.rgox) / g.dnx, (phs - g.rgoz) / g.dnz };
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            if (!(u[a] >= 0.0 && u[a] <= nn[a] - 1))
+                return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: source %d lies outside the grid", q);
+            const int cc = std::min((int)floor(u[a]), nn[a] - 2);
+            lo[a] = std::max(0, cc - radius);
+            hi[a] = std::min(nn[a] - 1, cc + 1 + radius);
+        }
+        // the refined sub-grid over coarse nodes lo..hi
+        Grid f = g;
+        f.nnr = (hi[0] - lo[0]) * refine + 1;
+        f.nnx = (hi[1] - lo[1]) * refine + 1;
+        f.nnz = (hi[2] - lo[2]) * refine + 1;
+        f.rgor = g.rgor - lo[0] * g.dnr;
+        f.rgox = g.rgox + lo[1] * g.dnx;
+        f.rgoz = g.rgoz + lo[2] * g.dnz;
+        f.dnr = g.dnr / refine; f.dnx = g.dnx / refine; f.dnz = g.dnz / refine;
+        if ((double)f.nnr * f.nnx * f.nnz > 64.0e6)
+            return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: refined block of %dx%dx%d nodes is too large", f.nnr, f.nnx, f.nnz);
+        fmm3d_ctx* fc = c->fine;
+        if (fc && (fc->g.nnr != f.nnr || fc->g.nnx != f.nnx || fc->g.nnz != f.nnz)) {
+            fmm3d_destroy(fc);
+            c->fine = fc = nullptr;
+        }
+        if (!fc) {
+            fmm3d_ctx* n = new fmm3d_ctx();
+            n->g = f;
+            int rc = ctx_setup(n, c->device, &c->fine);
+            if (rc) return fail(c, rc, "fmm3d_solve_point: refined grid: %s", fmm3d_last_error(nullptr));
+            fc = c->fine;
+        } else {
+            fc->g = f;
+            int rc = upload_metrics(fc);
+            if (rc) return fail(c, rc, "fmm3d_solve_point: refined grid: %s", fc->err.c_str());
+        }
+        int rc = fmm3d_set_model_vtx(fc, velv);
+        if (rc) return fail(c, rc, "fmm3d_solve_point: refined gridder: %s", fc->err.c_str());
+        // straight-ray times on the corners of the fine cell holding the source
+        int fcell[3];
+        const double fu[3] = { (f.rgor - hs) / f.dnr, (ths - f.rgox) / f.dnx, (phs - f.rgoz) / f.dnz };
+        const int fn[3] = { f.nnr, f.nnx, f.nnz };
+        for (int a = 0; a < 3; ++a) fcell[a] = std::min(std::max((int)floor(fu[a]), 0), fn[a] - 2);
+        double xs[3];
+        cart3(g, hs, ths, phs, xs);
+        const double ss = 1.0 / host_velocity(g, velv, hs, g.gor - hs, ths, phs);
+        int64_t fnode[8];
+        double ftime[8];
+        int m = 0;
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int k = 0; k < 2; ++k, ++m) {
+                    const int r = fcell[0] + i, x = fcell[1] + j, z = fcell[2] + k;
+                    const double hn = f.rgor - r * f.dnr, thn = f.rgox + x * f.dnx, phn = f.rgoz + z * f.dnz;
+                    double xn[3];
+                    cart3(g, hn, thn, phn, xn);
+                    const double d = sqrt((xn[0] - xs[0]) * (xn[0] - xs[0]) + (xn[1] - xs[1]) * (xn[1] - xs[1]) +
+                                          (xn[2] - xs[2]) * (xn[2] - xs[2]));
+                    const double sn = 1.0 / host_velocity(f, velv, hn, f.gor - f.rgor + r * f.dnr, thn, phn);
+                    fnode[m] = ((int64_t)r * f.nnx + x) * f.nnz + z;
+                    ftime[m] = d * 0.5 * (ss + sn);
+                }
+        const int64_t foffs[2] = { 0, 8 };
+        rc = fmm3d_solve_fixed(fc, 1, foffs, fnode, ftime, order, FMM3D_PREC_FP64, FMM3D_WALK_ALTERNATE);
+        if (rc) return fail(c, rc, "fmm3d_solve_point: refined solve: %s", fc->err.c_str());
+        fine.resize((size_t)f.nnr * f.nnx * f.nnz);
+        rc = fmm3d_get_field(fc, 0, fine.data());
+        if (rc) return fail(c, rc, "fmm3d_solve_point: refined field: %s", fc->err.c_str());
+        // the coarse nodes the sub-grid covers seed the coarse solve
+        for (int r = lo[0]; r <= hi[0]; ++r)
+            for (int x = lo[1]; x <= hi[1]; ++x)
+                for (int z = lo[2]; z <= hi[2]; ++z) {
+                    const size_t fq = ((size_t)(r - lo[0]) * refine * f.nnx + (size_t)(x - lo[1]) * refine) * f.nnz +
+                                      (size_t)(z - lo[2]) * refine;
+                    node.push_back(((int64_t)r * g.nnx + x) * g.nnz + z);
+                    tv.push_back(fine[fq]);
+                }
+        offs[q + 1] = (int64_t)node.size();
+    }
+    int rc = fmm3d_solve_fixed(c, nsrc, offs.data(), node.data(), tv.data(), order, precision, FMM3D_WALK_ALTERNATE);
+    if (rc) return rc;
+    // rays of these sources end at them
+    HIPCHK(c, hipMemcpy(c->sl.srcpos, src, (size_t)3 * nsrc * sizeof(double), hipMemcpyHostToDevice));
     return FMM3D_OK;
 }
 
@@ -1800,9 +2138,9 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
         RayOut o{ c->fr_cols, c->fr_vals, c->fr_count, rcap };
         int grid = (int)std::min<size_t>(workers, (size_t)nr);
         if (c->sl.prec == FMM3D_PREC_FP64)
-            k_rays<double><<<grid, 64, 0, c->stream>>>(g, (const double*)c->sl.T, c->sl.toff, c->veln, ns, nrc, drcv, r0, nr, c->ray_scratch, o);
+            k_rays<double><<<grid, 64, 0, c->stream>>>(g, (const double*)c->sl.T, c->sl.toff, c->veln, ns, nrc, drcv, r0, nr, c->ray_scratch, o, c->sl.srcpos);
         else
-            k_rays<float><<<grid, 64, 0, c->stream>>>(g, (const float*)c->sl.T, c->sl.toff, c->veln, ns, nrc, drcv, r0, nr, c->ray_scratch, o);
+            k_rays<float><<<grid, 64, 0, c->stream>>>(g, (const float*)c->sl.T, c->sl.toff, c->veln, ns, nrc, drcv, r0, nr, c->ray_scratch, o, c->sl.srcpos);
         e = hipGetLastError();
         cnt.resize(nr);
         if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), c->fr_count, nr * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);

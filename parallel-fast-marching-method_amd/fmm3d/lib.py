@@ -20,8 +20,10 @@ OK = 0
 E_ARG, E_DEVICE, E_NOMEM, E_NOCONV, E_STATE, E_CAPACITY = -1, -2, -3, -4, -5, -6
 ORDER_FIRST, ORDER_MIXED = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
+WALK_ALTERNATE = 1
 
 EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model_vtx",
+           "fmm3d_solve_fixed", "fmm3d_solve_point",
            "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_frechet", "fmm3d_get_field",
            "fmm3d_get_stats", "fmm3d_set_option")
 
@@ -73,6 +75,8 @@ def load(path: Optional[str] = None):
     L.fmm3d_last_error.restype = C.c_char_p
     L.fmm3d_set_model_vtx.argtypes = [vp, dp]
     L.fmm3d_solve_tele.argtypes = [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int]
+    L.fmm3d_solve_fixed.argtypes = [vp, C.c_int, lp, lp, dp, C.c_int, C.c_int, C.c_int]
+    L.fmm3d_solve_point.argtypes = [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L.fmm3d_receivers.argtypes = [vp, C.c_int, dp, fp]
     L.fmm3d_frechet.argtypes = [vp, C.c_int, dp, lp, ip, fp, C.c_int64]
     L.fmm3d_get_field.argtypes = [vp, C.c_int, dp]
@@ -137,6 +141,26 @@ class Context:
             raise ValueError(f"faces must be (nsrc, {self.g.nnx}, {self.g.nnz}), got {faces.shape}")
         self._check(self.L.fmm3d_solve_tele(self._h, faces.shape[0], faces, order, precision, 0))
         self.nsrc = faces.shape[0]
+
+    def solve_fixed(self, sets, order: int = ORDER_MIXED, precision: int = PREC_FP64, alternate: bool = False) -> None:
+        """sets: one (node_index array, time array) pair per source."""
+        offs = np.zeros(len(sets) + 1, np.int64)
+        for i, (n, _) in enumerate(sets):
+            offs[i + 1] = offs[i] + len(n)
+        node = np.ascontiguousarray(np.concatenate([np.asarray(n, np.int64) for n, _ in sets]) if sets else np.zeros(0, np.int64))
+        tv = np.ascontiguousarray(np.concatenate([np.asarray(t, np.float64) for _, t in sets]) if sets else np.zeros(0))
+        if node.size == 0:
+            node = np.zeros(1, np.int64); tv = np.zeros(1)
+        self._check(self.L.fmm3d_solve_fixed(self._h, len(sets), offs, node, tv, order, precision,
+                                             WALK_ALTERNATE if alternate else 0))
+        self.nsrc = len(sets)
+
+    def solve_point(self, src: np.ndarray, refine: int = 5, radius: int = 10, order: int = ORDER_MIXED,
+                    precision: int = PREC_FP64) -> None:
+        """src: (nsrc, 3) height km, colatitude rad, longitude rad."""
+        src = np.ascontiguousarray(np.atleast_2d(src), dtype=np.float64)
+        self._check(self.L.fmm3d_solve_point(self._h, src.shape[0], src, refine, radius, order, precision, 0))
+        self.nsrc = src.shape[0]
 
     def field(self, src: int) -> np.ndarray:
         out = np.zeros((self.g.nnr, self.g.nnx, self.g.nnz))

@@ -256,6 +256,29 @@ def tele_bottom_face(g: GridDesc, s: TeleSource) -> np.ndarray:
     return np.ascontiguousarray(s.t0 + s.p * (dist - s.d0))
 
 
+def tele_boundary_nodes(g: GridDesc, s: TeleSource, sides: bool = True):
+    """Fixed-node set for a plane wave injected on the bottom face and, with
+    `sides`, on the side faces the wave enters through (in theta and in phi,
+    the face where the 1-D times are earlier): (node index, time) with node
+    index (ir*nnx + ix)*nnz + iz (0-based, ir = 0 at the top) and 1-D-model
+    (tau-p) times.  The faces it leaves through stay free: fixing 1-D times
+    there would contradict the 3-D field that reaches them."""
+    T = tele_field_1d(g, s)
+    m = np.zeros(T.shape, bool)
+    m[-1] = True
+    if sides:
+        bot = T[-1]
+        m[:, 0 if bot[0, :].mean() <= bot[-1, :].mean() else -1, :] = True
+        m[:, :, 0 if bot[:, 0].mean() <= bot[:, -1].mean() else -1] = True
+    idx = np.flatnonzero(m.ravel())
+    return idx.astype(np.int64), T.ravel()[idx].astype(np.float64)
+
+
+def point_source_rad(g: GridDesc, src: "LocalSource"):
+    """(height km, colatitude rad, longitude rad) of a local source."""
+    return np.array([-src.depth_km, (90.0 - src.lat) * PI_REF / 180.0, src.lon * PI_REF / 180.0])
+
+
 @dataclasses.dataclass
 class LocalSource:
     depth_km: float

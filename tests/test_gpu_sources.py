@@ -1,0 +1,105 @@
+"""Point sources (refined neighbourhood) and side-face injection on the GPU,
+through the C ABI, against the CPU restatement (tests/test_oracle_sources.py
+pins that restatement by closed forms)."""
+import numpy as np
+import pytest
+
+from fmm3d import lib, synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg0():
+    g, velv, _, rcv = synth.tele_case((61, 61, 61), 400.0, (10.0, 10.0), (12, 12, 12), 0.05, 1, 20, 1)
+    srcs = np.array([[-20.0, g.rgox + 30.37 * g.dnx, g.rgoz + 30.61 * g.dnz],     # centred, 20 km
+                     [-55.0, g.rgox + 4.2 * g.dnx, g.rgoz + 50.9 * g.dnz],        # near two sides
+                     [-3.0, g.rgox + 12.5 * g.dnx, g.rgoz + 0.3 * g.dnz]])        # shallow, on a side
+    return g, velv, srcs, synth.receivers_rad(rcv)
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    g, velv, srcs, rcv = _cfg0()
+    veln = O.gridder(g, velv)
+    ref = [O.point_source(g, velv, veln, s, 5, 10, 1) for s in srcs]
+    return g, velv, veln, srcs, rcv, ref
+
+
+def test_point_sources_fp64_match_oracle(cfg0):
+    g, velv, veln, srcs, rcv, ref = cfg0
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        c.solve_point(srcs, 5, 10, lib.ORDER_MIXED, lib.PREC_FP64)
+        tr = c.receivers(rcv)
+        for s in range(len(srcs)):
+            F = c.field(s)
+            assert float(np.abs(F - ref[s]).max()) <= 1e-6, s
+            assert np.array_equal(tr[s], O.srtimes(g, F, rcv))
+
+
+def test_point_source_rays_match_oracle(cfg0):
+    g, velv, veln, srcs, rcv, ref = cfg0
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        c.solve_point(srcs[:1], 5, 10, lib.ORDER_MIXED, lib.PREC_FP64)
+        rp, cols, vals = c.frechet(rcv)
+        orp, oc, ov, ok, steps = O.rpaths(g, veln, c.field(0), rcv, src=srcs[0])
+        assert ok.all() and steps.max() < 1000
+        for i in range(len(rcv)):
+            a = dict(zip(cols[rp[i]:rp[i + 1]].tolist(), vals[rp[i]:rp[i + 1]].tolist()))
+            b = dict(zip(oc[orp[i]:orp[i + 1]].tolist(), ov[orp[i]:orp[i + 1]].tolist()))
+            ks = sorted(set(a) | set(b))
+            va = np.array([a.get(k, 0.0) for k in ks]); vb = np.array([b.get(k, 0.0) for k in ks])
+            assert np.abs(va - vb).max() <= 1e-6 * np.abs(vb).max(), i
+
+
+def test_point_sources_fp32_within_one_millisecond(cfg0):
+    g, velv, veln, srcs, rcv, ref = cfg0
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        c.solve_point(srcs, 5, 10, lib.ORDER_MIXED, lib.PREC_FP32)
+        for s in range(len(srcs)):
+            assert float(np.abs(c.field(s) - ref[s]).max()) <= 1e-3, s
+
+
+def test_side_face_injection_matches_relaxation():
+    """Bottom face plus incoming side faces with 1-D tau-p times: the GPU field
+    equals the oracle's relaxation of the same fixed set (float64), and the
+    float32 solve stays within 1 ms."""
+    g, velv, srcs, rcv = synth.tele_case((31, 33, 35), 400.0, (10.0, 10.0), (10, 11, 12), 0.05, 3, 20, 11)
+    veln = O.gridder(g, velv)
+    sets = [synth.tele_boundary_nodes(g, s, True) for s in srcs]
+    refs = []
+    for idx, tv in sets:
+        fixed = np.zeros(g.nnr * g.nnx * g.nnz, np.uint8)
+        fixed[idx] = 1
+        tf = np.zeros(fixed.shape)
+        tf[idx] = tv
+        R, _ = O.relax1_fixed(g, veln, fixed.reshape(g.nnr, g.nnx, g.nnz), tf.reshape(g.nnr, g.nnx, g.nnz), 1, False)
+        refs.append(R)
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        c.solve_fixed(sets, lib.ORDER_MIXED, lib.PREC_FP64)
+        for s in range(len(srcs)):
+            assert float(np.abs(c.field(s) - refs[s]).max()) <= 1e-7, s
+        c.solve_fixed(sets, lib.ORDER_MIXED, lib.PREC_FP32)
+        for s in range(len(srcs)):
+            assert float(np.abs(c.field(s) - refs[s]).max()) <= 1e-3, s
+
+
+def test_source_errors():
+    g, velv, srcs, rcv = _cfg0()
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        bad = srcs[:1].copy()
+        bad[0, 1] += 1.0                       # far outside in colatitude
+        with pytest.raises(lib.FmmError) as e:
+            c.solve_point(bad)
+        assert e.value.code == lib.E_ARG
+        with pytest.raises(lib.FmmError) as e:
+            c.solve_fixed([(np.array([g.nnr * g.nnx * g.nnz]), np.array([1.0]))])
+        assert e.value.code == lib.E_ARG
+        with pytest.raises(lib.FmmError) as e:
+            c.solve_fixed([(np.array([5]), np.array([np.nan]))])
+        assert e.value.code == lib.E_ARG
