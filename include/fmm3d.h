@@ -41,6 +41,7 @@ extern "C" {
 #define FMM3D_E_NOCONV     -4   /* solver hit its iteration cap             */
 #define FMM3D_E_STATE      -5   /* call out of order (e.g. no model yet)    */
 #define FMM3D_E_CAPACITY   -6   /* caller's output buffer too small         */
+#define FMM3D_E_TIMEOUT    -7   /* server: the request is not done yet      */
 
 /* solver options */
 #define FMM3D_ORDER_FIRST   0   /* first-order stencils (.in line 9 = 0)    */
@@ -167,6 +168,61 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *   "timing"        1: time every relaxation launch with events (stats.kernel_ms)
  * Returns FMM3D_E_ARG for an unknown key or a value out of range. */
 int fmm3d_set_option(fmm3d_ctx *ctx, const char *key, double value);
+
+/* ---------------------------------------------------------------------------
+ * Serving loop (no counterpart in the reference, which runs one source after
+ * another per process, fm3dt_parallel.f90:6371).  A server owns one device
+ * and a scheduler thread.  Requests -- sources on a grid and velocity model,
+ * with receivers -- are copied at submission.  Models stay resident (one
+ * context per grid, model id and precision), and requests are admitted by
+ * HBM footprint: a new model's context gets as many source slots as the
+ * budget leaves.  Sources join a running batch one slot at a time between
+ * sweep chunks (continuous batching), converged sources leave it, and
+ * contexts of different grids sweep concurrently on their own streams.
+ * ------------------------------------------------------------------------- */
+#define FMM3D_SRC_TELE      0   /* bottom faces (itimes.out records)        */
+#define FMM3D_SRC_POINT     1   /* point sources with a refined neighbourhood */
+
+typedef struct fmm3d_server fmm3d_server;
+
+typedef struct {
+    fmm3d_grid_desc grid;
+    const double *velv;         /* vertex model, (nvz+2)(nvx+2)(nvr+2), r fastest */
+    uint64_t model_id;          /* requests with equal grid, model id and precision share one resident model */
+    int kind;                   /* FMM3D_SRC_TELE or FMM3D_SRC_POINT */
+    int nsrc;
+    const double *faces;        /* TELE: nsrc records of nnx*nnz, phi fastest */
+    const double *src;          /* POINT: nsrc records (height km, colatitude rad, longitude rad) */
+    int refine, radius;         /* POINT: refinement factor and radius in nodes */
+    int nrc;
+    const double *rcv;          /* nrc records (height km, colatitude rad, longitude rad) */
+    int order, precision;       /* FMM3D_ORDER_*, FMM3D_PREC_* */
+} fmm3d_request;
+
+typedef struct {
+    int64_t requests_done;
+    int64_t sources_done;
+    double p50_ms, p99_ms;      /* request latency, submission to completion */
+    double source_solves_per_s; /* over the span from the first submission to the last completion */
+    double hbm_bytes;           /* device bytes held by resident models and slots */
+    int models_resident;
+    int sources_in_flight;
+} fmm3d_server_info;
+
+/* hbm_budget: device bytes the server may hold; max_slots: source slots per
+ * model context at most. */
+int fmm3d_server_create(int device, double hbm_budget, int max_slots, fmm3d_server **out);
+/* Copies the request; *ticket names it for wait/result. */
+int fmm3d_server_submit(fmm3d_server *srv, const fmm3d_request *req, int64_t *ticket);
+/* FMM3D_OK once done (or failed: result reports it), FMM3D_E_TIMEOUT if not
+ * within timeout_s seconds (0: poll). */
+int fmm3d_server_wait(fmm3d_server *srv, int64_t ticket, double timeout_s);
+/* Receiver times (nsrc*nrc float32, source-major) and the request's latency;
+ * releases the ticket.  Returns the request's status. */
+int fmm3d_server_result(fmm3d_server *srv, int64_t ticket, float *times, double *latency_ms);
+int fmm3d_server_stats(fmm3d_server *srv, fmm3d_server_info *out);
+/* Drains nothing: pending requests are dropped, the thread joins. */
+int fmm3d_server_destroy(fmm3d_server *srv);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,38 @@
+// fmm3d_engine.h -- continuous batching inside one context (library
+// internal; the serving loop in fmm3d_server.hip drives it).
+//
+// A context opened as an engine holds `cap` slots of one precision.  Slots
+// are admitted one source at a time while the column walk keeps sweeping:
+// a new slot starts at the current sweep number with every patch column
+// active; a slot converges when a sweep changes none of its values; its
+// receiver times are read and the slot is retired and free again.  Slots
+// never interact, so a source gives the same field whichever others share
+// the sweeps (up to float32 rounding of the concurrent halo reads).
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/fmm3d.h"
+
+#define FMM3D_HIDDEN __attribute__((visibility("hidden")))
+
+FMM3D_HIDDEN int fmm3d_engine_open(fmm3d_ctx* c, int cap, int precision, int order);
+// one source into a free slot: its fixed set (node index, time), whether
+// its column walk alternates, and a point-source position (or null)
+FMM3D_HIDDEN int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double* tv, int64_t n,
+                                    int walkalt, const double* src3);
+// queue `nsweeps` sweeps over the admitted slots on the context's stream
+FMM3D_HIDDEN int fmm3d_engine_launch(fmm3d_ctx* c, int nsweeps);
+// wait for the queued sweeps; slots that converged go to `done`, slots that
+// hit the sweep cap go to `failed`
+FMM3D_HIDDEN int fmm3d_engine_collect(fmm3d_ctx* c, std::vector<int>& done, std::vector<int>& failed);
+FMM3D_HIDDEN int fmm3d_engine_receivers(fmm3d_ctx* c, int slot, int nrc, const double* rcv, float* out);
+FMM3D_HIDDEN int fmm3d_engine_retire(fmm3d_ctx* c, int slot);
+FMM3D_HIDDEN int fmm3d_engine_active(const fmm3d_ctx* c);
+// fixed set of a point source (refined neighbourhood) / of a bottom face
+FMM3D_HIDDEN int fmm3d_engine_point_set(fmm3d_ctx* c, const double* src3, int refine, int radius, int order,
+                                        std::vector<int64_t>& node, std::vector<double>& tv);
+// device bytes: the model and metric tables; one slot of a precision
+FMM3D_HIDDEN size_t fmm3d_engine_model_bytes(const fmm3d_grid_desc* g);
+FMM3D_HIDDEN size_t fmm3d_engine_slot_bytes(const fmm3d_grid_desc* g, int precision);

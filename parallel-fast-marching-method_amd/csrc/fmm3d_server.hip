@@ -1,0 +1,368 @@
+// fmm3d_server.hip -- the serving loop: a scheduler thread that keeps
+// velocity models resident, admits requests by device-memory footprint and
+// runs continuous batching on the column-walk engine (fmm3d_engine.h).
+//
+// Each (grid, model id, precision, order) gets one context opened as an
+// engine with `cap` source slots, cap chosen so that the model and its
+// slots fit the server's HBM budget.  A request's sources take free slots
+// of their model's context as they appear; every loop iteration queues one
+// chunk of sweeps on every context that has sources in flight (each on its
+// own stream, so contexts of different grids overlap on the device), then
+// collects the converged sources, reads their receiver times and frees
+// their slots.  Idle contexts are evicted, least recently used first, when
+// a new model needs the memory.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/fmm3d.h"
+#include "fmm3d_engine.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr int SWEEPS_PER_CHUNK = 4;
+
+struct Request {
+    int64_t ticket = 0;
+    fmm3d_grid_desc grid{};
+    uint64_t model_id = 0;
+    int kind = FMM3D_SRC_TELE, nsrc = 0, refine = 5, radius = 10, nrc = 0;
+    int order = FMM3D_ORDER_MIXED, precision = FMM3D_PREC_FP32;
+    std::vector<double> velv, faces, src, rcv;
+    std::vector<float> times;
+    int admitted = 0, done = 0;
+    int status = FMM3D_OK;
+    bool finished = false;
+    Clock::time_point t_submit, t_done;
+};
+
+struct Model {
+    std::string key;
+    fmm3d_ctx* ctx = nullptr;
+    int cap = 0;
+    double bytes = 0.0;
+    std::vector<std::pair<Request*, int>> owner;   // per slot: request and source index
+    int in_flight = 0;
+    bool launched = false;
+    Clock::time_point last_used;
+};
+
+std::string model_key(const Request& r)
+{
+    std::string k(reinterpret_cast<const char*>(&r.grid), sizeof r.grid);
+    k.append(reinterpret_cast<const char*>(&r.model_id), sizeof r.model_id);
+    k.append(reinterpret_cast<const char*>(&r.precision), sizeof r.precision);
+    k.append(reinterpret_cast<const char*>(&r.order), sizeof r.order);
+    return k;
+}
+
+}  // namespace
+
+struct fmm3d_server {
+    int device = 0;
+    double budget = 0.0;
+    int max_slots = 64;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;                          // not yet fully admitted
+    std::map<int64_t, std::unique_ptr<Request>> reqs;    // live tickets
+    std::vector<std::unique_ptr<Model>> models;          // scheduler thread only
+    int64_t next_ticket = 1;
+    bool stop = false;
+    std::thread th;
+    // statistics (under mu)
+    std::vector<double> lat_ms;
+    int64_t req_done = 0, src_done = 0;
+    bool any = false;
+    Clock::time_point t_first, t_last;
+    double hbm_used = 0.0;
+    int in_flight = 0;
+    int nmodels = 0;
+
+    void run();
+    Model* model_for(Request* r);
+    void finish(Request* r, int status);
+};
+
+void fmm3d_server::finish(Request* r, int status)
+{
+    std::lock_guard<std::mutex> lk(mu);
+    if (r->finished) return;
+    if (status != FMM3D_OK && r->status == FMM3D_OK) r->status = status;
+    r->finished = true;
+    r->t_done = Clock::now();
+    lat_ms.push_back(std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count());
+    req_done++;
+    t_last = r->t_done;
+    cv_done.notify_all();
+}
+
+// The context of a request's model: found, or created if the budget allows
+// (evicting idle contexts, least recently used first).  Null: not now.
+Model* fmm3d_server::model_for(Request* r)
+{
+    const std::string key = model_key(*r);
+    for (auto& m : models)
+        if (m->key == key) return m.get();
+    const double mb = (double)fmm3d_engine_model_bytes(&r->grid);
+    const double sb = (double)fmm3d_engine_slot_bytes(&r->grid, r->precision);
+    for (;;) {
+        double used = 0.0;
+        for (auto& m : models) used += m->bytes;
+        const double room = budget - used - mb;
+        if (room >= sb) break;
+        // evict the least recently used idle context
+        Model* victim = nullptr;
+        for (auto& m : models)
+            if (m->in_flight == 0 && (!victim || m->last_used < victim->last_used)) victim = m.get();
+        if (!victim) return nullptr;
+        fmm3d_destroy(victim->ctx);
+        models.erase(std::remove_if(models.begin(), models.end(), [&](const std::unique_ptr<Model>& m) { return m.get() == victim; }),
+                     models.end());
+    }
+    double used = 0.0;
+    for (auto& m : models) used += m->bytes;
+    int cap = (int)std::min<double>(max_slots, floor((budget - used - mb) / sb));
+    if (cap < 1) return nullptr;
+    auto m = std::make_unique<Model>();
+    m->key = key;
+    int rc = fmm3d_create(&r->grid, device, &m->ctx);
+    if (rc == FMM3D_OK) rc = fmm3d_set_model_vtx(m->ctx, r->velv.data());
+    if (rc == FMM3D_OK) rc = fmm3d_engine_open(m->ctx, cap, r->precision, r->order);
+    if (rc != FMM3D_OK) {
+        if (m->ctx) fmm3d_destroy(m->ctx);
+        finish(r, rc);
+        return nullptr;
+    }
+    m->cap = cap;
+    m->bytes = mb + cap * sb;
+    m->owner.assign(cap, { nullptr, -1 });
+    m->last_used = Clock::now();
+    models.push_back(std::move(m));
+    return models.back().get();
+}
+
+void fmm3d_server::run()
+{
+    (void)hipSetDevice(device);
+    std::vector<int64_t> node;
+    std::vector<double> tv;
+    std::vector<int> done, failed;
+    for (;;) {
+        std::vector<Request*> waiting;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            if (stop) break;
+            if (queue.empty() && in_flight == 0)
+                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            if (stop) break;
+            waiting.assign(queue.begin(), queue.end());
+        }
+        // admission: free slots of each request's model take its next sources
+        for (Request* r : waiting) {
+            if (r->finished) continue;
+            Model* m = model_for(r);
+            if (!m) continue;
+            for (int slot = 0; slot < m->cap && r->admitted < r->nsrc && !r->finished; ++slot) {
+                if (m->owner[slot].first) continue;
+                const int q = r->admitted;
+                node.clear();
+                tv.clear();
+                int rc = FMM3D_OK;
+                const double* src3 = nullptr;
+                if (r->kind == FMM3D_SRC_TELE) {
+                    const size_t NL = (size_t)r->grid.nnx * r->grid.nnz;
+                    const double* f = r->faces.data() + (size_t)q * NL;
+                    for (size_t i = 0; i < NL; ++i)
+                        if (f[i] > 0.0) { node.push_back((int64_t)((r->grid.nnr - 1) * NL + i)); tv.push_back(f[i]); }
+                } else {
+                    src3 = r->src.data() + 3 * q;
+                    rc = fmm3d_engine_point_set(m->ctx, src3, r->refine, r->radius, r->order, node, tv);
+                }
+                if (rc == FMM3D_OK)
+                    rc = fmm3d_engine_admit(m->ctx, slot, node.data(), tv.data(), (int64_t)node.size(),
+                                            r->kind == FMM3D_SRC_POINT, src3);
+                if (rc != FMM3D_OK) { finish(r, rc); break; }
+                m->owner[slot] = { r, q };
+                m->in_flight++;
+                r->admitted++;
+                std::lock_guard<std::mutex> lk(mu);
+                in_flight++;
+            }
+            m->last_used = Clock::now();
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            queue.erase(std::remove_if(queue.begin(), queue.end(),
+                                       [](Request* r) { return r->finished || r->admitted >= r->nsrc; }),
+                        queue.end());
+            double used = 0.0;
+            for (auto& m : models) used += m->bytes;
+            hbm_used = used;
+            nmodels = (int)models.size();
+        }
+        // one chunk of sweeps on every context with sources in flight ...
+        for (auto& m : models) {
+            m->launched = false;
+            if (m->in_flight == 0) continue;
+            if (fmm3d_engine_launch(m->ctx, SWEEPS_PER_CHUNK) == FMM3D_OK) m->launched = true;
+        }
+        // ... then the converged sources leave their slots
+        for (auto& m : models) {
+            if (!m->launched) continue;
+            int rc = fmm3d_engine_collect(m->ctx, done, failed);
+            if (rc != FMM3D_OK) { failed.clear(); done.clear(); }
+            for (int pass = 0; pass < 2; ++pass)
+                for (int slot : (pass == 0 ? done : failed)) {
+                    Request* r = m->owner[slot].first;
+                    const int q = m->owner[slot].second;
+                    if (!r) continue;
+                    int st = pass == 0 ? fmm3d_engine_receivers(m->ctx, slot, r->nrc, r->rcv.data(), r->times.data() + (size_t)q * r->nrc)
+                                       : FMM3D_E_NOCONV;
+                    fmm3d_engine_retire(m->ctx, slot);
+                    m->owner[slot] = { nullptr, -1 };
+                    m->in_flight--;
+                    {
+                        std::lock_guard<std::mutex> lk(mu);
+                        in_flight--;
+                        src_done++;
+                        if (st != FMM3D_OK && r->status == FMM3D_OK) r->status = st;
+                    }
+                    if (++r->done == r->nsrc) finish(r, FMM3D_OK);
+                }
+            m->last_used = Clock::now();
+        }
+    }
+    for (auto& m : models) fmm3d_destroy(m->ctx);
+    models.clear();
+}
+
+extern "C" int fmm3d_server_create(int device, double hbm_budget, int max_slots, fmm3d_server** out)
+{
+    if (!out || !(hbm_budget > 0.0) || max_slots < 1) return FMM3D_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return FMM3D_E_DEVICE;
+    fmm3d_server* s = new fmm3d_server();
+    s->device = device;
+    s->budget = hbm_budget;
+    s->max_slots = max_slots;
+    s->th = std::thread([s] { s->run(); });
+    *out = s;
+    return FMM3D_OK;
+}
+
+extern "C" int fmm3d_server_submit(fmm3d_server* s, const fmm3d_request* q, int64_t* ticket)
+{
+    if (!s || !q || !ticket || !q->velv || q->nsrc <= 0 || q->nrc < 0 || (q->nrc > 0 && !q->rcv)) return FMM3D_E_ARG;
+    if (q->kind != FMM3D_SRC_TELE && q->kind != FMM3D_SRC_POINT) return FMM3D_E_ARG;
+    if ((q->kind == FMM3D_SRC_TELE && !q->faces) || (q->kind == FMM3D_SRC_POINT && !q->src)) return FMM3D_E_ARG;
+    if (q->precision != FMM3D_PREC_FP32 && q->precision != FMM3D_PREC_FP64) return FMM3D_E_ARG;
+    if (q->order != FMM3D_ORDER_FIRST && q->order != FMM3D_ORDER_MIXED) return FMM3D_E_ARG;
+    const fmm3d_grid_desc& g = q->grid;
+    if (g.nnr < 3 || g.nnx < 3 || g.nnz < 3 || g.nvr < 2 || g.nvx < 2 || g.nvz < 2) return FMM3D_E_ARG;
+    auto r = std::make_unique<Request>();
+    r->grid = g;
+    r->model_id = q->model_id;
+    r->kind = q->kind;
+    r->nsrc = q->nsrc;
+    r->refine = q->refine;
+    r->radius = q->radius;
+    r->nrc = q->nrc;
+    r->order = q->order;
+    r->precision = q->precision;
+    const size_t nv = (size_t)(g.nvr + 2) * (g.nvx + 2) * (g.nvz + 2);
+    r->velv.assign(q->velv, q->velv + nv);
+    if (q->kind == FMM3D_SRC_TELE) r->faces.assign(q->faces, q->faces + (size_t)q->nsrc * g.nnx * g.nnz);
+    else r->src.assign(q->src, q->src + 3 * (size_t)q->nsrc);
+    r->rcv.assign(q->rcv, q->rcv + 3 * (size_t)q->nrc);
+    r->times.assign((size_t)q->nsrc * q->nrc, NAN);
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->any) { s->any = true; s->t_first = r->t_submit; }
+    r->ticket = s->next_ticket++;
+    *ticket = r->ticket;
+    s->queue.push_back(r.get());
+    s->reqs[r->ticket] = std::move(r);
+    s->cv_work.notify_all();
+    return FMM3D_OK;
+}
+
+extern "C" int fmm3d_server_wait(fmm3d_server* s, int64_t ticket, double timeout_s)
+{
+    if (!s) return FMM3D_E_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return FMM3D_E_ARG;
+    Request* r = it->second.get();
+    if (r->finished) return FMM3D_OK;
+    if (timeout_s <= 0.0) return FMM3D_E_TIMEOUT;
+    const bool ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] { return r->finished; });
+    return ok ? FMM3D_OK : FMM3D_E_TIMEOUT;
+}
+
+extern "C" int fmm3d_server_result(fmm3d_server* s, int64_t ticket, float* times, double* latency_ms)
+{
+    if (!s) return FMM3D_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return FMM3D_E_ARG;
+    Request* r = it->second.get();
+    if (!r->finished) return FMM3D_E_TIMEOUT;
+    if (times) memcpy(times, r->times.data(), r->times.size() * sizeof(float));
+    if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    const int st = r->status;
+    s->reqs.erase(it);
+    return st;
+}
+
+extern "C" int fmm3d_server_stats(fmm3d_server* s, fmm3d_server_info* o)
+{
+    if (!s || !o) return FMM3D_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *o = fmm3d_server_info{};
+    o->requests_done = s->req_done;
+    o->sources_done = s->src_done;
+    std::vector<double> l = s->lat_ms;
+    if (!l.empty()) {
+        std::sort(l.begin(), l.end());
+        auto pct = [&](double p) { return l[std::min(l.size() - 1, (size_t)floor(p * (l.size() - 1) + 0.5))]; };
+        o->p50_ms = pct(0.50);
+        o->p99_ms = pct(0.99);
+    }
+    if (s->any && s->req_done > 0) {
+        const double span = std::chrono::duration<double>(s->t_last - s->t_first).count();
+        o->source_solves_per_s = span > 0.0 ? s->src_done / span : 0.0;
+    }
+    o->hbm_bytes = s->hbm_used;
+    o->models_resident = s->nmodels;
+    o->sources_in_flight = s->in_flight;
+    return FMM3D_OK;
+}
+
+extern "C" int fmm3d_server_destroy(fmm3d_server* s)
+{
+    if (!s) return FMM3D_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    if (s->th.joinable()) s->th.join();
+    delete s;
+    return FMM3D_OK;
+}

@@ -39,6 +39,7 @@
 
 #include "../../include/fmm3d.h"
 #include "fmm3d_device.h"
+#include "fmm3d_engine.h"
 
 using namespace fmm3d;
 
@@ -385,7 +386,8 @@ template <typename R> struct ColArgs {
     int* prog;              // per slot per patch: iter*(nnr+1) + layers finished, or null (no ordering)
     const int* flow;        // per slot: lateral flow signs (fx, fz), or null
     int spin_limit;         // polls before a wait on a neighbour is given up
-    int down;               // 1: walk top to bottom
+    int down;               // 1: walk top to bottom (when walkalt is null)
+    const int* walkalt;     // per slot: 1 = alternate the walk (down on even sweeps), or null
 };
 
 // float32 helpers for the axis-sum octant form: fused multiply-subtract,
@@ -503,7 +505,9 @@ k_column(ColArgs<R> a)
     const int bz = fz > 0 ? rz : a.npz - 1 - rz;
     const int b = bx * a.npz + bz;
     int* st = a.stamp + (size_t)slot * np;
-    int* pg = a.prog ? a.prog + (size_t)slot * np : nullptr;
+    const bool alt = a.walkalt && a.walkalt[slot];
+    const bool down = a.walkalt ? (alt && !(a.iter & 1)) : (a.down != 0);
+    int* pg = (a.prog && !alt) ? a.prog + (size_t)slot * np : nullptr;
     const int pbase = a.iter * (a.nnr + 1);
     const int t = threadIdx.x;
     if (t == 0) {
@@ -535,9 +539,9 @@ k_column(ColArgs<R> a)
     R orig[S];
     // walk: bottom to top, or top to bottom when a.down; "prev" layers are the
     // two this block relaxed just before (b1, b2), "next" ones come from HBM (u1, u2)
-    const int dn = a.down ? 1 : -1;
+    const int dn = down ? 1 : -1;
     for (int q = 0; q < a.nnr; ++q) {
-        const int ir = a.down ? q : a.nnr - 1 - q;
+        const int ir = down ? q : a.nnr - 1 - q;
         const size_t lay = (size_t)ir * NL;
         unsigned nfree = 0;
         // wait (bounded) until the upstream patches have finished this layer
@@ -625,8 +629,8 @@ k_column(ColArgs<R> a)
                     in1[4] = iz >= 1; in2[4] = iz >= 2;
                     in1[5] = iz + 1 < a.nnz; in2[5] = iz + 2 < a.nnz;
                     // side 0 is the layer above (ir-1), side 1 the one below (ir+1)
-                    v1[0] = a.down ? b1[pl] : u1[pl]; v2[0] = a.down ? b2[pl] : u2[pl];
-                    v1[1] = a.down ? u1[pl] : b1[pl]; v2[1] = a.down ? u2[pl] : b2[pl];
+                    v1[0] = down ? b1[pl] : u1[pl]; v2[0] = down ? b2[pl] : u2[pl];
+                    v1[1] = down ? u1[pl] : b1[pl]; v2[1] = down ? u2[pl] : b2[pl];
                     v1[2] = cur[own - LZ]; v2[2] = cur[own - 2 * LZ];
                     v1[3] = cur[own + LZ]; v2[3] = cur[own + 2 * LZ];
                     v1[4] = cur[own - 1]; v2[4] = cur[own - 2];
@@ -1020,6 +1024,7 @@ struct Slots {
     int* prog = nullptr;      // column walk: per slot per patch layer progress
     int* flow = nullptr;      // column walk: per slot lateral flow signs
     double* srcpos = nullptr; // per slot point-source position (height, colat, lon), NaN if none
+    int* walkalt = nullptr;   // per slot: alternate the column walk direction
 };
 
 }  // namespace
@@ -1050,6 +1055,14 @@ struct fmm3d_ctx {
     int walk_alt = 0;              // this solve: alternate the column walk direction per sweep
     std::vector<double> velv_host; // the vertex model (for refined sub-grids)
     fmm3d_ctx* fine = nullptr;     // refined source neighbourhood (point sources)
+    // continuous-batching engine
+    bool eng_open = false;
+    int eng_k = 1, eng_order = FMM3D_ORDER_MIXED, eng_pending = 0;
+    std::vector<int> eng_active, eng_born;
+    int* eng_lc = nullptr;         // pinned host copy of the per-slot last-change sweep
+    int64_t* eng_node = nullptr;   // device staging of an admission's fixed set
+    double* eng_tv = nullptr;
+    size_t eng_stage = 0;
     int ordered = 1;               // column walk: patches wait for their upstream neighbours
     int spin_limit = 20000;        // polls (~64 cycles each) before such a wait is given up
     int max_sweeps = 0;            // column walk: stop after this many sweeps (0: at convergence)
@@ -1241,7 +1254,7 @@ static void free_slots(Slots& s)
 {
     (void)hipFree(s.T); (void)hipFree(s.P); (void)hipFree(s.fix); (void)hipFree(s.stamp);
     (void)hipFree(s.lastchg); (void)hipFree(s.on); (void)hipFree(s.toff);
-    (void)hipFree(s.valchg); (void)hipFree(s.prog); (void)hipFree(s.flow); (void)hipFree(s.srcpos);
+    (void)hipFree(s.valchg); (void)hipFree(s.prog); (void)hipFree(s.flow); (void)hipFree(s.srcpos); (void)hipFree(s.walkalt);
     s = Slots();
 }
 
@@ -1251,6 +1264,8 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->fine) { fmm3d_destroy(c->fine); c->fine = nullptr; }
+    if (c->eng_lc) (void)hipHostFree(c->eng_lc);
+    (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv);
     free_slots(c->sl);
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
     (void)hipFree(c->hx64); (void)hipFree(c->hz64); (void)hipFree(c->hx32); (void)hipFree(c->hz32);
@@ -1329,6 +1344,7 @@ static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
     HIPCHK(c, dalloc(&s.prog, (size_t)nsrc * nbmax * sizeof(int)));
     HIPCHK(c, dalloc(&s.flow, (size_t)nsrc * 2 * sizeof(int)));
     HIPCHK(c, dalloc(&s.srcpos, (size_t)nsrc * 3 * sizeof(double)));
+    HIPCHK(c, dalloc(&s.walkalt, (size_t)nsrc * sizeof(int)));
     s.cap = nsrc;
     s.prec = prec;
     return FMM3D_OK;
@@ -1598,8 +1614,9 @@ static hipError_t launch_column(fmm3d_ctx* c, const ColArgs<R>& a, int blocks)
     return hipErrorInvalidValue;
 }
 
-template <typename R, bool FAST, bool SHIFT>
-static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
+// Kernel arguments of the column walk over `nslots` resident slots.
+template <typename R>
+static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& blocks)
 {
     Slots& s = c->sl;
     const Grid& g = c->g;
@@ -1608,7 +1625,6 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     const size_t nbmax = (size_t)((g.nnr + 3) / 4) * ((g.nnx + 3) / 4) * ((g.nnz + 7) / 8);
     if ((size_t)np > nbmax)
         return fail(c, FMM3D_E_ARG, "solver: %d column patches exceed the per-slot tables (%zu)", np, nbmax);
-    ColArgs<R> a;
     a.T = (R*)s.T;
     a.slo = std::is_same<R, double>::value ? (const R*)c->slo64 : (const R*)c->slo32;
     a.hx = std::is_same<R, double>::value ? (const R*)c->hx64 : (const R*)c->hx32;
@@ -1621,19 +1637,34 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     a.slot_on = s.on;
     a.nnr = g.nnr; a.nnx = g.nnx; a.nnz = g.nnz;
     a.npx = npx; a.npz = npz;
-    a.nslots = nsrc;
+    a.nslots = nslots;
     a.fom = order == FMM3D_ORDER_MIXED;
     a.maxpass = c->col_passes;
     a.tol = std::is_same<R, double>::value ? c->tol64 : c->tol32;
     a.visits = c->d_visits;
     a.updates = c->d_visits + 1;
     a.nodes = c->d_visits + 2;
-    const bool ordered = c->ordered && !c->walk_alt;
-    a.prog = ordered ? s.prog : nullptr;
+    a.prog = c->ordered ? s.prog : nullptr;
+    a.walkalt = s.walkalt;
     a.flow = s.flow;
     a.spin_limit = c->spin_limit;
-    const int blocks = 8 * ((nsrc * npx + 7) / 8) * npz;   // rows of patches round-robin over 8 XCDs
     a.down = 0;
+    a.iter = 1;
+    blocks = 8 * ((nslots * npx + 7) / 8) * npz;   // rows of patches round-robin over 8 XCDs
+    return FMM3D_OK;
+}
+
+template <typename R, bool FAST, bool SHIFT>
+static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
+{
+    Slots& s = c->sl;
+    const Grid& g = c->g;
+    ColArgs<R> a;
+    int blocks = 0;
+    int rc0 = column_args<R>(c, nsrc, order, a, blocks);
+    if (rc0) return rc0;
+    const int np = a.npx * a.npz;
+    const bool ordered = a.prog != nullptr;
     if (ordered) {
         k_fill_int<<<(nsrc * np + 255) / 256, 256, 0, c->stream>>>(s.prog, nsrc * np, 0);
         HIPCHK(c, hipGetLastError());
@@ -1651,7 +1682,6 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
         const int chunk = c->max_sweeps > 0 ? std::min(FMM3D_CHECK, c->max_sweeps - k + 1) : FMM3D_CHECK;
         for (int j = 0; j < chunk; ++j, ++k) {
             a.iter = k;
-            a.down = c->walk_alt && !(k & 1);
             if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j], c->stream));
             hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks);
             if (le == hipErrorInvalidValue)
@@ -1697,6 +1727,7 @@ static int solve_run(fmm3d_ctx* c, int nsrc, int order, int precision)
 {
     Slots& s = c->sl;
     k_fill_real<double><<<(3 * nsrc + 255) / 256, 256, 0, c->stream>>>(s.srcpos, (size_t)3 * nsrc, __builtin_nan(""));
+    k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.walkalt, nsrc, c->walk_alt);
     HIPCHK(c, hipGetLastError());
     c->stats = fmm3d_stats{};
     c->stats.tile_r = c->brick_sl; c->stats.tile_x = c->tile_x; c->stats.tile_z = c->tile_z;
@@ -1928,6 +1959,95 @@ static void cart3(const Grid& g, double h, double th, double ph, double x[3])
     x[2] = r * cos(th);
 }
 
+// The refined neighbourhood of one point source: its fixed set for the
+// coarse grid (appended to node/tv).  Uses the context's refined sub-grid.
+static int point_fixed_set(fmm3d_ctx* c, const double* src3, int refine, int radius, int order,
+                           std::vector<int64_t>& node, std::vector<double>& tv)
+{
+    const Grid& g = c->g;
+    const int nn[3] = { g.nnr, g.nnx, g.nnz };
+    const double* velv = c->velv_host.data();
+    const double hs = src3[0], ths = src3[1], phs = src3[2];
+    const double u[3] = { (g.rgor - hs) / g.dnr, (ths - g.rgox) / g.dnx, (phs - g.rgoz) / g.dnz };
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        if (!(u[a] >= 0.0 && u[a] <= nn[a] - 1))
+            return fail(c, FMM3D_E_ARG, "point source (%g km, %g, %g rad) lies outside the grid", hs, ths, phs);
+        const int cc = std::min((int)floor(u[a]), nn[a] - 2);
+        lo[a] = std::max(0, cc - radius);
+        hi[a] = std::min(nn[a] - 1, cc + 1 + radius);
+    }
+    // the refined sub-grid over coarse nodes lo..hi
+    Grid f = g;
+    f.nnr = (hi[0] - lo[0]) * refine + 1;
+    f.nnx = (hi[1] - lo[1]) * refine + 1;
+    f.nnz = (hi[2] - lo[2]) * refine + 1;
+    f.rgor = g.rgor - lo[0] * g.dnr;
+    f.rgox = g.rgox + lo[1] * g.dnx;
+    f.rgoz = g.rgoz + lo[2] * g.dnz;
+    f.dnr = g.dnr / refine; f.dnx = g.dnx / refine; f.dnz = g.dnz / refine;
+    if ((double)f.nnr * f.nnx * f.nnz > 64.0e6)
+        return fail(c, FMM3D_E_ARG, "refined block of %dx%dx%d nodes is too large", f.nnr, f.nnx, f.nnz);
+    fmm3d_ctx* fc = c->fine;
+    if (fc && (fc->g.nnr != f.nnr || fc->g.nnx != f.nnx || fc->g.nnz != f.nnz)) {
+        fmm3d_destroy(fc);
+        c->fine = fc = nullptr;
+    }
+    if (!fc) {
+        fmm3d_ctx* n = new fmm3d_ctx();
+        n->g = f;
+        int rc = ctx_setup(n, c->device, &c->fine);
+        if (rc) return fail(c, rc, "refined grid: %s", fmm3d_last_error(nullptr));
+        fc = c->fine;
+    } else {
+        fc->g = f;
+        int rc = upload_metrics(fc);
+        if (rc) return fail(c, rc, "refined grid: %s", fc->err.c_str());
+    }
+    int rc = fmm3d_set_model_vtx(fc, velv);
+    if (rc) return fail(c, rc, "refined gridder: %s", fc->err.c_str());
+    // straight-ray times on the corners of the fine cell holding the source
+    int fcell[3];
+    const double fu[3] = { (f.rgor - hs) / f.dnr, (ths - f.rgox) / f.dnx, (phs - f.rgoz) / f.dnz };
+    const int fn[3] = { f.nnr, f.nnx, f.nnz };
+    for (int a = 0; a < 3; ++a) fcell[a] = std::min(std::max((int)floor(fu[a]), 0), fn[a] - 2);
+    double xs[3];
+    cart3(g, hs, ths, phs, xs);
+    const double ss = 1.0 / host_velocity(g, velv, hs, g.gor - hs, ths, phs);
+    int64_t fnode[8];
+    double ftime[8];
+    int m = 0;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 2; ++k, ++m) {
+                const int r = fcell[0] + i, x = fcell[1] + j, z = fcell[2] + k;
+                const double hn = f.rgor - r * f.dnr, thn = f.rgox + x * f.dnx, phn = f.rgoz + z * f.dnz;
+                double xn[3];
+                cart3(g, hn, thn, phn, xn);
+                const double d = sqrt((xn[0] - xs[0]) * (xn[0] - xs[0]) + (xn[1] - xs[1]) * (xn[1] - xs[1]) +
+                                      (xn[2] - xs[2]) * (xn[2] - xs[2]));
+                const double sn = 1.0 / host_velocity(f, velv, hn, f.gor - f.rgor + r * f.dnr, thn, phn);
+                fnode[m] = ((int64_t)r * f.nnx + x) * f.nnz + z;
+                ftime[m] = d * 0.5 * (ss + sn);
+            }
+    const int64_t foffs[2] = { 0, 8 };
+    rc = fmm3d_solve_fixed(fc, 1, foffs, fnode, ftime, order, FMM3D_PREC_FP64, FMM3D_WALK_ALTERNATE);
+    if (rc) return fail(c, rc, "refined solve: %s", fc->err.c_str());
+    std::vector<double> fine((size_t)f.nnr * f.nnx * f.nnz);
+    rc = fmm3d_get_field(fc, 0, fine.data());
+    if (rc) return fail(c, rc, "refined field: %s", fc->err.c_str());
+    // the coarse nodes the sub-grid covers seed the coarse solve
+    for (int r = lo[0]; r <= hi[0]; ++r)
+        for (int x = lo[1]; x <= hi[1]; ++x)
+            for (int z = lo[2]; z <= hi[2]; ++z) {
+                const size_t fq = ((size_t)(r - lo[0]) * refine * f.nnx + (size_t)(x - lo[1]) * refine) * f.nnz +
+                                  (size_t)(z - lo[2]) * refine;
+                node.push_back(((int64_t)r * g.nnx + x) * g.nnz + z);
+                tv.push_back(fine[fq]);
+            }
+    return FMM3D_OK;
+}
+
 extern "C" int fmm3d_solve_point(fmm3d_ctx* c, int nsrc, const double* src, int refine, int radius,
                                  int order, int precision, int flags)
 {
@@ -1939,90 +2059,11 @@ extern "C" int fmm3d_solve_point(fmm3d_ctx* c, int nsrc, const double* src, int 
     if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: bad precision %d", precision);
     if (flags != 0) return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: flags must be 0");
     if (!c->have_model) return fail(c, FMM3D_E_STATE, "fmm3d_solve_point: no velocity model (call fmm3d_set_model_vtx)");
-    const Grid& g = c->g;
-    const int nn[3] = { g.nnr, g.nnx, g.nnz };
     std::vector<int64_t> offs(nsrc + 1, 0), node;
-    std::vector<double> tv, fine;
-    const double* velv = c->velv_host.data();
+    std::vector<double> tv;
     for (int q = 0; q < nsrc; ++q) {
-        const double hs = src[3 * q], ths = src[3 * q + 1], phs = src[3 * q + 2];
-        const double u[3] = { (g.rgor - hs) / g.dnr, (ths - g.rgox) / g.dnx, (phs - g.rgoz) / g.dnz };
-        int lo[3], hi[3];
-        for (int a = 0; a < 3; ++a) {
-            if (!(u[a] >= 0.0 && u[a] <= nn[a] - 1))
-                return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: source %d lies outside the grid", q);
-            const int cc = std::min((int)floor(u[a]), nn[a] - 2);
-            lo[a] = std::max(0, cc - radius);
-            hi[a] = std::min(nn[a] - 1, cc + 1 + radius);
-        }
-        // the refined sub-grid over coarse nodes lo..hi
-        Grid f = g;
-        f.nnr = (hi[0] - lo[0]) * refine + 1;
-        f.nnx = (hi[1] - lo[1]) * refine + 1;
-        f.nnz = (hi[2] - lo[2]) * refine + 1;
-        f.rgor = g.rgor - lo[0] * g.dnr;
-        f.rgox = g.rgox + lo[1] * g.dnx;
-        f.rgoz = g.rgoz + lo[2] * g.dnz;
-        f.dnr = g.dnr / refine; f.dnx = g.dnx / refine; f.dnz = g.dnz / refine;
-        if ((double)f.nnr * f.nnx * f.nnz > 64.0e6)
-            return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: refined block of %dx%dx%d nodes is too large", f.nnr, f.nnx, f.nnz);
-        fmm3d_ctx* fc = c->fine;
-        if (fc && (fc->g.nnr != f.nnr || fc->g.nnx != f.nnx || fc->g.nnz != f.nnz)) {
-            fmm3d_destroy(fc);
-            c->fine = fc = nullptr;
-        }
-        if (!fc) {
-            fmm3d_ctx* n = new fmm3d_ctx();
-            n->g = f;
-            int rc = ctx_setup(n, c->device, &c->fine);
-            if (rc) return fail(c, rc, "fmm3d_solve_point: refined grid: %s", fmm3d_last_error(nullptr));
-            fc = c->fine;
-        } else {
-            fc->g = f;
-            int rc = upload_metrics(fc);
-            if (rc) return fail(c, rc, "fmm3d_solve_point: refined grid: %s", fc->err.c_str());
-        }
-        int rc = fmm3d_set_model_vtx(fc, velv);
-        if (rc) return fail(c, rc, "fmm3d_solve_point: refined gridder: %s", fc->err.c_str());
-        // straight-ray times on the corners of the fine cell holding the source
-        int fcell[3];
-        const double fu[3] = { (f.rgor - hs) / f.dnr, (ths - f.rgox) / f.dnx, (phs - f.rgoz) / f.dnz };
-        const int fn[3] = { f.nnr, f.nnx, f.nnz };
-        for (int a = 0; a < 3; ++a) fcell[a] = std::min(std::max((int)floor(fu[a]), 0), fn[a] - 2);
-        double xs[3];
-        cart3(g, hs, ths, phs, xs);
-        const double ss = 1.0 / host_velocity(g, velv, hs, g.gor - hs, ths, phs);
-        int64_t fnode[8];
-        double ftime[8];
-        int m = 0;
-        for (int i = 0; i < 2; ++i)
-            for (int j = 0; j < 2; ++j)
-                for (int k = 0; k < 2; ++k, ++m) {
-                    const int r = fcell[0] + i, x = fcell[1] + j, z = fcell[2] + k;
-                    const double hn = f.rgor - r * f.dnr, thn = f.rgox + x * f.dnx, phn = f.rgoz + z * f.dnz;
-                    double xn[3];
-                    cart3(g, hn, thn, phn, xn);
-                    const double d = sqrt((xn[0] - xs[0]) * (xn[0] - xs[0]) + (xn[1] - xs[1]) * (xn[1] - xs[1]) +
-                                          (xn[2] - xs[2]) * (xn[2] - xs[2]));
-                    const double sn = 1.0 / host_velocity(f, velv, hn, f.gor - f.rgor + r * f.dnr, thn, phn);
-                    fnode[m] = ((int64_t)r * f.nnx + x) * f.nnz + z;
-                    ftime[m] = d * 0.5 * (ss + sn);
-                }
-        const int64_t foffs[2] = { 0, 8 };
-        rc = fmm3d_solve_fixed(fc, 1, foffs, fnode, ftime, order, FMM3D_PREC_FP64, FMM3D_WALK_ALTERNATE);
-        if (rc) return fail(c, rc, "fmm3d_solve_point: refined solve: %s", fc->err.c_str());
-        fine.resize((size_t)f.nnr * f.nnx * f.nnz);
-        rc = fmm3d_get_field(fc, 0, fine.data());
-        if (rc) return fail(c, rc, "fmm3d_solve_point: refined field: %s", fc->err.c_str());
-        // the coarse nodes the sub-grid covers seed the coarse solve
-        for (int r = lo[0]; r <= hi[0]; ++r)
-            for (int x = lo[1]; x <= hi[1]; ++x)
-                for (int z = lo[2]; z <= hi[2]; ++z) {
-                    const size_t fq = ((size_t)(r - lo[0]) * refine * f.nnx + (size_t)(x - lo[1]) * refine) * f.nnz +
-                                      (size_t)(z - lo[2]) * refine;
-                    node.push_back(((int64_t)r * g.nnx + x) * g.nnz + z);
-                    tv.push_back(fine[fq]);
-                }
+        int rc = point_fixed_set(c, src + 3 * q, refine, radius, order, node, tv);
+        if (rc) return rc;
         offs[q + 1] = (int64_t)node.size();
     }
     int rc = fmm3d_solve_fixed(c, nsrc, offs.data(), node.data(), tv.data(), order, precision, FMM3D_WALK_ALTERNATE);
@@ -2191,6 +2232,220 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "frechet: %s", hipGetErrorString(e));
     if (short_cap) return fail(c, FMM3D_E_CAPACITY, "fmm3d_frechet: capacity %lld < %lld entries", (long long)cap, (long long)row_ptr[nrays]);
     return FMM3D_OK;
+}
+
+// ------------------------------------------------------------ engine
+
+int fmm3d_engine_open(fmm3d_ctx* c, int cap, int precision, int order)
+{
+    if (!c || cap <= 0) return fail(c, FMM3D_E_ARG, "engine: bad capacity");
+    if (!c->have_model) return fail(c, FMM3D_E_STATE, "engine: no model");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = ensure_slots(c, cap, precision);
+    if (rc) return rc;
+    Slots& s = c->sl;
+    const size_t nbmax = (size_t)((c->g.nnr + 3) / 4) * ((c->g.nnx + 3) / 4) * ((c->g.nnz + 7) / 8);
+    k_fill_int<<<(cap + 255) / 256, 256, 0, c->stream>>>(s.on, cap, 0);
+    k_fill_int<<<(cap + 255) / 256, 256, 0, c->stream>>>(s.lastchg, cap, 0);
+    k_fill_int<<<(cap + 255) / 256, 256, 0, c->stream>>>(s.valchg, cap, 0);
+    k_fill_int<<<(int)((cap * nbmax + 255) / 256), 256, 0, c->stream>>>(s.prog, (int)(cap * nbmax), 0);
+    k_fill_int<<<(int)((cap * nbmax + 255) / 256), 256, 0, c->stream>>>(s.stamp, (int)(cap * nbmax), -1);
+    k_fill_real<double><<<(3 * cap + 255) / 256, 256, 0, c->stream>>>(s.srcpos, (size_t)3 * cap, __builtin_nan(""));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!c->eng_lc) HIPCHK(c, hipHostMalloc((void**)&c->eng_lc, sizeof(int) * 65536, hipHostMallocDefault));
+    if (cap > 65536) return fail(c, FMM3D_E_ARG, "engine: at most 65536 slots");
+    c->eng_open = true;
+    c->eng_k = 1;
+    c->eng_order = order;
+    c->eng_pending = 0;
+    c->eng_active.assign(cap, 0);
+    c->eng_born.assign(cap, 0);
+    s.n = cap;
+    c->stats = fmm3d_stats{};
+    return FMM3D_OK;
+}
+
+int fmm3d_engine_active(const fmm3d_ctx* c)
+{
+    int n = 0;
+    for (int v : c->eng_active) n += v;
+    return n;
+}
+
+int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double* tv, int64_t n, int walkalt,
+                       const double* src3)
+{
+    if (!c || !c->eng_open || slot < 0 || slot >= c->sl.cap || c->eng_active[slot] || n < 0)
+        return fail(c, FMM3D_E_ARG, "engine: bad admission");
+    Slots& s = c->sl;
+    double m = __builtin_huge_val();
+    for (int64_t i = 0; i < n; ++i) {
+        if (node[i] < 0 || (size_t)node[i] >= c->N || !(tv[i] >= 0.0) || !isfinite(tv[i]))
+            return fail(c, FMM3D_E_ARG, "engine: bad fixed node %lld", (long long)node[i]);
+        m = std::min(m, tv[i]);
+    }
+    const double toff = isfinite(m) ? m : 0.0;
+    int flow[2] = { 1, 1 };
+    fixed_flow(c->g, node, tv, n, flow);
+    if (c->eng_stage < (size_t)std::max<int64_t>(n, 1)) {
+        (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv);
+        c->eng_node = nullptr; c->eng_tv = nullptr; c->eng_stage = 0;
+        const size_t want = (size_t)std::max<int64_t>(n, 1 << 16);
+        HIPCHK(c, dalloc(&c->eng_node, want * sizeof(int64_t)));
+        HIPCHK(c, dalloc(&c->eng_tv, want * sizeof(double)));
+        c->eng_stage = want;
+    }
+    // the previous launch may still read this slot's arrays: admissions wait for it
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t N = c->N, NW = (N + 31) / 32;
+    const Grid& g = c->g;
+    const int npx = (g.nnx + c->col_px - 1) / c->col_px, npz = (g.nnz + c->col_pz - 1) / c->col_pz;
+    const int np = npx * npz;
+    const int64_t offs[2] = { 0, n };
+    int64_t* doffs = nullptr;
+    HIPCHK(c, dalloc(&doffs, 2 * sizeof(int64_t)));
+    hipError_t e = hipMemcpyAsync(doffs, offs, sizeof offs, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && n > 0) e = hipMemcpyAsync(c->eng_node, node, n * sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && n > 0) e = hipMemcpyAsync(c->eng_tv, tv, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.toff + slot, &toff, sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.flow + 2 * slot, flow, sizeof flow, hipMemcpyHostToDevice, c->stream);
+    double sp[3] = { __builtin_nan(""), __builtin_nan(""), __builtin_nan("") };
+    if (src3) { sp[0] = src3[0]; sp[1] = src3[1]; sp[2] = src3[2]; }
+    if (e == hipSuccess) e = hipMemcpyAsync(s.srcpos + 3 * slot, sp, sizeof sp, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        if (s.prec == FMM3D_PREC_FP64) {
+            double* T = (double*)s.T + (size_t)slot * N;
+            k_fill_real<double><<<1024, 256, 0, c->stream>>>(T, N, Lim<double>::inf());
+            k_fill_u32<<<1024, 256, 0, c->stream>>>(s.fix + (size_t)slot * NW, NW);
+            if (n > 0) k_scatter_fixed<double><<<(int)std::min<int64_t>(1024, (n + 255) / 256), 256, 0, c->stream>>>(
+                1, N, doffs, c->eng_node, c->eng_tv, s.toff + slot, T, s.fix + (size_t)slot * NW);
+        } else {
+            float* T = (float*)s.T + (size_t)slot * N;
+            k_fill_real<float><<<1024, 256, 0, c->stream>>>(T, N, Lim<float>::inf());
+            k_fill_u32<<<1024, 256, 0, c->stream>>>(s.fix + (size_t)slot * NW, NW);
+            if (n > 0) k_scatter_fixed<float><<<(int)std::min<int64_t>(1024, (n + 255) / 256), 256, 0, c->stream>>>(
+                1, N, doffs, c->eng_node, c->eng_tv, s.toff + slot, T, s.fix + (size_t)slot * NW);
+        }
+        // every patch column of the new slot runs in the next sweep
+        k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.stamp + (size_t)slot * np, np, c->eng_k - 1);
+        k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.prog + (size_t)slot * np, np, 0);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.lastchg + slot, 1, c->eng_k);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.walkalt + slot, 1, walkalt ? 1 : 0);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.on + slot, 1, 1);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(doffs);
+    if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "engine admit: %s", hipGetErrorString(e));
+    c->eng_active[slot] = 1;
+    c->eng_born[slot] = c->eng_k;
+    return FMM3D_OK;
+}
+
+template <typename R, bool FAST, bool SHIFT>
+static int engine_sweeps(fmm3d_ctx* c, int nsweeps)
+{
+    ColArgs<R> a;
+    int blocks = 0;
+    int rc = column_args<R>(c, c->sl.cap, c->eng_order, a, blocks);
+    if (rc) return rc;
+    for (int j = 0; j < nsweeps; ++j) {
+        a.iter = c->eng_k++;
+        hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks);
+        if (le == hipErrorInvalidValue) return fail(c, FMM3D_E_ARG, "engine: unsupported column patch");
+        HIPCHK(c, le);
+        c->stats.kernel_launches++;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->eng_lc, c->sl.lastchg, c->sl.cap * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    return FMM3D_OK;
+}
+
+int fmm3d_engine_launch(fmm3d_ctx* c, int nsweeps)
+{
+    if (!c || !c->eng_open || nsweeps <= 0) return fail(c, FMM3D_E_ARG, "engine: bad launch");
+    if (c->eng_pending) return fail(c, FMM3D_E_STATE, "engine: collect the previous launch first");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = c->sl.prec == FMM3D_PREC_FP64 ? engine_sweeps<double, false, false>(c, nsweeps)
+                                           : engine_sweeps<float, true, true>(c, nsweeps);
+    if (rc) return rc;
+    c->eng_pending = 1;
+    return FMM3D_OK;
+}
+
+int fmm3d_engine_collect(fmm3d_ctx* c, std::vector<int>& done, std::vector<int>& failed)
+{
+    done.clear();
+    failed.clear();
+    if (!c || !c->eng_open) return fail(c, FMM3D_E_ARG, "engine: not open");
+    if (!c->eng_pending) return FMM3D_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->eng_pending = 0;
+    const Grid& g = c->g;
+    const int maxit = 4 * (g.nnr + g.nnx + g.nnz) + 200;
+    const int last = c->eng_k - 1;      // the sweep just run
+    for (int q = 0; q < c->sl.cap; ++q) {
+        if (!c->eng_active[q]) continue;
+        if (c->eng_lc[q] < last && last > c->eng_born[q]) done.push_back(q);
+        else if (last - c->eng_born[q] > maxit) failed.push_back(q);
+    }
+    return FMM3D_OK;
+}
+
+int fmm3d_engine_receivers(fmm3d_ctx* c, int slot, int nrc, const double* rcv, float* out)
+{
+    if (!c || !c->eng_open || slot < 0 || slot >= c->sl.cap || nrc < 0) return fail(c, FMM3D_E_ARG, "engine: bad receivers");
+    if (nrc == 0) return FMM3D_OK;
+    double* drcv = nullptr;
+    float* dout = nullptr;
+    HIPCHK(c, dalloc(&drcv, (size_t)nrc * 3 * sizeof(double)));
+    HIPCHK(c, dalloc(&dout, (size_t)nrc * sizeof(float)));
+    hipError_t e = hipMemcpyAsync(drcv, rcv, (size_t)nrc * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        if (c->sl.prec == FMM3D_PREC_FP64)
+            k_receivers<double><<<(nrc + 255) / 256, 256, 0, c->stream>>>(c->g, (const double*)c->sl.T + (size_t)slot * c->N,
+                                                                        c->sl.toff + slot, 1, nrc, drcv, dout);
+        else
+            k_receivers<float><<<(nrc + 255) / 256, 256, 0, c->stream>>>(c->g, (const float*)c->sl.T + (size_t)slot * c->N,
+                                                                       c->sl.toff + slot, 1, nrc, drcv, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)nrc * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(drcv); (void)hipFree(dout);
+    if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "engine receivers: %s", hipGetErrorString(e));
+    return FMM3D_OK;
+}
+
+int fmm3d_engine_retire(fmm3d_ctx* c, int slot)
+{
+    if (!c || !c->eng_open || slot < 0 || slot >= c->sl.cap) return fail(c, FMM3D_E_ARG, "engine: bad retire");
+    k_fill_int<<<1, 1, 0, c->stream>>>(c->sl.on + slot, 1, 0);
+    HIPCHK(c, hipGetLastError());
+    c->eng_active[slot] = 0;
+    return FMM3D_OK;
+}
+
+int fmm3d_engine_point_set(fmm3d_ctx* c, const double* src3, int refine, int radius, int order,
+                           std::vector<int64_t>& node, std::vector<double>& tv)
+{
+    if (!c || !c->have_model) return fail(c, FMM3D_E_STATE, "engine: no model");
+    if (refine < 1 || refine > 16 || radius < 1 || radius > 64) return fail(c, FMM3D_E_ARG, "engine: bad refinement");
+    return point_fixed_set(c, src3, refine, radius, order, node, tv);
+}
+
+size_t fmm3d_engine_model_bytes(const fmm3d_grid_desc* d)
+{
+    const size_t N = (size_t)d->nnr * d->nnx * d->nnz;
+    const size_t NH = (size_t)(d->nnr + 3) * (d->nnx + 3) * (d->nnz + 3);
+    return NH * 8 + N * 12 + (size_t)d->nnr * d->nnx * 12 + ((size_t)64 << 20);   // + refined sub-grid allowance
+}
+
+size_t fmm3d_engine_slot_bytes(const fmm3d_grid_desc* d, int precision)
+{
+    const size_t N = (size_t)d->nnr * d->nnx * d->nnz;
+    const size_t nbmax = (size_t)((d->nnr + 3) / 4) * ((d->nnx + 3) / 4) * ((d->nnz + 7) / 8);
+    return N * (precision == FMM3D_PREC_FP64 ? 8 : 4) + (N + 31) / 32 * 4 + nbmax * 8 + 64;
 }
 
 extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)

@@ -28,7 +28,7 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["fmm3d_solver.hip"]
+SOURCES = ["fmm3d_solver.hip", "fmm3d_server.hip"]
 
 
 def build_hip(verbose: bool = False) -> str:

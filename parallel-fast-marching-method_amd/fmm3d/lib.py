@@ -17,13 +17,16 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libfmm3d_hip.so")
 
 OK = 0
-E_ARG, E_DEVICE, E_NOMEM, E_NOCONV, E_STATE, E_CAPACITY = -1, -2, -3, -4, -5, -6
+E_ARG, E_DEVICE, E_NOMEM, E_NOCONV, E_STATE, E_CAPACITY, E_TIMEOUT = -1, -2, -3, -4, -5, -6, -7
+SRC_TELE, SRC_POINT = 0, 1
 ORDER_FIRST, ORDER_MIXED = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 WALK_ALTERNATE = 1
 
 EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model_vtx",
            "fmm3d_solve_fixed", "fmm3d_solve_point",
+           "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_result",
+           "fmm3d_server_stats", "fmm3d_server_destroy",
            "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_frechet", "fmm3d_get_field",
            "fmm3d_get_stats", "fmm3d_set_option")
 
@@ -44,6 +47,19 @@ class StatsC(C.Structure):
                 ("kernel_ms", C.c_double), ("kernel_launches", C.c_int64),
                 ("phase_sweeps", C.c_int32 * 6), ("brick_visits", C.c_int64),
                 ("node_visits", C.c_int64)]
+
+
+class RequestC(C.Structure):
+    _fields_ = [("grid", GridDescC), ("velv", C.c_void_p), ("model_id", C.c_uint64), ("kind", C.c_int),
+                ("nsrc", C.c_int), ("faces", C.c_void_p), ("src", C.c_void_p), ("refine", C.c_int),
+                ("radius", C.c_int), ("nrc", C.c_int), ("rcv", C.c_void_p), ("order", C.c_int),
+                ("precision", C.c_int)]
+
+
+class ServerInfoC(C.Structure):
+    _fields_ = [("requests_done", C.c_int64), ("sources_done", C.c_int64), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("source_solves_per_s", C.c_double), ("hbm_bytes", C.c_double),
+                ("models_resident", C.c_int), ("sources_in_flight", C.c_int)]
 
 
 class FmmError(RuntimeError):
@@ -75,6 +91,12 @@ def load(path: Optional[str] = None):
     L.fmm3d_last_error.restype = C.c_char_p
     L.fmm3d_set_model_vtx.argtypes = [vp, dp]
     L.fmm3d_solve_tele.argtypes = [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int]
+    L.fmm3d_server_create.argtypes = [C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
+    L.fmm3d_server_submit.argtypes = [vp, C.POINTER(RequestC), C.POINTER(C.c_int64)]
+    L.fmm3d_server_wait.argtypes = [vp, C.c_int64, C.c_double]
+    L.fmm3d_server_result.argtypes = [vp, C.c_int64, fp, C.POINTER(C.c_double)]
+    L.fmm3d_server_stats.argtypes = [vp, C.POINTER(ServerInfoC)]
+    L.fmm3d_server_destroy.argtypes = [vp]
     L.fmm3d_solve_fixed.argtypes = [vp, C.c_int, lp, lp, dp, C.c_int, C.c_int, C.c_int]
     L.fmm3d_solve_point.argtypes = [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L.fmm3d_receivers.argtypes = [vp, C.c_int, dp, fp]
@@ -201,3 +223,75 @@ class Context:
         d = {f: getattr(s, f) for f, _ in StatsC._fields_}
         d["phase_sweeps"] = list(d["phase_sweeps"])
         return d
+
+
+class Server:
+    """The serving loop (fmm3d_server_*): submit requests, wait, fetch results."""
+
+    def __init__(self, device: int = 0, hbm_budget: float = 64e9, max_slots: int = 64):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.fmm3d_server_create(device, float(hbm_budget), max_slots, C.byref(h))
+        if rc != OK:
+            raise FmmError(rc, "fmm3d_server_create failed")
+        self._h = h
+        self._keep = {}
+
+    def submit(self, g, velv, model_id: int, *, faces=None, src=None, refine: int = 5, radius: int = 10,
+               rcv=None, order: int = ORDER_MIXED, precision: int = PREC_FP32) -> int:
+        velv = np.ascontiguousarray(velv, dtype=np.float64)
+        rcv = np.ascontiguousarray(np.zeros((0, 3)) if rcv is None else rcv, dtype=np.float64)
+        r = RequestC()
+        r.grid = grid_c(g)
+        r.velv = velv.ctypes.data
+        r.model_id = model_id
+        if faces is not None:
+            faces = np.ascontiguousarray(faces, dtype=np.float64)
+            if faces.ndim == 2:
+                faces = faces[None]
+            r.kind, r.nsrc, r.faces = SRC_TELE, faces.shape[0], faces.ctypes.data
+            keep = (velv, rcv, faces)
+        else:
+            src = np.ascontiguousarray(np.atleast_2d(src), dtype=np.float64)
+            r.kind, r.nsrc, r.src = SRC_POINT, src.shape[0], src.ctypes.data
+            keep = (velv, rcv, src)
+        r.refine, r.radius = refine, radius
+        r.nrc, r.rcv = rcv.shape[0], rcv.ctypes.data if rcv.size else None
+        r.order, r.precision = order, precision
+        t = C.c_int64()
+        rc = self.L.fmm3d_server_submit(self._h, C.byref(r), C.byref(t))
+        if rc != OK:
+            raise FmmError(rc, "fmm3d_server_submit rejected the request")
+        self._keep[t.value] = (r.nsrc, r.nrc)
+        return t.value
+
+    def wait(self, ticket: int, timeout_s: float = 60.0) -> bool:
+        rc = self.L.fmm3d_server_wait(self._h, ticket, timeout_s)
+        if rc not in (OK, E_TIMEOUT):
+            raise FmmError(rc, "fmm3d_server_wait failed")
+        return rc == OK
+
+    def result(self, ticket: int):
+        nsrc, nrc = self._keep.pop(ticket)
+        out = np.zeros((nsrc, nrc), np.float32)
+        lat = C.c_double()
+        rc = self.L.fmm3d_server_result(self._h, ticket, out if out.size else np.zeros(1, np.float32), C.byref(lat))
+        if rc != OK:
+            raise FmmError(rc, f"request {ticket} failed")
+        return out, lat.value
+
+    def stats(self) -> dict:
+        s = ServerInfoC()
+        self.L.fmm3d_server_stats(self._h, C.byref(s))
+        return {k: getattr(s, k) for k, _ in ServerInfoC._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.fmm3d_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
