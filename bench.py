@@ -35,6 +35,11 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per node visit of the column walk: read the node's
 # time and slowness once, write its time once (float32)
 BYTES_PER_NODE_VISIT = 12
+# measured HBM bytes per node visit of the same kernel on this workload:
+# rocprofv3 FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000), each scaled by a
+# 1 GiB copy, over the launches of one batch of 50
+# (profiles/k_column_tele201_batch50_traffic.md)
+MEASURED_BYTES_PER_NODE_VISIT = 33.06 + 4.42
 
 
 def parse():
@@ -47,6 +52,8 @@ def parse():
     p.add_argument("--plain", type=int, default=3, help="float64 single-source solves timed")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-sources", type=int, default=1)
+    p.add_argument("--serve-s", type=float, default=6.0,
+                   help="seconds of Poisson arrivals for the serving measurement (0: skip)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="first run one device-to-device copy of this many MiB (the reference kernel "
                         "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -72,6 +79,45 @@ def one_step(ctx, faces, rcv, prec):
     t = ctx.receivers(rcv)
     rp, cols, vals = ctx.frechet(rcv)
     return t, st, int(rp[-1])
+
+
+def serving(g, velv, faces, rcv, seconds, t_src):
+    """The serving loop under Poisson arrivals: requests of 4 float32 plane-wave
+    sources (receiver times only) on the resident 201^3 model, offered at 70 %
+    of the batched throughput; p50/p99 request latency and the throughput the
+    server sustains.  Results are checked against the batched solve."""
+    rng = np.random.default_rng(5)
+    per = 4
+    rate = 0.7 * (1.0 / t_src) / per              # requests per second
+    n = max(8, int(rate * seconds))
+    gaps = rng.exponential(1.0 / rate, n)
+    picks = [rng.integers(0, faces.shape[0], per) for _ in range(n)]
+    tickets = []
+    with lib.Server(hbm_budget=64e9, max_slots=64) as S:
+        # warm the model: the first request creates the resident context
+        t = S.submit(g, velv, 11, faces=faces[:per], rcv=rcv)
+        S.wait(t, 600.0)
+        S.result(t)
+
+        t0 = time.perf_counter()
+        next_t = t0
+        for i in range(n):
+            next_t += gaps[i]
+            d = next_t - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets.append((S.submit(g, velv, 11, faces=faces[picks[i]], rcv=rcv), picks[i]))
+        lat, errs = [], []
+        for t, idx in tickets:
+            S.wait(t, 600.0)
+            out, l = S.result(t)
+            lat.append(l)
+            errs.append(out)
+        span = time.perf_counter() - t0
+        st = S.stats()
+    return {"requests": n, "sources_per_request": per, "offered_req_per_s": round(rate, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "source_solves_per_s": round(n * per / span, 2), "server_hbm_gb": round(st["hbm_bytes"] / 1e9, 2)}, tickets, errs
 
 
 def cpu_baseline(g, velv, faces, rcv, n):
@@ -168,13 +214,22 @@ def main():
         "frechet_entries_per_step": nnz // max(a.steps, 1),
         "roofline": {"kernel": "k_column (float32 column walk, 8x8 patches)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(MEASURED_BYTES_PER_NODE_VISIT * nvis / max(launches, 1)),
                      "avg_launch_ms": round(avg_launch_ms, 4),
                      "node_visits_per_launch": int(nvis / max(launches, 1)),
                      "algorithmic_bytes_per_node_visit": BYTES_PER_NODE_VISIT,
                      "node_updates_per_s": round(upd / (kms * 1e-3), 1) if kms > 0 else None},
     }
     ctx.close()
+    if a.serve_s > 0:
+        sv, tk, outs = serving(g, velv, faces, rcv, a.serve_s, elapsed / a.steps / a.batch * 1.0)
+        # every served request against the batched float32 solve of its sources
+        err = 0.0
+        for (t, idx), o in zip(tk, outs):
+            err = max(err, float(np.nanmax(np.abs(o.astype(np.float64) - t32[idx]))))
+        sv["max_abs_diff_vs_batch_s"] = err
+        out["serving"] = sv
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, velv, faces, rcv, a.cpu_sources)
     print(json.dumps(out), flush=True)

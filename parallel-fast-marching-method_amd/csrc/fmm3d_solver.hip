@@ -2288,10 +2288,10 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
     const double toff = isfinite(m) ? m : 0.0;
     int flow[2] = { 1, 1 };
     fixed_flow(c->g, node, tv, n, flow);
-    if (c->eng_stage < (size_t)std::max<int64_t>(n, 1)) {
+    if (c->eng_stage < (size_t)n + 2) {
         (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv);
         c->eng_node = nullptr; c->eng_tv = nullptr; c->eng_stage = 0;
-        const size_t want = (size_t)std::max<int64_t>(n, 1 << 16);
+        const size_t want = (size_t)std::max<int64_t>(n + 2, 1 << 16);
         HIPCHK(c, dalloc(&c->eng_node, want * sizeof(int64_t)));
         HIPCHK(c, dalloc(&c->eng_tv, want * sizeof(double)));
         c->eng_stage = want;
@@ -2303,8 +2303,7 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
     const int npx = (g.nnx + c->col_px - 1) / c->col_px, npz = (g.nnz + c->col_pz - 1) / c->col_pz;
     const int np = npx * npz;
     const int64_t offs[2] = { 0, n };
-    int64_t* doffs = nullptr;
-    HIPCHK(c, dalloc(&doffs, 2 * sizeof(int64_t)));
+    int64_t* doffs = c->eng_node + c->eng_stage - 2;   // the staging's last two entries hold the offsets
     hipError_t e = hipMemcpyAsync(doffs, offs, sizeof offs, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && n > 0) e = hipMemcpyAsync(c->eng_node, node, n * sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && n > 0) e = hipMemcpyAsync(c->eng_tv, tv, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
@@ -2336,7 +2335,6 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(doffs);
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "engine admit: %s", hipGetErrorString(e));
     c->eng_active[slot] = 1;
     c->eng_born[slot] = c->eng_k;
