@@ -163,6 +163,11 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *                   their upstream neighbours (along the source's lateral
  *                   flow) finished it; 0 no ordering
  *   "spin_limit"    polls before such a wait is given up (default 20000)
+ *   "graph"         1 (default): replay 4-sweep chunks from a captured hipGraph
+ *   "autotune"      1: the first solve of a kind (precision, order, walk,
+ *                   batch size class) times patch shapes 8x8/8x16/16x8/16x16
+ *                   with 8 or 16 passes on the real batch and keeps the
+ *                   fastest (stats.tile_x/tile_z/tile_r report the choice)
  *   "max_sweeps"    column walk: stop after this many sweeps (0 = at
  *                   convergence); a diagnostic, the field is then approximate
  *   "timing"        1: time every relaxation launch with events (stats.kernel_ms)

@@ -33,6 +33,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -387,6 +388,7 @@ template <typename R> struct ColArgs {
     const int* flow;        // per slot: lateral flow signs (fx, fz), or null
     int spin_limit;         // polls before a wait on a neighbour is given up
     int down;               // 1: walk top to bottom (when walkalt is null)
+    const int* iterp;       // sweep number from device memory (graph replays), or null: iter
     const int* walkalt;     // per slot: 1 = alternate the walk (down on even sweeps), or null
 };
 
@@ -499,6 +501,8 @@ k_column(ColArgs<R> a)
     if (row >= a.nslots * a.npx) return;
     const int slot = row / a.npx;
     if (!a.slot_on[slot]) return;
+    // sweep number: an argument, or a device counter (graph replays)
+    const int iter = a.iterp ? __hip_atomic_load(a.iterp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : a.iter;
     const int fx = a.flow ? a.flow[2 * slot] : 1, fz = a.flow ? a.flow[2 * slot + 1] : 1;
     const int rx = row - slot * a.npx, rz = k % a.npz;
     const int bx = fx > 0 ? rx : a.npx - 1 - rx;
@@ -506,9 +510,9 @@ k_column(ColArgs<R> a)
     const int b = bx * a.npz + bz;
     int* st = a.stamp + (size_t)slot * np;
     const bool alt = a.walkalt && a.walkalt[slot];
-    const bool down = a.walkalt ? (alt && !(a.iter & 1)) : (a.down != 0);
+    const bool down = a.walkalt ? (alt && !(iter & 1)) : (a.down != 0);
     int* pg = (a.prog && !alt) ? a.prog + (size_t)slot * np : nullptr;
-    const int pbase = a.iter * (a.nnr + 1);
+    const int pbase = iter * (a.nnr + 1);
     const int t = threadIdx.x;
     if (t == 0) {
         int m = st[b];
@@ -516,7 +520,7 @@ k_column(ColArgs<R> a)
         if (bx < a.npx - 1) m = max(m, st[b + a.npz]);
         if (bz > 0) m = max(m, st[b - 1]);
         if (bz < a.npz - 1) m = max(m, st[b + 1]);
-        s_act = (m >= a.iter - 1);
+        s_act = (m >= iter - 1);
         // an idle patch is finished with every layer of this sweep
         if (!s_act && pg) __hip_atomic_store(pg + b, pbase + a.nnr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -681,8 +685,8 @@ k_column(ColArgs<R> a)
     }
     const bool blk = __syncthreads_or(anyv);
     if (t == 0 && blk) {
-        st[b] = a.iter;
-        atomicMax(a.lastchg + slot, a.iter);
+        st[b] = iter;
+        atomicMax(a.lastchg + slot, iter);
         atomicOr(a.valchg + slot, 1);
     }
 }
@@ -1052,6 +1056,13 @@ struct fmm3d_ctx {
     int strategy = 1;              // 0: brick sweeps in three phases; 1: column walk, one phase
     int col_px = 8, col_pz = 8;    // column-walk patch (theta x phi)
     int col_passes = 16;           // line passes per layer at most
+    int use_graph = 1;             // replay sweep chunks from a captured hipGraph
+    int autotune = 0;              // time candidate patch shapes / pass caps at first use
+    std::vector<std::array<int, 8>> tuned;   // key (prec, order, walk, batch class) -> (px, pz, passes)
+    int* d_iter = nullptr;         // device sweep counter (graph replays)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<char> g_sig;       // kernel arguments, grid and type the graph was captured with
     int walk_alt = 0;              // this solve: alternate the column walk direction per sweep
     std::vector<double> velv_host; // the vertex model (for refined sub-grids)
     fmm3d_ctx* fine = nullptr;     // refined source neighbourhood (point sources)
@@ -1060,6 +1071,8 @@ struct fmm3d_ctx {
     int eng_k = 1, eng_order = FMM3D_ORDER_MIXED, eng_pending = 0;
     std::vector<int> eng_active, eng_born;
     int* eng_lc = nullptr;         // pinned host copy of the per-slot last-change sweep
+    double* d_faces = nullptr;     // boundary faces of the last teleseismic solve
+    size_t d_faces_n = 0;
     int64_t* eng_node = nullptr;   // device staging of an admission's fixed set
     double* eng_tv = nullptr;
     size_t eng_stage = 0;
@@ -1215,6 +1228,7 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     CREATECHK(dalloc(&c->slo32, c->N * sizeof(float)));
     CREATECHK(dalloc(&c->d_updates, sizeof(int64_t)));
     CREATECHK(dalloc(&c->d_visits, 3 * sizeof(unsigned long long)));
+    CREATECHK(dalloc(&c->d_iter, sizeof(int)));
     rc = upload_metrics(c);
     if (rc) { fmm3d_destroy(c); return rc; }
 #undef CREATECHK
@@ -1232,6 +1246,10 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     if (od) c->ordered = atoi(od) != 0;
     const char* sp = getenv("FMM3D_SPIN");
     if (sp) c->spin_limit = std::max(0, atoi(sp));
+    const char* at = getenv("FMM3D_AUTOTUNE");
+    if (at) c->autotune = atoi(at) != 0;
+    const char* gv = getenv("FMM3D_GRAPH");
+    if (gv) c->use_graph = atoi(gv) != 0;
     const char* tr = getenv("FMM3D_TRACE");
     if (tr) c->trace = atoi(tr);
     const char* sg = getenv("FMM3D_STRATEGY");   // "brick" | "column[:PXxPZ]"
@@ -1265,6 +1283,10 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->fine) { fmm3d_destroy(c->fine); c->fine = nullptr; }
     if (c->eng_lc) (void)hipHostFree(c->eng_lc);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    (void)hipFree(c->d_iter);
+    (void)hipFree(c->d_faces);
     (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv);
     free_slots(c->sl);
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
@@ -1447,6 +1469,12 @@ __global__ void k_pack_rows(int nr, int64_t rcap, const int32_t* __restrict__ co
             pvals[dst + i] = svals[src + i];
         }
     }
+}
+
+// the next sweep number, for graph replays of a sweep chunk
+__global__ void k_next_iter(int* p)
+{
+    if (threadIdx.x == 0) p[0] = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
 }
 
 __global__ void k_fill_int(int* p, int n, int v)
@@ -1654,6 +1682,96 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     return FMM3D_OK;
 }
 
+static bool column_supported(int px, int pz)
+{
+    static const int ok[][2] = { { 16, 16 }, { 16, 32 }, { 32, 16 }, { 32, 32 }, { 16, 64 }, { 64, 16 }, { 8, 64 },
+                                 { 64, 8 }, { 8, 32 }, { 32, 8 }, { 8, 16 }, { 16, 8 }, { 8, 8 } };
+    for (auto& q : ok)
+        if (q[0] == px && q[1] == pz) return true;
+    return false;
+}
+
+// Queue `n` sweeps starting at sweep k.  A full chunk replays a captured
+// hipGraph (captured once per argument set: n column-walk launches, each
+// followed by k_next_iter, the sweep number living in c->d_iter; kernel
+// nodes only); other counts, and tracing, launch directly.  With timing on,
+// each launch (or the whole replay, shared out evenly) is timed with events.
+template <typename R, bool FAST, bool SHIFT>
+static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k)
+{
+    if (!column_supported(c->col_px, c->col_pz))
+        return fail(c, FMM3D_E_ARG, "solver: unsupported column patch %dx%d", c->col_px, c->col_pz);
+    if (!c->use_graph || n != FMM3D_CHECK || c->trace) {
+        a.iterp = nullptr;
+        for (int j = 0; j < n; ++j) {
+            a.iter = k + j;
+            if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j], c->stream));
+            hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks);
+            HIPCHK(c, le);
+            if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j + 1], c->stream));
+            c->stats.kernel_launches++;
+            if (c->trace) {
+                unsigned long long v[2];
+                HIPCHK(c, hipMemcpyAsync(v, c->d_visits, sizeof v, hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                fprintf(stderr, "fmm3d trace: sweep %d visits %llu updates %llu\n", k + j, v[0], v[1]);
+            }
+        }
+        if (c->timing) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            for (int j = 0; j < n; ++j) {
+                float ms = 0.0f;
+                HIPCHK(c, hipEventElapsedTime(&ms, c->kev[2 * j], c->kev[2 * j + 1]));
+                c->stats.kernel_ms += ms;
+            }
+        }
+        return FMM3D_OK;
+    }
+    a.iter = 0;
+    a.iterp = c->d_iter;
+    std::vector<char> sig(sizeof a + 2 * sizeof(int));
+    memcpy(sig.data(), &a, sizeof a);
+    const int tag[2] = { blocks, (int)sizeof(R) * 100 + c->col_px * 10000 + c->col_pz };
+    memcpy(sig.data() + sizeof a, tag, sizeof tag);
+    if (!c->gexec || sig != c->g_sig) {
+        if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+        if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (int j = 0; j < n && e == hipSuccess; ++j) {
+            e = launch_column<R, FAST, SHIFT>(c, a, blocks);
+            if (e == hipSuccess) {
+                k_next_iter<<<1, 64, 0, c->stream>>>(c->d_iter);
+                e = hipGetLastError();
+            }
+        }
+        hipGraph_t gr = nullptr;
+        hipError_t e2 = hipStreamEndCapture(c->stream, &gr);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec, gr, nullptr, nullptr, 0);
+        c->graph = gr;
+        if (e != hipSuccess) {
+            if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+            c->gexec = nullptr;
+            return fail(c, FMM3D_E_DEVICE, "solver: graph capture: %s", hipGetErrorString(e));
+        }
+        c->g_sig = sig;
+    }
+    k_fill_int<<<1, 1, 0, c->stream>>>(c->d_iter, 1, k);
+    HIPCHK(c, hipGetLastError());
+    if (c->timing) HIPCHK(c, hipEventRecord(c->kev[0], c->stream));
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    if (c->timing) {
+        HIPCHK(c, hipEventRecord(c->kev[1], c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->kev[0], c->kev[1]));
+        c->stats.kernel_ms += ms;
+    }
+    c->stats.kernel_launches += n;
+    return FMM3D_OK;
+}
+
 template <typename R, bool FAST, bool SHIFT>
 static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
 {
@@ -1680,30 +1798,11 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     bool done = false;
     while (!done) {
         const int chunk = c->max_sweeps > 0 ? std::min(FMM3D_CHECK, c->max_sweeps - k + 1) : FMM3D_CHECK;
-        for (int j = 0; j < chunk; ++j, ++k) {
-            a.iter = k;
-            if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j], c->stream));
-            hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks);
-            if (le == hipErrorInvalidValue)
-                return fail(c, FMM3D_E_ARG, "solver: unsupported column patch %dx%d", c->col_px, c->col_pz);
-            HIPCHK(c, le);
-            if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j + 1], c->stream));
-            c->stats.kernel_launches++;
-            if (c->trace) {
-                unsigned long long v[2];
-                HIPCHK(c, hipMemcpyAsync(v, c->d_visits, sizeof v, hipMemcpyDeviceToHost, c->stream));
-                HIPCHK(c, hipStreamSynchronize(c->stream));
-                fprintf(stderr, "fmm3d trace: sweep %d visits %llu updates %llu\n", k, v[0], v[1]);
-            }
-        }
+        int rcl = launch_chunk<R, FAST, SHIFT>(c, a, blocks, chunk, k);
+        if (rcl) return rcl;
+        k += chunk;
         HIPCHK(c, hipMemcpyAsync(lc.data(), s.lastchg, nsrc * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        if (c->timing)
-            for (int j = 0; j < chunk; ++j) {
-                float ms = 0.0f;
-                HIPCHK(c, hipEventElapsedTime(&ms, c->kev[2 * j], c->kev[2 * j + 1]));
-                c->stats.kernel_ms += ms;
-            }
         int mx = 0;
         for (int q = 0; q < nsrc; ++q) mx = std::max(mx, lc[q]);
         if (c->trace) {
@@ -1723,7 +1822,65 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
 
 // Relax the prepared slots (fields, fixed bits, offsets and flows are set)
 // and record the statistics.
+static int solve_run_once(fmm3d_ctx* c, int nsrc, int order, int precision);
+
+// Autotuner: the first solve of a kind (precision, order, walk mode, batch
+// size class) on a context times every candidate (patch shape x pass cap)
+// on the real batch, restarting from a copy of the initial fields, and keeps
+// the fastest for the rest of the context's life.
+static int autotune_column(fmm3d_ctx* c, int nsrc, int order, int precision)
+{
+    int cls = 0;
+    while ((1 << cls) < nsrc) ++cls;
+    const std::array<int, 8> key = { precision, order, c->walk_alt, cls, 0, 0, 0, 0 };
+    for (auto& e : c->tuned)
+        if (e[0] == key[0] && e[1] == key[1] && e[2] == key[2] && e[3] == key[3]) {
+            c->col_px = e[4]; c->col_pz = e[5]; c->col_passes = e[6];
+            return FMM3D_OK;
+        }
+    static const int cand[][3] = { { 8, 8, 16 }, { 8, 8, 8 }, { 8, 16, 16 }, { 16, 8, 16 },
+                                   { 16, 16, 16 }, { 16, 16, 8 }, { 8, 16, 8 }, { 16, 8, 8 } };
+    Slots& s = c->sl;
+    const size_t bytes = (size_t)nsrc * c->N * (precision == FMM3D_PREC_FP64 ? 8 : 4);
+    void* save = nullptr;
+    HIPCHK(c, dalloc(&save, bytes));
+    HIPCHK(c, hipMemcpyAsync(save, s.T, bytes, hipMemcpyDeviceToDevice, c->stream));
+    double best = 1e300;
+    int bi = 0, rc = FMM3D_OK;
+    for (int i = 0; i < (int)(sizeof cand / sizeof cand[0]); ++i) {
+        c->col_px = cand[i][0]; c->col_pz = cand[i][1]; c->col_passes = cand[i][2];
+        hipError_t e = hipMemcpyAsync(s.T, save, bytes, hipMemcpyDeviceToDevice, c->stream);
+        if (e != hipSuccess) { rc = fail(c, FMM3D_E_DEVICE, "autotune: %s", hipGetErrorString(e)); break; }
+        rc = solve_run_once(c, nsrc, order, precision);
+        if (rc) break;
+        if (c->stats.solve_ms < best) { best = c->stats.solve_ms; bi = i; }
+    }
+    if (rc == FMM3D_OK) {
+        hipError_t e = hipMemcpyAsync(s.T, save, bytes, hipMemcpyDeviceToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(c, FMM3D_E_DEVICE, "autotune: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(save);
+    if (rc) return rc;
+    c->col_px = cand[bi][0]; c->col_pz = cand[bi][1]; c->col_passes = cand[bi][2];
+    c->tuned.push_back({ key[0], key[1], key[2], key[3], c->col_px, c->col_pz, c->col_passes, 0 });
+    return FMM3D_OK;
+}
+
 static int solve_run(fmm3d_ctx* c, int nsrc, int order, int precision)
+{
+    if (c->autotune && (c->strategy == 1 || c->walk_alt)) {
+        int rc = autotune_column(c, nsrc, order, precision);
+        if (rc) return rc;
+    }
+    int rc = solve_run_once(c, nsrc, order, precision);
+    if (rc == FMM3D_OK && (c->strategy == 1 || c->walk_alt)) {
+        c->stats.tile_r = c->col_passes; c->stats.tile_x = c->col_px; c->stats.tile_z = c->col_pz;
+    }
+    return rc;
+}
+
+static int solve_run_once(fmm3d_ctx* c, int nsrc, int order, int precision)
 {
     Slots& s = c->sl;
     k_fill_real<double><<<(3 * nsrc + 255) / 256, 256, 0, c->stream>>>(s.srcpos, (size_t)3 * nsrc, __builtin_nan(""));
@@ -1765,9 +1922,11 @@ static int solve_run(fmm3d_ctx* c, int nsrc, int order, int precision)
 static void lateral_flow(const Grid& g, const double* face, int out[2])
 {
     const int nx = g.nnx, nz = g.nnz;
+    // only the signs are used: a sub-lattice of at most 64 x 64 nodes fits the plane
+    const int sx_ = std::max(1, nx / 64), sz_ = std::max(1, nz / 64);
     double n = 0, sx = 0, sz = 0, sxx = 0, szz = 0, sxz = 0, st = 0, sxt = 0, szt = 0;
-    for (int ix = 0; ix < nx; ++ix)
-        for (int iz = 0; iz < nz; ++iz) {
+    for (int ix = 0; ix < nx; ix += sx_)
+        for (int iz = 0; iz < nz; iz += sz_) {
             const double v = face[(size_t)ix * nz + iz];
             if (!(v > 0.0)) continue;
             n += 1; sx += ix; sz += iz; sxx += (double)ix * ix; szz += (double)iz * iz; sxz += (double)ix * iz;
@@ -1798,12 +1957,16 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     // boundary minimum per source: the float32 offset (and a sanity check)
     std::vector<double> toff(nsrc, 0.0);
     for (int s = 0; s < nsrc; ++s) {
+        // branch-free scan (vectorisable): smallest positive value, and any NaN
+        const double* f = faces + (size_t)s * F;
         double m = __builtin_huge_val();
+        int nan = 0;
         for (size_t i = 0; i < F; ++i) {
-            double v = faces[(size_t)s * F + i];
-            if (isnan(v)) return fail(c, FMM3D_E_ARG, "fmm3d_solve_tele: NaN boundary time (source %d)", s);
-            if (v > 0.0 && v < m) m = v;
+            const double v = f[i];
+            nan |= (v != v);
+            m = (v > 0.0 && v < m) ? v : m;
         }
+        if (nan) return fail(c, FMM3D_E_ARG, "fmm3d_solve_tele: NaN boundary time (source %d)", s);
         toff[s] = isfinite(m) ? m : 0.0;
     }
     int rc = ensure_slots(c, nsrc, precision);
@@ -1811,8 +1974,14 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     Slots& s = c->sl;
     s.n = 0;
     c->walk_alt = 0;
-    double* dfaces = nullptr;
-    HIPCHK(c, dalloc(&dfaces, (size_t)nsrc * F * sizeof(double)));
+    if (c->d_faces_n < (size_t)nsrc * F) {
+        (void)hipFree(c->d_faces);
+        c->d_faces = nullptr;
+        c->d_faces_n = 0;
+        HIPCHK(c, dalloc(&c->d_faces, (size_t)nsrc * F * sizeof(double)));
+        c->d_faces_n = (size_t)nsrc * F;
+    }
+    double* dfaces = c->d_faces;
     HIPCHK(c, hipMemcpyAsync(dfaces, faces, (size_t)nsrc * F * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(s.toff, toff.data(), nsrc * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (precision == FMM3D_PREC_FP64)
@@ -1826,10 +1995,8 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
         HIPCHK(c, hipMemcpyAsync(s.flow, flow.data(), flow.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     }
     hipError_t le = hipGetLastError();
-    if (le != hipSuccess) { (void)hipFree(dfaces); return fail(c, FMM3D_E_DEVICE, "init: %s", hipGetErrorString(le)); }
-    int rc2 = solve_run(c, nsrc, order, precision);
-    (void)hipFree(dfaces);
-    return rc2;
+    if (le != hipSuccess) return fail(c, FMM3D_E_DEVICE, "init: %s", hipGetErrorString(le));
+    return solve_run(c, nsrc, order, precision);
 }
 
 // Lateral flow signs for a general fixed set: the (theta, phi) slope of a
@@ -2348,13 +2515,9 @@ static int engine_sweeps(fmm3d_ctx* c, int nsweeps)
     int blocks = 0;
     int rc = column_args<R>(c, c->sl.cap, c->eng_order, a, blocks);
     if (rc) return rc;
-    for (int j = 0; j < nsweeps; ++j) {
-        a.iter = c->eng_k++;
-        hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks);
-        if (le == hipErrorInvalidValue) return fail(c, FMM3D_E_ARG, "engine: unsupported column patch");
-        HIPCHK(c, le);
-        c->stats.kernel_launches++;
-    }
+    rc = launch_chunk<R, FAST, SHIFT>(c, a, blocks, nsweeps, c->eng_k);
+    if (rc) return rc;
+    c->eng_k += nsweeps;
     HIPCHK(c, hipMemcpyAsync(c->eng_lc, c->sl.lastchg, c->sl.cap * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     return FMM3D_OK;
 }
@@ -2480,6 +2643,13 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "spin_limit")) {
         if (iv < 0 || iv > 10000000) return fail(c, FMM3D_E_ARG, "spin_limit out of range");
         c->spin_limit = iv;
+    } else if (!strcmp(key, "autotune")) {
+        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "autotune must be 0 or 1");
+        c->autotune = iv;
+        if (!iv) c->tuned.clear();
+    } else if (!strcmp(key, "graph")) {
+        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "graph must be 0 or 1");
+        c->use_graph = iv;
     } else if (!strcmp(key, "max_sweeps")) {
         if (iv < 0) return fail(c, FMM3D_E_ARG, "max_sweeps must be >= 0");
         c->max_sweeps = iv;

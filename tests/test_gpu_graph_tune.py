@@ -1,0 +1,48 @@
+"""Sweep chunks replayed from a captured hipGraph give the fields the direct
+launches give, solve after solve with other work in between; the autotuner
+picks one of its candidates and its solve matches the untuned one."""
+import numpy as np
+import pytest
+
+import goldens as G
+from fmm3d import lib, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_graph_replays_match_direct_launches():
+    gd = G.load("tele_41x45x39_mixed")
+    with lib.Context(gd.g) as d, lib.Context(gd.g) as c:
+        d.set_model(gd.velv)
+        c.set_model(gd.velv)
+        d.set_option("graph", 0)
+        c.set_option("graph", 1)
+        for rep in range(3):
+            for prec, tol in ((lib.PREC_FP64, 1e-7), (lib.PREC_FP32, 1e-3)):
+                faces = gd.faces[rep:rep + 4]
+                d.solve_tele(faces, gd.fom, prec)
+                c.solve_tele(faces, gd.fom, prec)
+                for s in range(faces.shape[0]):
+                    assert float(np.abs(c.field(s) - d.field(s)).max()) <= tol, (rep, prec, s)
+                # eager work between replays
+                c.receivers(gd.rcv)
+
+
+def test_autotuner_choice_and_result():
+    g, velv, srcs, rcv = synth.config_201(nsrc=8, nrc=50, seed=2)
+    g2, velv2, _, _ = synth.tele_case((61, 61, 61), 400.0, (10.0, 10.0), (12, 12, 12), 0.05, 1, 20, 3)
+    faces = np.stack([synth.tele_bottom_face(g2, s) for s in synth.tele_sources(g2, 6, 3)])
+    with lib.Context(g2) as c, lib.Context(g2) as d:
+        c.set_model(velv2)
+        d.set_model(velv2)
+        c.set_option("autotune", 1)
+        c.solve_tele(faces, lib.ORDER_MIXED, lib.PREC_FP64)
+        st = c.stats()
+        assert (st["tile_x"], st["tile_z"], st["tile_r"]) in {(8, 8, 16), (8, 8, 8), (8, 16, 16), (16, 8, 16),
+                                                             (16, 16, 16), (16, 16, 8), (8, 16, 8), (16, 8, 8)}
+        d.solve_tele(faces, lib.ORDER_MIXED, lib.PREC_FP64)
+        for s in range(faces.shape[0]):
+            assert float(np.abs(c.field(s) - d.field(s)).max()) <= 1e-7
+        # the cached choice is reused without re-tuning
+        c.solve_tele(faces, lib.ORDER_MIXED, lib.PREC_FP64)
+        assert c.stats()["tile_x"] == st["tile_x"] and c.stats()["tile_r"] == st["tile_r"]
