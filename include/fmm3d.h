@@ -154,7 +154,13 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *   "tol_fp64", "tol_fp32"  relative change below which a sweep counts as
  *                   settled
  *   "direction"     0: sweep directions alternate; 1: per brick, along time
- *   "strategy"      0: brick sweeps in three phases; 1: column walk (default)
+ *   "strategy"      0: brick sweeps in three phases; 1: column walk (default);
+ *                   2: hyperplane walk (8 x 8 patches, lane (i, j) relaxes
+ *                   layer s - i - j at step s: one Gauss-Seidel sweep in flow
+ *                   order per launch, one update per node)
+ *   "plane_chunk"   hyperplane walk: steps per staging chunk (2, 4 or 8; default 4)
+ *   "plane_sweeps"  hyperplane walk: sweeps before the column walk takes over
+ *                   (0, the default: hyperplane sweeps throughout)
  *   "column_x", "column_z"  column-walk patch (8, 16, 32 or 64 each; supported
  *                   pairs 8x8, 8x16, 16x8, 16x16, 8x32, 32x8, 16x32, 32x16,
  *                   32x32, 16x64, 64x16, 8x64, 64x8)

@@ -1015,6 +1015,46 @@ long or_relax1_fixed(const or_grid *g, const double *veln, const unsigned char *
     return sweeps;
 }
 
+/* Gauss-Seidel sweeps of the value-based mixed-order update (relax_node, the
+ * node's own value orders its neighbours, unsupported values dropped) in
+ * flow order: walk layers bottom to top (top to bottom when `down`), inside
+ * a layer theta in the direction of fx (+1 ascending, -1 descending), then
+ * phi in the direction of fz; every free node is updated once per sweep.
+ * Starts from ttn as given (fixed nodes keep their values).  With maxsweeps
+ * > 0 it stops after that many sweeps.  Returns the sweep that changed
+ * nothing (converged), or -(sweeps run) when it stopped before that. */
+long or_gs_flow(const or_grid *g, const double *veln, const unsigned char *fixed, int fom,
+                int fx, int fz, int down, int maxsweeps, double *ttn)
+{
+    const int nr = g->nnr, nx = g->nnx, nz = g->nnz;
+    double *sth = malloc((nx + 2) * sizeof(double));
+    double *ri = malloc((nr + 2) * sizeof(double));
+    metric_tables(g, ri, sth);
+    long res = -1;
+    const int lim = maxsweeps > 0 ? maxsweeps : 100000;
+    int it;
+    for (it = 1; it <= lim; ++it) {
+        int moved = 0;
+        for (int q = 0; q < nr; ++q) {
+            const int ir = down ? 1 + q : nr - q;
+            for (int a = 0; a < nx; ++a) {
+                const int ix = fx > 0 ? 1 + a : nx - a;
+                for (int b = 0; b < nz; ++b) {
+                    const int iz = fz > 0 ? 1 + b : nz - b;
+                    size_t k = ((size_t)(ir - 1) * nx + (ix - 1)) * nz + (iz - 1);
+                    if (fixed[k]) continue;
+                    double nv = relax_node(g, veln, ri, sth, fom, ttn, ttn, ir, ix, iz);
+                    if (nv != ttn[k]) { ttn[k] = nv; moved = 1; }
+                }
+            }
+        }
+        if (!moved) { res = it; break; }
+    }
+    if (res < 0) res = -(long)(it - 1);
+    free(sth); free(ri);
+    return res;
+}
+
 /* Cartesian position of (height km, colat, lon) */
 static void cart(const or_grid *g, double h, double th, double ph, double x[3])
 {

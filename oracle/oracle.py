@@ -63,6 +63,8 @@ def lib():
         L.or_fmm_fixed.restype = C.c_long
         L.or_relax1_fixed.argtypes = [C.POINTER(OrGrid), dp, up, dp, C.c_int, C.c_int, dp]
         L.or_relax1_fixed.restype = C.c_long
+        L.or_gs_flow.argtypes = [C.POINTER(OrGrid), dp, up, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp]
+        L.or_gs_flow.restype = C.c_long
         ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
         L.or_point_region.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, up, dp]
         L.or_point_region.restype = C.c_long
@@ -128,6 +130,20 @@ def relax1_fixed(g, veln, fixed, tfix, fom=1, alternate=True):
                               np.ascontiguousarray(tfix, dtype=np.float64), fom, int(alternate), ttn)
     if n < 0:
         raise RuntimeError("or_relax1_fixed: no convergence")
+    return ttn, n
+
+
+def gs_flow(g, veln, fixed, tfix, fom=1, fx=1, fz=1, down=False, sweeps=0, start=None):
+    """Gauss-Seidel sweeps in flow order (walk layer, theta along fx, phi
+    along fz) from the fixed set (or from the field `start`); `sweeps` > 0
+    stops after that many.  Returns (field, n): n = the sweep that changed
+    nothing, or -(sweeps run)."""
+    if start is None:
+        ttn = np.where(np.asarray(fixed, bool), tfix, np.inf).astype(np.float64)
+    else:
+        ttn = np.array(start, dtype=np.float64, order="C")
+    n = lib().or_gs_flow(C.byref(or_grid(g)), veln, np.ascontiguousarray(fixed, dtype=np.uint8),
+                         fom, int(fx), int(fz), int(down), int(sweeps), ttn)
     return ttn, n
 
 

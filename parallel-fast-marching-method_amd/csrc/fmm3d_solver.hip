@@ -384,7 +384,8 @@ template <typename R> struct ColArgs {
     unsigned long long* visits;
     unsigned long long* updates;
     unsigned long long* nodes;   // free nodes staged
-    int* prog;              // per slot per patch: iter*(nnr+1) + layers finished, or null (no ordering)
+    int* prog;              // per slot per patch: iter*pstride + layers (steps) finished, or null (no ordering)
+    int pstride;            // progress counts per sweep (both column walks: nnr + 16)
     const int* flow;        // per slot: lateral flow signs (fx, fz), or null
     int spin_limit;         // polls before a wait on a neighbour is given up
     int down;               // 1: walk top to bottom (when walkalt is null)
@@ -503,7 +504,7 @@ k_column(ColArgs<R> a)
     if (!a.slot_on[slot]) return;
     // sweep number: an argument, or a device counter (graph replays)
     const int iter = a.iterp ? __hip_atomic_load(a.iterp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : a.iter;
-    const int fx = a.flow ? a.flow[2 * slot] : 1, fz = a.flow ? a.flow[2 * slot + 1] : 1;
+    const int fx = a.flow && a.flow[2 * slot] < 0 ? -1 : 1, fz = a.flow && a.flow[2 * slot + 1] < 0 ? -1 : 1;
     const int rx = row - slot * a.npx, rz = k % a.npz;
     const int bx = fx > 0 ? rx : a.npx - 1 - rx;
     const int bz = fz > 0 ? rz : a.npz - 1 - rz;
@@ -512,7 +513,7 @@ k_column(ColArgs<R> a)
     const bool alt = a.walkalt && a.walkalt[slot];
     const bool down = a.walkalt ? (alt && !(iter & 1)) : (a.down != 0);
     int* pg = (a.prog && !alt) ? a.prog + (size_t)slot * np : nullptr;
-    const int pbase = iter * (a.nnr + 1);
+    const int pbase = iter * a.pstride;
     const int t = threadIdx.x;
     if (t == 0) {
         int m = st[b];
@@ -674,7 +675,7 @@ k_column(ColArgs<R> a)
         __syncthreads();
         if (pg && t == 0) {
             __builtin_amdgcn_s_waitcnt(0);
-            __hip_atomic_store(pg + b, pbase + (a.nnr - ir), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(pg + b, pbase + q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     {
@@ -682,6 +683,294 @@ k_column(ColArgs<R> a)
         for (int off = 32; off > 0; off >>= 1) { w += __shfl_down(w, off, 64); wv += __shfl_down(wv, off, 64); }
         if (t == 0 && w) atomicAdd(a.updates, w);
         if (t == 0 && wv) atomicAdd(a.nodes, wv);
+    }
+    const bool blk = __syncthreads_or(anyv);
+    if (t == 0 && blk) {
+        st[b] = iter;
+        atomicMax(a.lastchg + slot, iter);
+        atomicOr(a.valchg + slot, 1);
+    }
+}
+
+// Lateral walk direction of one axis in a sweep: flow entries are +-1 (that
+// direction every sweep) or +-2 (a weak lateral gradient: that direction on
+// odd sweeps, the other one on even sweeps).
+__device__ __forceinline__ int flow_dir(int f, int iter)
+{
+    const int s = f > 0 ? 1 : -1;
+    return (f == 2 || f == -2) && !(iter & 1) ? -s : s;
+}
+
+// Hyperplane column walk.  A one-wavefront block owns an 8 x 8 patch of
+// every layer; lane (i, j) holds the column at flow coordinates (i, j) --
+// i counts theta rows along the slot's theta flow, j phi columns along its
+// phi flow -- and at step s relaxes walk layer w = s - i - j.  The 64 nodes
+// of a step are independent, and every node finds its walk-previous,
+// theta-upstream and phi-upstream neighbours (one and two nodes away)
+// already relaxed in this sweep and the other three from the previous one:
+// a sweep is one Gauss-Seidel pass over the slot in (walk layer, theta flow,
+// phi flow) order that updates each free node once.  Its fixed point is the
+// column walk's (and the heap march's).
+//
+// Each lane keeps its column's window in registers: the two layers it
+// relaxed last (P1, P2), the current one as the previous sweep left it (C0)
+// and the next two (N1, N2).  On the skewed schedule the lateral neighbours
+// of (i, j, w) sit in the neighbouring lanes' windows -- lane (i-1, j) is one
+// layer ahead, so its P1 is (i-1, j, w) relaxed; lane (i+1, j) is one layer
+// behind, so its N1 is (i+1, j, w) unrelaxed; two lanes away P2 and N2 -- and
+// come over by cross-lane permutes.  Only the patch-boundary halo goes
+// through LDS: a ring of 16 steps x 64 cells, cell (combo, position) of a
+// step being the halo value some boundary lane reads at that step.  Staging
+// goes by chunks of CH steps: the own-column values (the layer three ahead,
+// slowness, fixed bits, spacings) are loaded into registers at the start of
+// a chunk, and the chunk's halo cells once the upstream patches have
+// published the steps that produce them.  Relaxed values collect in an LDS
+// ring of 16 layers and go to HBM a layer at a time, coalesced, at the step
+// that completes the layer (w + 14), the changed ones only; after each
+// chunk the patch publishes the steps it has completed, so a published
+// count c means layers below c - 14 are in HBM (the upstream's count s + 15
+// feeds this patch's step s).
+template <typename R, int CH, bool FAST, bool SHIFT>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 5)))
+k_plane(ColArgs<R> a)
+{
+    constexpr int P = 8, HR = 16;
+    static_assert(CH >= 1 && CH + 1 <= HR, "the halo ring keeps the previous step's cells");
+    constexpr int WR = 16, LAG = 2 * (P - 1);    // layer ring; steps from a layer's first node to its last
+    __shared__ R hbuf[HR * 64];
+    __shared__ R wbuf[WR * 64];
+    __shared__ unsigned char wchg[WR * 64];
+    __shared__ int s_act;
+    const int np = a.npx * a.npz;
+    // Block -> patch by anti-diagonals of the flow order across all slots:
+    // diagonal d (patches with rx + rz = d, rx and rz counted along the
+    // flow) of every slot, then diagonal d + 1, each padded to W patches.
+    // A patch's upstream neighbours lie on the previous diagonal, so the
+    // blocks resident together are mostly a few diagonals apart and ready
+    // to run, rather than one slot's whole chain waiting on its head.
+    const int W = min(a.npx, a.npz), per = a.nslots * W;
+    const int dg = blockIdx.x / per, rem = blockIdx.x - dg * per;
+    const int slot = rem / W;
+    const int rx = max(0, dg - (a.npz - 1)) + (rem - slot * W), rz = dg - rx;
+    if (dg >= a.npx + a.npz - 1 || rx >= a.npx || rz < 0) return;
+    if (!a.slot_on[slot]) return;
+    const int iter = a.iterp ? __hip_atomic_load(a.iterp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : a.iter;
+    const bool alt = a.walkalt && a.walkalt[slot];
+    // an alternating slot (waves in every direction) cycles through the
+    // eight walk orders: down/up every sweep, theta every two, phi every four
+    const int fx = alt ? (((iter >> 1) & 1) ? -1 : 1) : (a.flow ? flow_dir(a.flow[2 * slot], iter) : 1);
+    const int fz = alt ? (((iter >> 2) & 1) ? -1 : 1) : (a.flow ? flow_dir(a.flow[2 * slot + 1], iter) : 1);
+    const int bx = fx > 0 ? rx : a.npx - 1 - rx;
+    const int bz = fz > 0 ? rz : a.npz - 1 - rz;
+    const int b = bx * a.npz + bz;
+    int* st = a.stamp + (size_t)slot * np;
+    const bool down = a.walkalt ? (alt && !(iter & 1)) : (a.down != 0);
+    int* pg = (a.prog && !alt) ? a.prog + (size_t)slot * np : nullptr;
+    const int nr = a.nnr, nsteps = nr + 2 * (P - 1);
+    const int pbase = iter * a.pstride;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        int m = st[b];
+        if (bx > 0) m = max(m, st[b - a.npz]);
+        if (bx < a.npx - 1) m = max(m, st[b + a.npz]);
+        if (bz > 0) m = max(m, st[b - 1]);
+        if (bz < a.npz - 1) m = max(m, st[b + 1]);
+        s_act = (m >= iter - 1);
+        if (!s_act && pg) __hip_atomic_store(pg + b, pbase + nsteps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (!s_act) return;
+    if (t == 0) atomicAdd(a.visits, 1ull);
+    const int ux = (bx - fx >= 0 && bx - fx < a.npx) ? b - fx * a.npz : -1;
+    const int uz = (bz - fz >= 0 && bz - fz < a.npz) ? b - fz : -1;
+    bool wx = pg && ux >= 0, wz = pg && uz >= 0;
+    const size_t NL = (size_t)a.nnx * a.nnz;
+    const size_t N = NL * nr;
+    R* T = a.T + (size_t)slot * N;
+    const uint32_t* fb = a.fixbits + (size_t)slot * ((N + 31) / 32);
+    const R inf = Lim<R>::inf();
+    const int x0 = bx * P, z0 = bz * P;
+    // grid theta / phi index of flow coordinate i / j (halo: -2..P+1)
+    auto gxo = [&](int i) { return x0 + (fx > 0 ? i : P - 1 - i); };
+    auto gzo = [&](int j) { return z0 + (fz > 0 ? j : P - 1 - j); };
+    auto lay = [&](int w) { return (size_t)(down ? w : nr - 1 - w) * NL; };
+    const int li = t >> 3, lj = t & 7;
+    const int gx = gxo(li), gz = gzo(lj);
+    const bool mine = gx < a.nnx && gz < a.nnz;
+    const size_t col = mine ? (size_t)gx * a.nnz + gz : 0;
+    const int hzrow = min(gx, a.nnx - 1);
+    // Per-lane flags, packed in one register and rebuilt at each use (as
+    // lane masks they would hold sixteen scalar register pairs through the
+    // step loop): bits 0-7 which lateral stencil sides exist (flow-upstream
+    // and downstream theta and phi, one and two nodes away), bits 8-15 which
+    // lateral neighbours come from the halo ring, bit 16 the column is in
+    // the grid.
+    auto inr = [](int v, int n) { return v >= 0 && v < n; };
+    const unsigned lane_flags =
+        (inr(gx - fx, a.nnx) ? 1u : 0u) | (inr(gx - 2 * fx, a.nnx) ? 2u : 0u) |
+        (inr(gx + fx, a.nnx) ? 4u : 0u) | (inr(gx + 2 * fx, a.nnx) ? 8u : 0u) |
+        (inr(gz - fz, a.nnz) ? 16u : 0u) | (inr(gz - 2 * fz, a.nnz) ? 32u : 0u) |
+        (inr(gz + fz, a.nnz) ? 64u : 0u) | (inr(gz + 2 * fz, a.nnz) ? 128u : 0u) |
+        (li == 0 ? 0x100u : 0u) | (li <= 1 ? 0x200u : 0u) | (li == P - 1 ? 0x400u : 0u) | (li >= P - 2 ? 0x800u : 0u) |
+        (lj == 0 ? 0x1000u : 0u) | (lj <= 1 ? 0x2000u : 0u) | (lj == P - 1 ? 0x4000u : 0u) | (lj >= P - 2 ? 0x8000u : 0u) |
+        (gx < a.nnx && gz < a.nnz ? 0x10000u : 0u);
+    // the halo cell this lane stages, lane = combo * 8 + position: combos
+    // 0/1 theta-upstream depth 1/2, 2/3 phi-upstream, 4/5 theta-downstream,
+    // 6/7 phi-downstream.  The cell of layer w is read first at step
+    // w + position (upstream) or w + 6 + position (downstream), the ring
+    // index of the cell.
+    const int hc = t >> 3, hp = t & 7, hside = hc >> 1, hd = hc & 1;
+    const int hi = hside == 0 ? -1 - hd : (hside == 2 ? P + hd : hp);
+    const int hj = hside == 1 ? -1 - hd : (hside == 3 ? P + hd : hp);
+    const int hlag = hside < 2 ? hp : P - 2 + hp;
+    const int hgx = gxo(hi), hgz = gzo(hj);
+    const bool hin = hgx >= 0 && hgx < a.nnx && hgz >= 0 && hgz < a.nnz;
+    const size_t hcol = hin ? (size_t)hgx * a.nnz + hgz : 0;
+    // where this lane's boundary neighbours come from: ring cell and step
+    // delay (0: this step's cells, 1: the previous step's)
+    const int oxm1 = 0 * 8 + lj, oxm2 = (li == 0 ? 1 : 0) * 8 + lj, dxm2 = li == 0 ? 0 : 1;
+    const int oxp1 = 4 * 8 + lj, oxp2 = (li == P - 1 ? 5 : 4) * 8 + lj, dxp2 = li == P - 1 ? 1 : 0;
+    const int ozm1 = 2 * 8 + li, ozm2 = (lj == 0 ? 3 : 2) * 8 + li, dzm2 = lj == 0 ? 0 : 1;
+    const int ozp1 = 6 * 8 + li, ozp2 = (lj == P - 1 ? 7 : 6) * 8 + li, dzp2 = lj == P - 1 ? 1 : 0;
+    const R hr = a.hr, ihr = R(1) / a.hr, tolr = (R)a.tol;
+    // own-column window (unknown layers outside the grid read as +inf)
+    auto old_at = [&](bool in, int w) -> R { return (in && w >= 0 && w < nr) ? T[lay(w) + col] : inf; };
+    R P1 = inf, P2 = inf;
+    R C0 = old_at(mine, -li - lj), N1 = old_at(mine, 1 - li - lj), N2 = old_at(mine, 2 - li - lj);
+    bool anyv = false;
+    unsigned nupd = 0;
+    const int nchunks = (nsteps + CH - 1) / CH;
+#pragma unroll 1
+    for (int kc = 0; kc < nchunks; ++kc) {
+        const int s0 = kc * CH, s1 = min(s0 + CH, nsteps);
+        // own-column values of the chunk's steps: the layer three ahead of the
+        // one relaxed, and slowness, fixed-bit word and spacings of that one
+        R qn[CH], qs[CH], qhx[CH], qhz[CH];
+        uint32_t qf[CH];
+        unsigned lfc = lane_flags;
+        asm volatile("" : "+v"(lfc));
+        const bool minec = lfc & 0x10000u;
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int w = s0 + q - li - lj;
+            qn[q] = old_at(minec, w + 3);
+            const bool ok = minec && w >= 0 && w < nr;
+            const int ir = ok ? (down ? w : nr - 1 - w) : 0;
+            const size_t c = ok ? lay(w) + col : 0;
+            qs[q] = ok ? a.slo[c] : R(-1);
+            qf[q] = ok ? fb[c >> 5] : 0u;
+            qhx[q] = a.hx[ir];
+            qhz[q] = a.hz[(size_t)ir * a.nnx + hzrow];
+        }
+        // wait (bounded) until the upstream patches have stored the layers of
+        // this chunk's upstream halo (the last read is layer s1 - 1)
+        if (wx || wz) {
+            if (t == 0) {
+                const int need = pbase + min(s1 + LAG, nsteps);
+                int spins = 0;
+                while (wx && __hip_atomic_load(pg + ux, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    if (++spins > a.spin_limit) { wx = false; break; }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                spins = 0;
+                while (wz && __hip_atomic_load(pg + uz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                    if (++spins > a.spin_limit) { wz = false; break; }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+                s_act = (wx ? 1 : 0) | (wz ? 2 : 0);
+            }
+            __syncthreads();
+            wx = s_act & 1; wz = s_act & 2;
+        }
+        // halo cells read first in this chunk (device-coherent loads: the
+        // neighbouring patches may run on another XCD)
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int w = s0 + q - hlag;
+            const R v = (hin && w >= 0 && w < nr)
+                ? __hip_atomic_load(T + lay(w) + hcol, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : inf;
+            hbuf[((s0 + q) & (HR - 1)) * 64 + t] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int s = s0 + q;
+            if (s >= s1) break;
+            const int w = s - li - lj;
+            unsigned lf = lane_flags;
+            asm volatile("" : "+v"(lf));
+            const bool bxm1 = lf & 0x100u, bxm2 = lf & 0x200u, bxp1 = lf & 0x400u, bxp2 = lf & 0x800u;
+            const bool bzm1 = lf & 0x1000u, bzm2 = lf & 0x2000u, bzp1 = lf & 0x4000u, bzp2 = lf & 0x8000u;
+            // lateral neighbours from the neighbouring lanes' windows ...
+            R xm1 = __shfl(P1, (t - 8) & 63, 64), xm2 = __shfl(P2, (t - 16) & 63, 64);
+            R xp1 = __shfl(N1, (t + 8) & 63, 64), xp2 = __shfl(N2, (t + 16) & 63, 64);
+            R zm1 = __shfl(P1, (t - 1) & 63, 64), zm2 = __shfl(P2, (t - 2) & 63, 64);
+            R zp1 = __shfl(N1, (t + 1) & 63, 64), zp2 = __shfl(N2, (t + 2) & 63, 64);
+            // ... or, across the patch boundary, from the halo ring (read by
+            // every lane, kept by the boundary ones: no branches)
+            const int r0 = (s & (HR - 1)) * 64, r1 = ((s - 1) & (HR - 1)) * 64;
+            {
+                const R h_xm1 = hbuf[r0 + oxm1], h_xm2 = hbuf[(dxm2 ? r1 : r0) + oxm2];
+                const R h_xp1 = hbuf[r1 + oxp1], h_xp2 = hbuf[(dxp2 ? r1 : r0) + oxp2];
+                const R h_zm1 = hbuf[r0 + ozm1], h_zm2 = hbuf[(dzm2 ? r1 : r0) + ozm2];
+                const R h_zp1 = hbuf[r1 + ozp1], h_zp2 = hbuf[(dzp2 ? r1 : r0) + ozp2];
+                xm1 = bxm1 ? h_xm1 : xm1; xm2 = bxm2 ? h_xm2 : xm2;
+                xp1 = bxp1 ? h_xp1 : xp1; xp2 = bxp2 ? h_xp2 : xp2;
+                zm1 = bzm1 ? h_zm1 : zm1; zm2 = bzm2 ? h_zm2 : zm2;
+                zp1 = bzp1 ? h_zp1 : zp1; zp2 = bzp2 ? h_zp2 : zp2;
+            }
+            R cur = C0;
+            bool chg = false;
+            if (w >= 0 && w < nr && (lf & 0x10000u)) {
+                const size_t c = lay(w) + col;
+                const bool fixed = (qf[q] >> (c & 31)) & 1u;
+                if (!fixed) {
+                    const int ir = down ? w : nr - 1 - w;
+                    R v1[6], v2[6];
+                    bool in1[6], in2[6];
+                    // side 0 is layer ir-1, side 1 layer ir+1; the walk reached ir from ir+1 (up) or ir-1 (down)
+                    v1[0] = down ? P1 : N1; v2[0] = down ? P2 : N2;
+                    v1[1] = down ? N1 : P1; v2[1] = down ? N2 : P2;
+                    v1[2] = xm1; v2[2] = xm2; v1[3] = xp1; v2[3] = xp2;
+                    v1[4] = zm1; v2[4] = zm2; v1[5] = zp1; v2[5] = zp2;
+                    in1[0] = ir >= 1; in2[0] = ir >= 2;
+                    in1[1] = ir + 1 < nr; in2[1] = ir + 2 < nr;
+                    in1[2] = lf & 1u; in2[2] = lf & 2u; in1[3] = lf & 4u; in2[3] = lf & 8u;
+                    in1[4] = lf & 16u; in2[4] = lf & 32u; in1[5] = lf & 64u; in2[5] = lf & 128u;
+                    const R h[3] = { hr, qhx[q], qhz[q] };
+                    const R ih[3] = { ihr, fast_rcp(qhx[q]), fast_rcp(qhz[q]) };
+                    const R nv = col_update<R, FAST, SHIFT>(a.fom, C0, v1, v2, in1, in2, h, ih, qs[q]);
+                    ++nupd;
+                    if (nv != C0) {
+                        cur = nv;
+                        chg = true;
+                        const bool fin_n = nv < inf, fin_t = C0 < inf;
+                        if (fin_n != fin_t || !(fabs(nv - C0) <= tolr * fmax(fabs(nv), fabs(C0))))
+                            anyv = true;
+                    }
+                }
+            }
+            if (w >= 0 && w < nr) {
+                wbuf[(w & (WR - 1)) * 64 + t] = cur;
+                wchg[(w & (WR - 1)) * 64 + t] = chg ? 1 : 0;
+            }
+            // advance the window one layer
+            P2 = P1; P1 = cur; C0 = N1; N1 = N2; N2 = qn[q];
+            // layer s - 14 is complete (lane (7, 7) relaxed it in this step): store its changes
+            const int L = s - LAG;
+            if (L >= 0 && L < nr && (lf & 0x10000u) && wchg[(L & (WR - 1)) * 64 + t])
+                __hip_atomic_store(T + lay(L) + col, wbuf[(L & (WR - 1)) * 64 + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // publish the completed steps once their stores have landed
+        if (pg) {
+            __builtin_amdgcn_s_waitcnt(0);
+            if (t == 0) __hip_atomic_store(pg + b, pbase + s1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    {
+        unsigned long long w = nupd;
+        for (int off = 32; off > 0; off >>= 1) w += __shfl_down(w, off, 64);
+        if (t == 0 && w) { atomicAdd(a.updates, w); atomicAdd(a.nodes, w); }
     }
     const bool blk = __syncthreads_or(anyv);
     if (t == 0 && blk) {
@@ -1053,9 +1342,11 @@ struct fmm3d_ctx {
     int brick_sl = 8;              // layers per brick
     int kin = 2;                   // lateral relaxations per layer (at most; stops when settled)
     int dirmode = 0;               // sweep direction policy
-    int strategy = 1;              // 0: brick sweeps in three phases; 1: column walk, one phase
+    int strategy = 1;              // 0: brick sweeps in three phases; 1: column walk; 2: hyperplane walk
     int col_px = 8, col_pz = 8;    // column-walk patch (theta x phi)
     int col_passes = 16;           // line passes per layer at most
+    int plane_ch = 4;              // hyperplane walk: steps per staging chunk (2, 4 or 8)
+    int plane_sweeps = 0;          // hyperplane walk: sweeps before the column walk takes over (0: all)
     int use_graph = 1;             // replay sweep chunks from a captured hipGraph
     int autotune = 0;              // time candidate patch shapes / pass caps at first use
     std::vector<std::array<int, 8>> tuned;   // key (prec, order, walk, batch class) -> (px, pz, passes)
@@ -1103,6 +1394,10 @@ struct fmm3d_ctx {
 };
 
 static std::string g_create_err;
+
+// Patch shape of the column walks (the hyperplane walk's is 8 x 8).
+static inline int patch_x(const fmm3d_ctx* c) { return c->strategy == 2 ? 8 : c->col_px; }
+static inline int patch_z(const fmm3d_ctx* c) { return c->strategy == 2 ? 8 : c->col_pz; }
 
 static int fail(fmm3d_ctx* c, int code, const char* fmt, ...)
 {
@@ -1252,9 +1547,14 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     if (gv) c->use_graph = atoi(gv) != 0;
     const char* tr = getenv("FMM3D_TRACE");
     if (tr) c->trace = atoi(tr);
-    const char* sg = getenv("FMM3D_STRATEGY");   // "brick" | "column[:PXxPZ]"
+    const char* sg = getenv("FMM3D_STRATEGY");   // "brick" | "column[:PXxPZ[:passes]]" | "plane[:chunk]"
     if (sg) {
         if (!strncmp(sg, "brick", 5)) c->strategy = 0;
+        else if (!strncmp(sg, "plane", 5)) {
+            c->strategy = 2;
+            int ch = 0;
+            if (sscanf(sg, "plane:%d", &ch) == 1 && (ch == 2 || ch == 4 || ch == 8)) c->plane_ch = ch;
+        }
         else if (!strncmp(sg, "column", 6)) {
             c->strategy = 1;
             int a, b;
@@ -1616,9 +1916,34 @@ static int run_phases(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     return FMM3D_OK;
 }
 
-template <typename R, bool FAST, bool SHIFT>
-static hipError_t launch_column(fmm3d_ctx* c, const ColArgs<R>& a, int blocks)
+// Kernel of sweep k: 2 the hyperplane walk, 1 the column walk.  Under the
+// hyperplane strategy the first plane_sweeps sweeps of a solve are
+// hyperplane sweeps (one update per node, exact along the lateral flow) and
+// the column walk (lateral relaxation both ways in every layer) finishes
+// what a wave crossing the flow left; the serving engine, whose slots start
+// at different sweeps, keeps to one kernel.
+static int sweep_kernel(const fmm3d_ctx* c, int k, bool engine)
 {
+    if (c->strategy != 2) return 1;
+    if (c->plane_sweeps == 0) return 2;
+    return (!engine && k <= c->plane_sweeps) ? 2 : 1;
+}
+
+template <typename R, bool FAST, bool SHIFT>
+static hipError_t launch_column(fmm3d_ctx* c, const ColArgs<R>& a, int blocks, int kind)
+{
+    if (kind == 2) {
+        // diagonal-major grid: every anti-diagonal of every slot, padded
+        const int pb = (a.npx + a.npz - 1) * a.nslots * std::min(a.npx, a.npz);
+        if (c->plane_ch == 2) k_plane<R, 2, FAST, SHIFT><<<pb, 64, 0, c->stream>>>(a);
+        else if (c->plane_ch == 8) k_plane<R, 8, FAST, SHIFT><<<pb, 64, 0, c->stream>>>(a);
+        else k_plane<R, 4, FAST, SHIFT><<<pb, 64, 0, c->stream>>>(a);
+        return hipGetLastError();
+    }
+    if (c->strategy == 2) {   // the hyperplane strategy's patches are 8 x 8
+        k_column<R, 8, 8, FAST, SHIFT><<<blocks, 64, 0, c->stream>>>(a);
+        return hipGetLastError();
+    }
     const int px = c->col_px, pz = c->col_pz;
 #define FMM3D_COL_CASE(A, B)                                                   \
     if (px == A && pz == B) {                                                  \
@@ -1648,7 +1973,7 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
 {
     Slots& s = c->sl;
     const Grid& g = c->g;
-    const int npx = (g.nnx + c->col_px - 1) / c->col_px, npz = (g.nnz + c->col_pz - 1) / c->col_pz;
+    const int npx = (g.nnx + patch_x(c) - 1) / patch_x(c), npz = (g.nnz + patch_z(c) - 1) / patch_z(c);
     const int np = npx * npz;
     const size_t nbmax = (size_t)((g.nnr + 3) / 4) * ((g.nnx + 3) / 4) * ((g.nnz + 7) / 8);
     if ((size_t)np > nbmax)
@@ -1673,6 +1998,7 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     a.updates = c->d_visits + 1;
     a.nodes = c->d_visits + 2;
     a.prog = c->ordered ? s.prog : nullptr;
+    a.pstride = g.nnr + 16;
     a.walkalt = s.walkalt;
     a.flow = s.flow;
     a.spin_limit = c->spin_limit;
@@ -1697,16 +2023,17 @@ static bool column_supported(int px, int pz)
 // nodes only); other counts, and tracing, launch directly.  With timing on,
 // each launch (or the whole replay, shared out evenly) is timed with events.
 template <typename R, bool FAST, bool SHIFT>
-static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k)
+static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k, bool engine = false)
 {
-    if (!column_supported(c->col_px, c->col_pz))
+    if (c->strategy != 2 && !column_supported(c->col_px, c->col_pz))
         return fail(c, FMM3D_E_ARG, "solver: unsupported column patch %dx%d", c->col_px, c->col_pz);
-    if (!c->use_graph || n != FMM3D_CHECK || c->trace) {
+    const int kind = sweep_kernel(c, k, engine);
+    if (!c->use_graph || n != FMM3D_CHECK || c->trace || kind != sweep_kernel(c, k + n - 1, engine)) {
         a.iterp = nullptr;
         for (int j = 0; j < n; ++j) {
             a.iter = k + j;
             if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j], c->stream));
-            hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks);
+            hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks, sweep_kernel(c, k + j, engine));
             HIPCHK(c, le);
             if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j + 1], c->stream));
             c->stats.kernel_launches++;
@@ -1731,7 +2058,8 @@ static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k)
     a.iterp = c->d_iter;
     std::vector<char> sig(sizeof a + 2 * sizeof(int));
     memcpy(sig.data(), &a, sizeof a);
-    const int tag[2] = { blocks, (int)sizeof(R) * 100 + c->col_px * 10000 + c->col_pz };
+    const int tag[2] = { blocks, (int)sizeof(R) * 100 + kind * 10000000 + c->strategy * 1000000 +
+                                     c->plane_ch * 100000 + c->col_px * 10000 + c->col_pz };
     memcpy(sig.data() + sizeof a, tag, sizeof tag);
     if (!c->gexec || sig != c->g_sig) {
         if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
@@ -1739,7 +2067,7 @@ static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k)
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = hipSuccess;
         for (int j = 0; j < n && e == hipSuccess; ++j) {
-            e = launch_column<R, FAST, SHIFT>(c, a, blocks);
+            e = launch_column<R, FAST, SHIFT>(c, a, blocks, kind);
             if (e == hipSuccess) {
                 k_next_iter<<<1, 64, 0, c->stream>>>(c->d_iter);
                 e = hipGetLastError();
@@ -1869,12 +2197,14 @@ static int autotune_column(fmm3d_ctx* c, int nsrc, int order, int precision)
 
 static int solve_run(fmm3d_ctx* c, int nsrc, int order, int precision)
 {
-    if (c->autotune && (c->strategy == 1 || c->walk_alt)) {
+    if (c->autotune && (c->strategy == 1 || (c->strategy == 0 && c->walk_alt))) {
         int rc = autotune_column(c, nsrc, order, precision);
         if (rc) return rc;
     }
     int rc = solve_run_once(c, nsrc, order, precision);
-    if (rc == FMM3D_OK && (c->strategy == 1 || c->walk_alt)) {
+    if (rc == FMM3D_OK && c->strategy == 2) {
+        c->stats.tile_r = c->plane_ch; c->stats.tile_x = 8; c->stats.tile_z = 8;
+    } else if (rc == FMM3D_OK && (c->strategy == 1 || c->walk_alt)) {
         c->stats.tile_r = c->col_passes; c->stats.tile_x = c->col_px; c->stats.tile_z = c->col_pz;
     }
     return rc;
@@ -1891,7 +2221,7 @@ static int solve_run_once(fmm3d_ctx* c, int nsrc, int order, int precision)
     HIPCHK(c, hipMemsetAsync(c->d_visits, 0, 3 * sizeof(unsigned long long), c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     int sweeps = 0, rc;
-    if (c->strategy == 1 || c->walk_alt) {
+    if (c->strategy >= 1 || c->walk_alt) {
         if (precision == FMM3D_PREC_FP64) rc = run_column<double, false, false>(c, nsrc, order, &sweeps);
         else rc = run_column<float, true, true>(c, nsrc, order, &sweeps);
     } else {
@@ -1912,6 +2242,19 @@ static int solve_run_once(fmm3d_ctx* c, int nsrc, int order, int precision)
     c->stats.node_visits = (int64_t)vis[2];
     s.n = nsrc;
     return FMM3D_OK;
+}
+
+// Lateral flow code of a slot from its (theta, phi) index gradient: the sign
+// per axis; an axis whose gradient is under a quarter of the other's carries
+// +-2 (the hyperplane walk then reverses it on alternate sweeps: the wave
+// crosses it both ways in different parts of the model).
+static void flow_code(double gx, double gz, int out[2])
+{
+    out[0] = gx >= 0.0 ? 1 : -1;
+    out[1] = gz >= 0.0 ? 1 : -1;
+    const double ax = fabs(gx), az = fabs(gz);
+    if (ax < 0.25 * az) out[0] *= 2;
+    else if (az < 0.25 * ax) out[1] *= 2;
 }
 
 // Lateral flow direction of a teleseismic source for the column walk's
@@ -1939,8 +2282,7 @@ static void lateral_flow(const Grid& g, const double* face, int out[2])
         const double det = cxx * czz - cxz * cxz;
         if (det > 0.0) { gx = (cxt * czz - czt * cxz) / det; gz = (czt * cxx - cxt * cxz) / det; }
     }
-    out[0] = gx >= 0.0 ? 1 : -1;
-    out[1] = gz >= 0.0 ? 1 : -1;
+    flow_code(gx, gz, out);
 }
 
 extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int order, int precision, int flags)
@@ -1989,7 +2331,7 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     else
         k_init_tele<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff);
     k_fixbits_tele<<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, s.fix);
-    if (c->strategy == 1) {
+    if (c->strategy >= 1) {
         std::vector<int> flow((size_t)nsrc * 2);
         for (int q = 0; q < nsrc; ++q) lateral_flow(g, faces + (size_t)q * F, &flow[2 * q]);
         HIPCHK(c, hipMemcpyAsync(s.flow, flow.data(), flow.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -2018,8 +2360,7 @@ static void fixed_flow(const Grid& g, const int64_t* node, const double* tv, int
         const double det = cxx * czz - cxz * cxz;
         if (det > 0.0) { gx = (cxt * czz - czt * cxz) / det; gz = (czt * cxx - cxt * cxz) / det; }
     }
-    out[0] = gx >= 0.0 ? 1 : -1;
-    out[1] = gz >= 0.0 ? 1 : -1;
+    flow_code(gx, gz, out);
 }
 
 extern "C" int fmm3d_solve_fixed(fmm3d_ctx* c, int nsrc, const int64_t* offs, const int64_t* node, const double* tv,
@@ -2467,7 +2808,7 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const size_t N = c->N, NW = (N + 31) / 32;
     const Grid& g = c->g;
-    const int npx = (g.nnx + c->col_px - 1) / c->col_px, npz = (g.nnz + c->col_pz - 1) / c->col_pz;
+    const int npx = (g.nnx + patch_x(c) - 1) / patch_x(c), npz = (g.nnz + patch_z(c) - 1) / patch_z(c);
     const int np = npx * npz;
     const int64_t offs[2] = { 0, n };
     int64_t* doffs = c->eng_node + c->eng_stage - 2;   // the staging's last two entries hold the offsets
@@ -2515,7 +2856,7 @@ static int engine_sweeps(fmm3d_ctx* c, int nsweeps)
     int blocks = 0;
     int rc = column_args<R>(c, c->sl.cap, c->eng_order, a, blocks);
     if (rc) return rc;
-    rc = launch_chunk<R, FAST, SHIFT>(c, a, blocks, nsweeps, c->eng_k);
+    rc = launch_chunk<R, FAST, SHIFT>(c, a, blocks, nsweeps, c->eng_k, true);
     if (rc) return rc;
     c->eng_k += nsweeps;
     HIPCHK(c, hipMemcpyAsync(c->eng_lc, c->sl.lastchg, c->sl.cap * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -2632,8 +2973,14 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "direction must be 0 or 1");
         c->dirmode = iv;
     } else if (!strcmp(key, "strategy")) {
-        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "strategy must be 0 or 1");
+        if (iv < 0 || iv > 2) return fail(c, FMM3D_E_ARG, "strategy must be 0, 1 or 2");
         c->strategy = iv;
+    } else if (!strcmp(key, "plane_sweeps")) {
+        if (iv < 0 || iv > 1000) return fail(c, FMM3D_E_ARG, "plane_sweeps must be in 0..1000");
+        c->plane_sweeps = iv;
+    } else if (!strcmp(key, "plane_chunk")) {
+        if (iv != 2 && iv != 4 && iv != 8) return fail(c, FMM3D_E_ARG, "plane_chunk must be 2, 4 or 8");
+        c->plane_ch = iv;
     } else if (!strcmp(key, "column_x") || !strcmp(key, "column_z")) {
         if (iv != 8 && iv != 16 && iv != 32 && iv != 64) return fail(c, FMM3D_E_ARG, "%s must be 8, 16, 32 or 64", key);
         (key[7] == 'x' ? c->col_px : c->col_pz) = iv;
