@@ -168,6 +168,8 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *   "ordered"       column walk: 1 (default) patches start each layer after
  *                   their upstream neighbours (along the source's lateral
  *                   flow) finished it; 0 no ordering
+ *   "ordered_sweeps" column walk: only sweeps up to this one wait on their
+ *                   upstream patches (default 0: every sweep)
  *   "spin_limit"    polls before such a wait is given up (default 20000)
  *   "graph"         1 (default): replay 4-sweep chunks from a captured hipGraph
  *   "autotune"      1: the first solve of a kind (precision, order, walk,

@@ -386,6 +386,7 @@ template <typename R> struct ColArgs {
     unsigned long long* nodes;   // free nodes staged
     int* prog;              // per slot per patch: iter*pstride + layers (steps) finished, or null (no ordering)
     int pstride;            // progress counts per sweep (both column walks: nnr + 16)
+    int ordered_until;      // sweeps up to this one wait on their upstream patches (0: all)
     const int* flow;        // per slot: lateral flow signs (fx, fz), or null
     int spin_limit;         // polls before a wait on a neighbour is given up
     int down;               // 1: walk top to bottom (when walkalt is null)
@@ -512,7 +513,7 @@ k_column(ColArgs<R> a)
     int* st = a.stamp + (size_t)slot * np;
     const bool alt = a.walkalt && a.walkalt[slot];
     const bool down = a.walkalt ? (alt && !(iter & 1)) : (a.down != 0);
-    int* pg = (a.prog && !alt) ? a.prog + (size_t)slot * np : nullptr;
+    int* pg = (a.prog && !alt && (a.ordered_until == 0 || iter <= a.ordered_until)) ? a.prog + (size_t)slot * np : nullptr;
     const int pbase = iter * a.pstride;
     const int t = threadIdx.x;
     if (t == 0) {
@@ -532,6 +533,7 @@ k_column(ColArgs<R> a)
     const int ux = (bx - fx >= 0 && bx - fx < a.npx) ? b - fx * a.npz : -1;
     const int uz = (bz - fz >= 0 && bz - fz < a.npz) ? b - fz : -1;
     bool wx = pg && ux >= 0, wz = pg && uz >= 0;   // still waiting on them (given up after a timeout)
+    int seen_x = 0, seen_z = 0;                     // their progress as last read (lane 0)
     const size_t NL = (size_t)a.nnx * a.nnz;
     const size_t N = NL * a.nnr;
     R* T = a.T + (size_t)slot * N;
@@ -549,18 +551,24 @@ k_column(ColArgs<R> a)
         const int ir = down ? q : a.nnr - 1 - q;
         const size_t lay = (size_t)ir * NL;
         unsigned nfree = 0;
-        // wait (bounded) until the upstream patches have finished this layer
+        // wait (bounded) until the upstream patches have finished this layer;
+        // progress read earlier that already covers it saves the poll (an
+        // upstream patch that runs ahead covers many layers with one read)
         if (wx || wz) {
             if (t == 0) {
                 const int need = pbase + q + 1;
                 int spins = 0;
-                while (wx && __hip_atomic_load(pg + ux, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-                    if (++spins > a.spin_limit) { wx = false; break; }
-                    __builtin_amdgcn_s_sleep(2);
-                }
-                spins = 0;
-                while (wz && __hip_atomic_load(pg + uz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-                    if (++spins > a.spin_limit) { wz = false; break; }
+                while ((wx && seen_x < need) || (wz && seen_z < need)) {
+                    // both neighbours' counters in flight together
+                    const int vx = (wx && seen_x < need) ? __hip_atomic_load(pg + ux, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : seen_x;
+                    const int vz = (wz && seen_z < need) ? __hip_atomic_load(pg + uz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : seen_z;
+                    seen_x = vx; seen_z = vz;
+                    if ((!wx || seen_x >= need) && (!wz || seen_z >= need)) break;
+                    if (++spins > a.spin_limit) {
+                        if (seen_x < need) wx = false;
+                        if (seen_z < need) wz = false;
+                        break;
+                    }
                     __builtin_amdgcn_s_sleep(2);
                 }
                 s_act = (wx ? 1 : 0) | (wz ? 2 : 0);
@@ -1046,227 +1054,320 @@ __device__ __forceinline__ double tri_vel(const Grid& g, const double* __restric
 struct RayOut {
     int32_t* cols;      // staging, ray r at cols + r*cap
     float* vals;
-    int32_t* count;     // entries per ray (-1 = start outside the grid, -2 = overflow)
+    int32_t* count;     // entries per ray (-1 = start outside the grid, -2 = row overflow, -3 = path overflow)
     int64_t cap;
 };
 
-// One wavefront per ray.  The ray is stepped redundantly by all 64 lanes;
-// each lane owns the vertices whose (r, theta, phi) indices are congruent to
-// its (lane/16, lane/4 % 4, lane % 4) mod 4, so the 64 taps of every
-// sub-segment land on 64 different lanes and every vertex is only ever
-// accumulated by one lane (no cross-lane ordering is needed).  The dense
-// per-worker Frechet scratch is scanned over the touched box at the end.
+// A sample of a ray path for the Frechet pass: a step's start point (kind 0:
+// its B-spline weights and velocity open the step's first sub-segment) or a
+// sub-segment's end (kind 1: the taps of the sub-segment from the previous
+// sample to this one, over length dinc).  The weights are taken in vertex
+// cell (ivr, ivx, ivz).
+struct RaySample {
+    double r, x, z;
+    double vel;
+    double dinc;
+    int ivr, ivx, ivz;
+    int kind;
+};
+
+struct RayPaths {
+    RaySample* rec;     // ray q's samples at rec + q * cap
+    int32_t* n;         // samples per ray; -1 start outside the grid, -3 more than cap
+    int32_t* box;       // vertices touched per ray: rmin, rmax, xmin, xmax, zmin, zmax
+    int64_t cap;
+};
+
+// Ray paths, one lane per ray: the reference's ray tracer (F:7347-7541) --
+// the field's gradient in the current cell, a step of dpl against it,
+// clipping at the first grid face crossed -- and its split of each step at
+// the vertex-cell faces (F:7542-7600), recorded as samples; the velocity of
+// every sample is interpolated here, once per ray.  The lanes of a
+// wavefront trace different rays, so none of this float64 work is repeated
+// (k_ray_frechet then spreads each sample's 64 taps over 64 lanes).
 template <typename R>
 __global__ void __launch_bounds__(64)
-k_rays(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const double* __restrict__ veln,
-       int nslots, int nrc, const double* __restrict__ rcv, int ray0, int nrays, float* __restrict__ scratch,
-       RayOut out, const double* __restrict__ srcpos)
+k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const double* __restrict__ veln,
+            int nrc, const double* __restrict__ rcv, int ray0, int nrays, RayPaths P,
+            const double* __restrict__ srcpos)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nrays) return;
+    const int rid = ray0 + q;
+    const int NVR = g.nvr + 2, NVX = g.nvx + 2, NVZ = g.nvz + 2;
+    const int s = rid / nrc, i = rid % nrc;
+    const size_t N = (size_t)g.nnr * g.nnx * g.nnz;
+    const R* F = T + (size_t)s * N;
+    const double to = toff[s];
+    auto TT = [&](int r, int x, int z) -> double {   // 1-based node access
+        return field_at<R>(F, to, ((size_t)(r - 1) * g.nnx + (x - 1)) * g.nnz + (z - 1));
+    };
+    RaySample* out = P.rec + (int64_t)q * P.cap;
+    int64_t ns = 0;
+    bool over = false;
+    auto put = [&](double r, double x, double z, double vel, double dinc, int a, int b, int c, int kind) {
+        if (ns >= P.cap) { over = true; return; }
+        RaySample S;
+        S.r = r; S.x = x; S.z = z; S.vel = vel; S.dinc = dinc; S.ivr = a; S.ivx = b; S.ivz = c; S.kind = kind;
+        out[ns++] = S;
+    };
+    double r0 = rcv[3 * i], x0 = rcv[3 * i + 1], z0 = rcv[3 * i + 2];
+    int ipr = (int)floor((g.rgor - r0) / g.dnr) + 1;
+    int ipx = (int)floor((x0 - g.rgox) / g.dnx) + 1;
+    int ipz = (int)floor((z0 - g.rgoz) / g.dnz) + 1;
+    int sw = 0;
+    if (ipr < 1 || ipr >= g.nnr) sw = 1;
+    if (ipx < 1 || ipx >= g.nnx) sw = 1;
+    if (ipz < 1 || ipz >= g.nnz) sw = 1;
+    if (ipr == g.nnr) { ipr--; sw = 0; }
+    if (ipx == g.nnx) { ipx--; sw = 0; }
+    if (ipz == g.nnz) { ipz--; sw = 0; }
+    if (sw) { P.n[q] = -1; return; }
+    int bmin[3] = { 1 << 30, 1 << 30, 1 << 30 }, bmax[3] = { -1, -1, -1 };
+    double pr = r0, px = x0, pz = z0;
+    const double dpl = g.dnr;
+    // a point source ends the ray: within one step of it, the last step goes to the source
+    const double sr = srcpos[3 * s], sxs = srcpos[3 * s + 1], szs = srcpos[3 * s + 2];
+    const bool point = !isnan(sr);
+    double xsrc[3] = { 0.0, 0.0, 0.0 };
+    if (point) {
+        const double rr = g.earth + sr;
+        xsrc[0] = rr * sin(sxs) * cos(szs); xsrc[1] = rr * sin(sxs) * sin(szs); xsrc[2] = rr * cos(sxs);
+    }
+    for (int step = 1; step <= 100000 && !over; ++step) {
+        double dtr = 0.0;
+        int cnt = 0;
+        const int cz[4] = { 0, 1, 1, 0 }, cx[4] = { 0, 0, 1, 1 };
+        for (int k = 0; k < 4; ++k) {
+            double lo = TT(ipr + 1, ipx + cx[k], ipz + cz[k]);
+            double hi = TT(ipr, ipx + cx[k], ipz + cz[k]);
+            // teleseismic: only edges where t decreases downward (the
+            // reference's rule); a point source: every edge, since rays
+            // climb towards a source above them
+            if (point || lo < hi) { dtr = dtr + lo - hi; cnt++; }
+        }
+        if (cnt) dtr = dtr / (cnt * g.dnr);
+        double dtx = TT(ipr, ipx + 1, ipz) - TT(ipr, ipx, ipz);
+        dtx = dtx + TT(ipr, ipx + 1, ipz + 1) - TT(ipr, ipx, ipz + 1);
+        dtx = dtx + TT(ipr + 1, ipx + 1, ipz + 1) - TT(ipr + 1, ipx, ipz + 1);
+        dtx = dtx + TT(ipr + 1, ipx + 1, ipz) - TT(ipr + 1, ipx, ipz);
+        double rcell = g.earth + g.rgor - (ipr - 1) * g.dnr;
+        dtx = dtx / (4.0 * rcell * g.dnx);
+        double dtz = TT(ipr, ipx, ipz + 1) - TT(ipr, ipx, ipz);
+        dtz = dtz + TT(ipr + 1, ipx + 1, ipz + 1) - TT(ipr + 1, ipx + 1, ipz);
+        dtz = dtz + TT(ipr, ipx + 1, ipz + 1) - TT(ipr, ipx + 1, ipz);
+        dtz = dtz + TT(ipr + 1, ipx, ipz + 1) - TT(ipr + 1, ipx, ipz);
+        const double sx = sin(px);
+        dtz = dtz / (4.0 * rcell * sx * g.dnz);
+        double crad = g.earth + pr;
+        double gn = sqrt(dtr * dtr + dtx * dtx + dtz * dtz);
+        double nr_ = pr + dpl * dtr / gn;
+        double nx_ = px - dpl * dtx / (crad * gn);
+        double nz_ = pz - dpl * dtz / (crad * sx * gn);
+        double seg = dpl;   // length the Frechet increments use (the reference's nominal step)
+        int done = 0;
+        if (point) {
+            const double sp = sin(px), cp = cos(px);
+            const double d0 = crad * sp * cos(pz) - xsrc[0], d1 = crad * sp * sin(pz) - xsrc[1], d2 = crad * cp - xsrc[2];
+            const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            if (d <= dpl || !(gn > 0.0)) { nr_ = sr; nx_ = sxs; nz_ = szs; seg = d; done = 1; }
+        }
+        int ipro = ipr, ipxo = ipx, ipzo = ipz;
+        ipr = (int)floor((g.rgor - nr_) / g.dnr) + 1;
+        ipx = (int)floor((nx_ - g.rgox) / g.dnx) + 1;
+        ipz = (int)floor((nz_ - g.rgoz) / g.dnz) + 1;
+        if (done) {   // the source cell, with the last-node rule of the start point
+            ipr = min(max(ipr, 1), g.nnr - 1);
+            ipx = min(max(ipx, 1), g.nnx - 1);
+            ipz = min(max(ipz, 1), g.nnz - 1);
+        }
+        int exr = !done && (ipr < 1 || ipr >= g.nnr);
+        int exx = !done && (ipx < 1 || ipx >= g.nnx);
+        int exz = !done && (ipz < 1 || ipz >= g.nnz);
+        if (exr || exx || exz) {
+            // clip to the first face crossed (F:7463-7538)
+            int which = 0;
+            double atio = 0.0;
+            if (exr) {
+                double rb;
+                if (ipr < 1) { rb = g.rgor; ipr = 1; }
+                else { rb = g.rgor - (g.nnr - 1) * g.dnr; ipr = g.nnr - 1; }
+                atio = (rb - pr) / (nr_ - pr);
+                which = 1;
+            }
+            if (exx) {
+                double xb;
+                if (ipx < 1) { xb = g.rgox; ipx = 1; }
+                else { xb = g.rgox + (g.nnx - 1) * g.dnx; ipx = g.nnx - 1; }
+                double qq = (xb - px) / (nx_ - px);
+                if (which == 1) { if (qq < atio) { atio = qq; which = 2; } }
+                else { atio = qq; which = 2; }
+            }
+            if (exz) {
+                double zb;
+                if (ipz < 1) { zb = g.rgoz; ipz = 1; }
+                else { zb = g.rgoz + (g.nnz - 1) * g.dnz; ipz = g.nnz - 1; }
+                double qq = (zb - pz) / (nz_ - pz);
+                if (which != 0) { if (qq < atio) { atio = qq; which = 3; } }
+                else { atio = qq; which = 3; }
+            }
+            if (which == 1) { ipx = ipxo; ipz = ipzo; }
+            else if (which == 2) { ipr = ipro; ipz = ipzo; }
+            else if (which == 3) { ipr = ipro; ipx = ipxo; }
+            nz_ = pz + atio * (nz_ - pz);
+            nx_ = px + atio * (nx_ - px);
+            nr_ = pr + atio * (nr_ - pr);
+            done = 1;
+        }
+        // the step's sub-segments between vertex-cell faces (F:7542-7600)
+        int ivr = (int)floor((g.gor - nr_) / g.dvr) + 1;
+        int ivx = (int)floor((nx_ - g.gox) / g.dvx) + 1;
+        int ivz = (int)floor((nz_ - g.goz) / g.dvz) + 1;
+        int ivro = (int)floor((g.gor - pr) / g.dvr) + 1;
+        int ivxo = (int)floor((px - g.gox) / g.dvx) + 1;
+        int ivzo = (int)floor((pz - g.goz) / g.dvz) + 1;
+        int nhp = 0;
+        double vrat[4];
+        int chp[4];
+        if (ivr != ivro) {
+            double rb = (ivr > ivro) ? g.gor - (ivr - 1) * g.dvr : g.gor - ivr * g.dvr;
+            vrat[nhp] = (rb - pr) / (nr_ - pr); chp[nhp] = 1; nhp++;
+        }
+        if (ivx != ivxo) {
+            double xb = (ivx > ivxo) ? g.gox + (ivx - 1) * g.dvx : g.gox + ivx * g.dvx;
+            double qq = (xb - px) / (nx_ - px);
+            int p = nhp;
+            while (p > 0 && qq < vrat[p - 1]) { vrat[p] = vrat[p - 1]; chp[p] = chp[p - 1]; p--; }
+            vrat[p] = qq; chp[p] = 2; nhp++;
+        }
+        if (ivz != ivzo) {
+            double zb = (ivz > ivzo) ? g.goz + (ivz - 1) * g.dvz : g.goz + ivz * g.dvz;
+            double qq = (zb - pz) / (nz_ - pz);
+            int p = nhp;
+            while (p > 0 && qq < vrat[p - 1]) { vrat[p] = vrat[p - 1]; chp[p] = chp[p - 1]; p--; }
+            vrat[p] = qq; chp[p] = 3; nhp++;
+        }
+        vrat[nhp] = 1.0; chp[nhp] = 0; nhp++;
+        put(pr, px, pz,
+            tri_vel(g, veln, ipro, ipxo, ipzo, (g.rgor - pr) - (ipro - 1) * g.dnr,
+                    (px - g.rgox) - (ipxo - 1) * g.dnx, (pz - g.rgoz) - (ipzo - 1) * g.dnz),
+            0.0, ivro, ivxo, ivzo, 0);
+        int ivrt = ivro, ivxt = ivxo, ivzt = ivzo;
+        for (int k = 0; k < nhp; ++k) {
+            if (k > 0) {
+                if (chp[k - 1] == 1) ivrt = ivr;
+                else if (chp[k - 1] == 2) ivxt = ivx;
+                else if (chp[k - 1] == 3) ivzt = ivz;
+            }
+            double sz_ = pz + vrat[k] * (nz_ - pz);
+            double sx_ = px + vrat[k] * (nx_ - px);
+            double sr_ = pr + vrat[k] * (nr_ - pr);
+            int iprt = (int)floor((g.rgor - sr_) / g.dnr) + 1;
+            int ipxt = (int)floor((sx_ - g.rgox) / g.dnx) + 1;
+            int ipzt = (int)floor((sz_ - g.rgoz) / g.dnz) + 1;
+            const double vel = tri_vel(g, veln, iprt, ipxt, ipzt,
+                                       (g.rgor - sr_) - (iprt - 1) * g.dnr,
+                                       (sx_ - g.rgox) - (ipxt - 1) * g.dnx,
+                                       (sz_ - g.rgoz) - (ipzt - 1) * g.dnz);
+            const double dinc = (k == 0) ? vrat[0] * seg : (vrat[k] - vrat[k - 1]) * seg;
+            put(sr_, sx_, sz_, vel, dinc, ivrt, ivxt, ivzt, 1);
+            bmin[0] = min(bmin[0], max(ivrt - 1, 0)); bmax[0] = max(bmax[0], min(ivrt + 2, NVR - 1));
+            bmin[1] = min(bmin[1], max(ivxt - 1, 0)); bmax[1] = max(bmax[1], min(ivxt + 2, NVX - 1));
+            bmin[2] = min(bmin[2], max(ivzt - 1, 0)); bmax[2] = max(bmax[2], min(ivzt + 2, NVZ - 1));
+        }
+        pr = nr_; px = nx_; pz = nz_;
+        if (done) break;
+    }
+    P.n[q] = over ? -3 : (int32_t)ns;
+    int32_t* bx = P.box + (int64_t)q * 6;
+    bx[0] = bmin[0]; bx[1] = bmax[0]; bx[2] = bmin[1]; bx[3] = bmax[1]; bx[4] = bmin[2]; bx[5] = bmax[2];
+}
+
+// Lane j's double as a wavefront-uniform value.
+__device__ __forceinline__ double lane_double(double v, int j)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, j);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), j);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Frechet rows from recorded ray paths, one wavefront per ray (F:7600-7764).
+// The samples come in 64 at a time, one per lane, and are broadcast one
+// after another.
+// Each lane owns the vertices whose (r, theta, phi) indices are congruent
+// to its (lane/16, lane/4 % 4, lane % 4) mod 4, so the 64 taps of every
+// sub-segment land on 64 different lanes and every vertex is only ever
+// accumulated by one lane (no cross-lane ordering is needed).  A lane keeps
+// its current vertex's value in a register while the ray's B-spline cell
+// keeps that vertex in its support and writes it back to the dense
+// per-worker scratch when the vertex changes; each increment is the
+// reference's float(double(acc) + inc).  The scratch is then scanned over
+// the touched box in column order, keeping |v| >= 1e-6 (F:7744).
+__global__ void __launch_bounds__(64)
+k_ray_frechet(Grid g, int nrays, RayPaths P, float* __restrict__ scratch, RayOut out)
 {
     const int lane = threadIdx.x;
     const int NVR = g.nvr + 2, NVX = g.nvx + 2, NVZ = g.nvz + 2;
     const size_t NV = (size_t)NVR * NVX * NVZ;
     float* fdm = scratch + (size_t)blockIdx.x * NV;
     const int own_r = lane / 16, own_x = (lane / 4) % 4, own_z = lane % 4;
-    for (int rid = ray0 + blockIdx.x; rid < ray0 + nrays; rid += gridDim.x) {
-        const int s = rid / nrc, i = rid % nrc;
-        const size_t N = (size_t)g.nnr * g.nnx * g.nnz;
-        const R* F = T + (size_t)s * N;
-        const double to = toff[s];
-        auto TT = [&](int r, int x, int z) -> double {   // 1-based node access
-            return field_at<R>(F, to, ((size_t)(r - 1) * g.nnx + (x - 1)) * g.nnz + (z - 1));
-        };
-        const int64_t orow = (int64_t)(rid - ray0) * out.cap;
-        double r0 = rcv[3 * i], x0 = rcv[3 * i + 1], z0 = rcv[3 * i + 2];
-        int ipr = (int)floor((g.rgor - r0) / g.dnr) + 1;
-        int ipx = (int)floor((x0 - g.rgox) / g.dnx) + 1;
-        int ipz = (int)floor((z0 - g.rgoz) / g.dnz) + 1;
-        int sw = 0;
-        if (ipr < 1 || ipr >= g.nnr) sw = 1;
-        if (ipx < 1 || ipx >= g.nnx) sw = 1;
-        if (ipz < 1 || ipz >= g.nnz) sw = 1;
-        if (ipr == g.nnr) { ipr--; sw = 0; }
-        if (ipx == g.nnx) { ipx--; sw = 0; }
-        if (ipz == g.nnz) { ipz--; sw = 0; }
-        if (sw) { if (lane == 0) out.count[rid - ray0] = -1; continue; }
-        int bmin[3] = { 1 << 30, 1 << 30, 1 << 30 }, bmax[3] = { -1, -1, -1 };
-        double pr = r0, px = x0, pz = z0;
-        const double dpl = g.dnr;
-        // a point source ends the ray: within one step of it, the last step goes to the source
-        const double sr = srcpos[3 * s], sxs = srcpos[3 * s + 1], szs = srcpos[3 * s + 2];
-        const bool point = !isnan(sr);
-        double xsrc[3] = { 0.0, 0.0, 0.0 };
-        if (point) {
-            const double rr = g.earth + sr;
-            xsrc[0] = rr * sin(sxs) * cos(szs); xsrc[1] = rr * sin(sxs) * sin(szs); xsrc[2] = rr * cos(sxs);
+    for (int q = blockIdx.x; q < nrays; q += gridDim.x) {
+        const int ns = P.n[q];
+        if (ns < 0) { if (lane == 0) out.count[q] = ns; continue; }
+        const RaySample* S = P.rec + (int64_t)q * P.cap;
+        int64_t acc_at = -1;   // scratch index of the vertex held in acc, -1 none
+        float acc = 0.0f;
+        double ui[4] = { 0, 0, 0, 0 }, vi[4] = { 0, 0, 0, 0 }, wi[4] = { 0, 0, 0, 0 }, vel = 1.0;
+        RaySample mine;   // this lane's sample of the current batch of 64
+        mine.r = mine.x = mine.z = mine.vel = mine.dinc = 0.0;
+        mine.ivr = mine.ivx = mine.ivz = mine.kind = 0;
+        for (int j = 0; j < ns; ++j) {
+            const int jb = j & 63;
+            if (jb == 0) {
+                const int k = j + lane;
+                if (k < ns) mine = S[k];
+                else { mine.r = mine.x = mine.z = mine.vel = mine.dinc = 0.0; mine.ivr = mine.ivx = mine.ivz = mine.kind = 0; }
+            }
+            RaySample sm;
+            sm.r = lane_double(mine.r, jb); sm.x = lane_double(mine.x, jb); sm.z = lane_double(mine.z, jb);
+            sm.vel = lane_double(mine.vel, jb); sm.dinc = lane_double(mine.dinc, jb);
+            sm.ivr = __builtin_amdgcn_readlane(mine.ivr, jb); sm.ivx = __builtin_amdgcn_readlane(mine.ivx, jb);
+            sm.ivz = __builtin_amdgcn_readlane(mine.ivz, jb); sm.kind = __builtin_amdgcn_readlane(mine.kind, jb);
+            double uio[4], vio[4], wio[4];
+            for (int k = 0; k < 4; ++k) { uio[k] = ui[k]; vio[k] = vi[k]; wio[k] = wi[k]; }
+            const double velo = vel;
+            vel = sm.vel;
+            bspline4(((g.gor - sm.r) - (sm.ivr - 1) * g.dvr) / g.dvr, ui);
+            bspline4(((sm.x - g.gox) - (sm.ivx - 1) * g.dvx) / g.dvx, vi);
+            bspline4(((sm.z - g.goz) - (sm.ivz - 1) * g.dvz) / g.dvz, wi);
+            if (sm.kind != 1) continue;
+            // this lane's tap: vertex (iv - 1 + n) with n the lane's residue offset
+            const int rr = sm.ivr - 1 + (((own_r - (sm.ivr - 1)) % 4 + 4) % 4);
+            const int xx = sm.ivx - 1 + (((own_x - (sm.ivx - 1)) % 4 + 4) % 4);
+            const int zz = sm.ivz - 1 + (((own_z - (sm.ivz - 1)) % 4 + 4) % 4);
+            const int n = rr - (sm.ivr - 1), m = xx - (sm.ivx - 1), l = zz - (sm.ivz - 1);
+            const double e1 = ui[n] * vi[m] * wi[l] / (vel * vel);
+            const double e0 = uio[n] * vio[m] * wio[l] / (velo * velo);
+            const double inc = -(e1 + e0) * sm.dinc / 2.0;
+            if (zz >= 0 && zz < NVZ && xx >= 0 && xx < NVX && rr >= 0 && rr < NVR) {
+                const int64_t at = ((int64_t)zz * NVX + xx) * NVR + rr;
+                if (at != acc_at) {
+                    if (acc_at >= 0) fdm[acc_at] = acc;
+                    acc = fdm[at];
+                    acc_at = at;
+                }
+                acc = (float)(inc + (double)acc);
+            }
         }
-        for (int step = 1; step <= 100000; ++step) {
-            double dtr = 0.0;
-            int cnt = 0;
-            const int cz[4] = { 0, 1, 1, 0 }, cx[4] = { 0, 0, 1, 1 };
-            for (int q = 0; q < 4; ++q) {
-                double lo = TT(ipr + 1, ipx + cx[q], ipz + cz[q]);
-                double hi = TT(ipr, ipx + cx[q], ipz + cz[q]);
-                // teleseismic: only edges where t decreases downward (the
-                // reference's rule); a point source: every edge, since rays
-                // climb towards a source above them
-                if (point || lo < hi) { dtr = dtr + lo - hi; cnt++; }
-            }
-            if (cnt) dtr = dtr / (cnt * g.dnr);
-            double dtx = TT(ipr, ipx + 1, ipz) - TT(ipr, ipx, ipz);
-            dtx = dtx + TT(ipr, ipx + 1, ipz + 1) - TT(ipr, ipx, ipz + 1);
-            dtx = dtx + TT(ipr + 1, ipx + 1, ipz + 1) - TT(ipr + 1, ipx, ipz + 1);
-            dtx = dtx + TT(ipr + 1, ipx + 1, ipz) - TT(ipr + 1, ipx, ipz);
-            double rcell = g.earth + g.rgor - (ipr - 1) * g.dnr;
-            dtx = dtx / (4.0 * rcell * g.dnx);
-            double dtz = TT(ipr, ipx, ipz + 1) - TT(ipr, ipx, ipz);
-            dtz = dtz + TT(ipr + 1, ipx + 1, ipz + 1) - TT(ipr + 1, ipx + 1, ipz);
-            dtz = dtz + TT(ipr, ipx + 1, ipz + 1) - TT(ipr, ipx + 1, ipz);
-            dtz = dtz + TT(ipr + 1, ipx, ipz + 1) - TT(ipr + 1, ipx, ipz);
-            const double sx = sin(px);
-            dtz = dtz / (4.0 * rcell * sx * g.dnz);
-            double crad = g.earth + pr;
-            double gn = sqrt(dtr * dtr + dtx * dtx + dtz * dtz);
-            double nr_ = pr + dpl * dtr / gn;
-            double nx_ = px - dpl * dtx / (crad * gn);
-            double nz_ = pz - dpl * dtz / (crad * sx * gn);
-            double seg = dpl;   // length the Frechet increments use (the reference's nominal step)
-            int done = 0;
-            if (point) {
-                const double sp = sin(px), cp = cos(px);
-                const double d0 = crad * sp * cos(pz) - xsrc[0], d1 = crad * sp * sin(pz) - xsrc[1], d2 = crad * cp - xsrc[2];
-                const double d = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                if (d <= dpl || !(gn > 0.0)) { nr_ = sr; nx_ = sxs; nz_ = szs; seg = d; done = 1; }
-            }
-            int ipro = ipr, ipxo = ipx, ipzo = ipz;
-            ipr = (int)floor((g.rgor - nr_) / g.dnr) + 1;
-            ipx = (int)floor((nx_ - g.rgox) / g.dnx) + 1;
-            ipz = (int)floor((nz_ - g.rgoz) / g.dnz) + 1;
-            if (done) {   // the source cell, with the last-node rule of the start point
-                ipr = min(max(ipr, 1), g.nnr - 1);
-                ipx = min(max(ipx, 1), g.nnx - 1);
-                ipz = min(max(ipz, 1), g.nnz - 1);
-            }
-            int exr = !done && (ipr < 1 || ipr >= g.nnr);
-            int exx = !done && (ipx < 1 || ipx >= g.nnx);
-            int exz = !done && (ipz < 1 || ipz >= g.nnz);
-            if (exr || exx || exz) {
-                // clip to the first face crossed (F:7463-7538)
-                int which = 0;
-                double atio = 0.0;
-                if (exr) {
-                    double rb;
-                    if (ipr < 1) { rb = g.rgor; ipr = 1; }
-                    else { rb = g.rgor - (g.nnr - 1) * g.dnr; ipr = g.nnr - 1; }
-                    atio = (rb - pr) / (nr_ - pr);
-                    which = 1;
-                }
-                if (exx) {
-                    double xb;
-                    if (ipx < 1) { xb = g.rgox; ipx = 1; }
-                    else { xb = g.rgox + (g.nnx - 1) * g.dnx; ipx = g.nnx - 1; }
-                    double q = (xb - px) / (nx_ - px);
-                    if (which == 1) { if (q < atio) { atio = q; which = 2; } }
-                    else { atio = q; which = 2; }
-                }
-                if (exz) {
-                    double zb;
-                    if (ipz < 1) { zb = g.rgoz; ipz = 1; }
-                    else { zb = g.rgoz + (g.nnz - 1) * g.dnz; ipz = g.nnz - 1; }
-                    double q = (zb - pz) / (nz_ - pz);
-                    if (which != 0) { if (q < atio) { atio = q; which = 3; } }
-                    else { atio = q; which = 3; }
-                }
-                if (which == 1) { ipx = ipxo; ipz = ipzo; }
-                else if (which == 2) { ipr = ipro; ipz = ipzo; }
-                else if (which == 3) { ipr = ipro; ipx = ipxo; }
-                nz_ = pz + atio * (nz_ - pz);
-                nx_ = px + atio * (nx_ - px);
-                nr_ = pr + atio * (nr_ - pr);
-                done = 1;
-            }
-            // Frechet contribution of this step (F:7542-7737)
-            int ivr = (int)floor((g.gor - nr_) / g.dvr) + 1;
-            int ivx = (int)floor((nx_ - g.gox) / g.dvx) + 1;
-            int ivz = (int)floor((nz_ - g.goz) / g.dvz) + 1;
-            int ivro = (int)floor((g.gor - pr) / g.dvr) + 1;
-            int ivxo = (int)floor((px - g.gox) / g.dvx) + 1;
-            int ivzo = (int)floor((pz - g.goz) / g.dvz) + 1;
-            int nhp = 0;
-            double vrat[4];
-            int chp[4];
-            if (ivr != ivro) {
-                double rb = (ivr > ivro) ? g.gor - (ivr - 1) * g.dvr : g.gor - ivr * g.dvr;
-                vrat[nhp] = (rb - pr) / (nr_ - pr); chp[nhp] = 1; nhp++;
-            }
-            if (ivx != ivxo) {
-                double xb = (ivx > ivxo) ? g.gox + (ivx - 1) * g.dvx : g.gox + ivx * g.dvx;
-                double q = (xb - px) / (nx_ - px);
-                int p = nhp;
-                while (p > 0 && q < vrat[p - 1]) { vrat[p] = vrat[p - 1]; chp[p] = chp[p - 1]; p--; }
-                vrat[p] = q; chp[p] = 2; nhp++;
-            }
-            if (ivz != ivzo) {
-                double zb = (ivz > ivzo) ? g.goz + (ivz - 1) * g.dvz : g.goz + ivz * g.dvz;
-                double q = (zb - pz) / (nz_ - pz);
-                int p = nhp;
-                while (p > 0 && q < vrat[p - 1]) { vrat[p] = vrat[p - 1]; chp[p] = chp[p - 1]; p--; }
-                vrat[p] = q; chp[p] = 3; nhp++;
-            }
-            vrat[nhp] = 1.0; chp[nhp] = 0; nhp++;
-            double vel = tri_vel(g, veln, ipro, ipxo, ipzo,
-                                 (g.rgor - pr) - (ipro - 1) * g.dnr,
-                                 (px - g.rgox) - (ipxo - 1) * g.dnx,
-                                 (pz - g.rgoz) - (ipzo - 1) * g.dnz);
-            double ui[4], vi[4], wi[4];
-            bspline4(((g.gor - pr) - (ivro - 1) * g.dvr) / g.dvr, ui);
-            bspline4(((px - g.gox) - (ivxo - 1) * g.dvx) / g.dvx, vi);
-            bspline4(((pz - g.goz) - (ivzo - 1) * g.dvz) / g.dvz, wi);
-            int ivrt = ivro, ivxt = ivxo, ivzt = ivzo;
-            for (int k = 0; k < nhp; ++k) {
-                double velo = vel, uio[4], vio[4], wio[4];
-                for (int q = 0; q < 4; ++q) { uio[q] = ui[q]; vio[q] = vi[q]; wio[q] = wi[q]; }
-                if (k > 0) {
-                    if (chp[k - 1] == 1) ivrt = ivr;
-                    else if (chp[k - 1] == 2) ivxt = ivx;
-                    else if (chp[k - 1] == 3) ivzt = ivz;
-                }
-                double sz_ = pz + vrat[k] * (nz_ - pz);
-                double sx_ = px + vrat[k] * (nx_ - px);
-                double sr_ = pr + vrat[k] * (nr_ - pr);
-                int iprt = (int)floor((g.rgor - sr_) / g.dnr) + 1;
-                int ipxt = (int)floor((sx_ - g.rgox) / g.dnx) + 1;
-                int ipzt = (int)floor((sz_ - g.rgoz) / g.dnz) + 1;
-                vel = tri_vel(g, veln, iprt, ipxt, ipzt,
-                              (g.rgor - sr_) - (iprt - 1) * g.dnr,
-                              (sx_ - g.rgox) - (ipxt - 1) * g.dnx,
-                              (sz_ - g.rgoz) - (ipzt - 1) * g.dnz);
-                bspline4(((g.gor - sr_) - (ivrt - 1) * g.dvr) / g.dvr, ui);
-                bspline4(((sx_ - g.gox) - (ivxt - 1) * g.dvx) / g.dvx, vi);
-                bspline4(((sz_ - g.goz) - (ivzt - 1) * g.dvz) / g.dvz, wi);
-                double dinc = (k == 0) ? vrat[0] * seg : (vrat[k] - vrat[k - 1]) * seg;
-                // this lane's tap: vertex (ivt-1+n) with n the lane's residue offset
-                int rr = ivrt - 1 + (((own_r - (ivrt - 1)) % 4 + 4) % 4);
-                int xx = ivxt - 1 + (((own_x - (ivxt - 1)) % 4 + 4) % 4);
-                int zz = ivzt - 1 + (((own_z - (ivzt - 1)) % 4 + 4) % 4);
-                int n = rr - (ivrt - 1), m = xx - (ivxt - 1), l = zz - (ivzt - 1);
-                double e1 = ui[n] * vi[m] * wi[l] / (vel * vel);
-                double e0 = uio[n] * vio[m] * wio[l] / (velo * velo);
-                double inc = -(e1 + e0) * dinc / 2.0;
-                if (zz >= 0 && zz < NVZ && xx >= 0 && xx < NVX && rr >= 0 && rr < NVR) {
-                    float* p = fdm + ((size_t)zz * NVX + xx) * NVR + rr;
-                    *p = (float)(inc + (double)*p);
-                }
-                bmin[0] = min(bmin[0], max(ivrt - 1, 0)); bmax[0] = max(bmax[0], min(ivrt + 2, NVR - 1));
-                bmin[1] = min(bmin[1], max(ivxt - 1, 0)); bmax[1] = max(bmax[1], min(ivxt + 2, NVX - 1));
-                bmin[2] = min(bmin[2], max(ivzt - 1, 0)); bmax[2] = max(bmax[2], min(ivzt + 2, NVZ - 1));
-            }
-            pr = nr_; px = nx_; pz = nz_;
-            if (done) break;
-        }
+        if (acc_at >= 0) fdm[acc_at] = acc;
         __syncthreads();
         // sparsify the touched box in column order (F:7739-7764)
-        const int BR = bmax[0] - bmin[0] + 1, BX = bmax[1] - bmin[1] + 1, BZ = bmax[2] - bmin[2] + 1;
-        const int nbox = (bmax[0] >= 0) ? BR * BX * BZ : 0;
+        const int32_t* bx = P.box + (int64_t)q * 6;
+        const int b0 = bx[0], b1 = bx[1], b2 = bx[2], b3 = bx[3], b4 = bx[4], b5 = bx[5];
+        const int BR = b1 - b0 + 1, BX = b3 - b2 + 1, BZ = b5 - b4 + 1;
+        const int nbox = (b1 >= 0) ? BR * BX * BZ : 0;
+        const int64_t orow = (int64_t)q * out.cap;
         int64_t written = 0;
         bool overflow = false;
         for (int base = 0; base < nbox; base += 64) {
@@ -1274,8 +1375,8 @@ k_rays(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const d
             float v = 0.0f;
             size_t vi_ = 0;
             if (e < nbox) {
-                int br = e % BR, bx = (e / BR) % BX, bz = e / (BR * BX);
-                vi_ = ((size_t)(bmin[2] + bz) * NVX + (bmin[1] + bx)) * NVR + (bmin[0] + br);
+                int br = e % BR, bxx = (e / BR) % BX, bz = e / (BR * BX);
+                vi_ = ((size_t)(b4 + bz) * NVX + (b2 + bxx)) * NVR + (b0 + br);
                 v = fdm[vi_];
                 if (v != 0.0f) fdm[vi_] = 0.0f;
             }
@@ -1294,7 +1395,7 @@ k_rays(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const d
             written += __popcll(mask);
         }
         uint64_t ov = __ballot(overflow);
-        if (lane == 0) out.count[rid - ray0] = ov ? -2 : (int32_t)written;
+        if (lane == 0) out.count[q] = ov ? -2 : (int32_t)written;
         __syncthreads();
     }
 }
@@ -1348,6 +1449,7 @@ struct fmm3d_ctx {
     int plane_ch = 4;              // hyperplane walk: steps per staging chunk (2, 4 or 8)
     int plane_sweeps = 0;          // hyperplane walk: sweeps before the column walk takes over (0: all)
     int use_graph = 1;             // replay sweep chunks from a captured hipGraph
+    int ordered_until = 0;         // column walk: sweeps up to this one wait on upstream patches (0: all)
     int autotune = 0;              // time candidate patch shapes / pass caps at first use
     std::vector<std::array<int, 8>> tuned;   // key (prec, order, walk, batch class) -> (px, pz, passes)
     int* d_iter = nullptr;         // device sweep counter (graph replays)
@@ -1391,6 +1493,12 @@ struct fmm3d_ctx {
     float* fr_pvals = nullptr;
     size_t fr_packed = 0;          // entries the packed arrays hold
     int64_t fr_rcap = 2048;        // staging entries per ray (grows on overflow)
+    RaySample* fr_path = nullptr;  // ray path samples, fr_pcap per ray
+    int32_t* fr_pn = nullptr;      // samples per ray
+    int32_t* fr_pbox = nullptr;    // touched vertex box per ray
+    size_t fr_path_n = 0;          // samples fr_path holds
+    size_t fr_path_rays = 0;       // rays fr_pn / fr_pbox hold
+    int64_t fr_pcap = 0;           // path samples per ray (grows on overflow)
 };
 
 static std::string g_create_err;
@@ -1592,6 +1700,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
     (void)hipFree(c->hx64); (void)hipFree(c->hz64); (void)hipFree(c->hx32); (void)hipFree(c->hz32);
     (void)hipFree(c->d_updates); (void)hipFree(c->d_visits); (void)hipFree(c->ray_scratch);
+    (void)hipFree(c->fr_path); (void)hipFree(c->fr_pn); (void)hipFree(c->fr_pbox);
     (void)hipFree(c->fr_cols); (void)hipFree(c->fr_vals); (void)hipFree(c->fr_count); (void)hipFree(c->fr_off);
     (void)hipFree(c->fr_pcols); (void)hipFree(c->fr_pvals);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1999,6 +2108,7 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     a.nodes = c->d_visits + 2;
     a.prog = c->ordered ? s.prog : nullptr;
     a.pstride = g.nnr + 16;
+    a.ordered_until = c->ordered_until;
     a.walkalt = s.walkalt;
     a.flow = s.flow;
     a.spin_limit = c->spin_limit;
@@ -2651,12 +2761,16 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
         HIPCHK(c, hipMemsetAsync(c->ray_scratch, 0, workers * NV * sizeof(float), c->stream));
         c->ray_scratch_workers = workers;
     }
-    // Rays go in chunks whose staging (fr_rcap entries per ray) fits a fixed
-    // budget.  Per chunk: rays -> counts to the host -> row offsets -> rows
-    // packed on the device -> two copies into the caller's arrays.  A chunk
-    // with a row longer than the staging runs again with room for the whole
-    // vertex grid.
+    // Rays go in chunks whose staging (fr_rcap entries per ray) and path
+    // samples (fr_pcap per ray) fit fixed budgets.  Per chunk: paths, one
+    // lane per ray -> Frechet rows, one wavefront per ray -> counts to the
+    // host -> row offsets -> rows packed on the device -> two copies into
+    // the caller's arrays.  A chunk with a row longer than the staging runs
+    // again with room for the whole vertex grid, one with a path longer than
+    // its samples with four times the samples.
     const size_t budget = (size_t)512 << 20;   // bytes of staging
+    const size_t pbudget = (size_t)2 << 30;    // bytes of path samples
+    if (c->fr_pcap <= 0) c->fr_pcap = 2 * ((int64_t)g.nnr + g.nnx + g.nnz) + 64;
     double* drcv = nullptr;
     HIPCHK(c, dalloc(&drcv, (size_t)nrc * 3 * sizeof(double)));
     hipError_t e = hipMemcpyAsync(drcv, rcv, (size_t)nrc * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
@@ -2666,7 +2780,24 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
     bool short_cap = false;
     for (int r0 = 0; r0 < nrays && e == hipSuccess;) {
         const int64_t rcap = std::min<int64_t>(c->fr_rcap, (int64_t)NV);
-        const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(nrays - r0, (int64_t)(budget / (rcap * 8))));
+        const int64_t pcap = c->fr_pcap;
+        const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>({ (int64_t)(nrays - r0), (int64_t)(budget / (rcap * 8)),
+                                                                        (int64_t)(pbudget / (pcap * sizeof(RaySample))) }));
+        if (c->fr_path_n < (size_t)chunk * pcap) {
+            (void)hipFree(c->fr_path);
+            c->fr_path = nullptr; c->fr_path_n = 0;
+            e = dalloc(&c->fr_path, (size_t)chunk * pcap * sizeof(RaySample));
+            if (e != hipSuccess) break;
+            c->fr_path_n = (size_t)chunk * pcap;
+        }
+        if (c->fr_path_rays < (size_t)chunk) {
+            (void)hipFree(c->fr_pn); (void)hipFree(c->fr_pbox);
+            c->fr_pn = nullptr; c->fr_pbox = nullptr; c->fr_path_rays = 0;
+            e = dalloc(&c->fr_pn, (size_t)chunk * sizeof(int32_t));
+            if (e == hipSuccess) e = dalloc(&c->fr_pbox, (size_t)chunk * 6 * sizeof(int32_t));
+            if (e != hipSuccess) break;
+            c->fr_path_rays = chunk;
+        }
         if (c->fr_stage < (size_t)chunk * rcap) {
             (void)hipFree(c->fr_cols); (void)hipFree(c->fr_vals);
             c->fr_cols = nullptr; c->fr_vals = nullptr; c->fr_stage = 0;
@@ -2685,18 +2816,29 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
         }
         const int nr = chunk;
         RayOut o{ c->fr_cols, c->fr_vals, c->fr_count, rcap };
-        int grid = (int)std::min<size_t>(workers, (size_t)nr);
+        RayPaths pp{ c->fr_path, c->fr_pn, c->fr_pbox, pcap };
+        const int pgrid = (nr + 63) / 64;
         if (c->sl.prec == FMM3D_PREC_FP64)
-            k_rays<double><<<grid, 64, 0, c->stream>>>(g, (const double*)c->sl.T, c->sl.toff, c->veln, ns, nrc, drcv, r0, nr, c->ray_scratch, o, c->sl.srcpos);
+            k_ray_paths<double><<<pgrid, 64, 0, c->stream>>>(g, (const double*)c->sl.T, c->sl.toff, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos);
         else
-            k_rays<float><<<grid, 64, 0, c->stream>>>(g, (const float*)c->sl.T, c->sl.toff, c->veln, ns, nrc, drcv, r0, nr, c->ray_scratch, o, c->sl.srcpos);
+            k_ray_paths<float><<<pgrid, 64, 0, c->stream>>>(g, (const float*)c->sl.T, c->sl.toff, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos);
         e = hipGetLastError();
+        if (e == hipSuccess) {
+            const int grid = (int)std::min<size_t>(workers, (size_t)nr);
+            k_ray_frechet<<<grid, 64, 0, c->stream>>>(g, nr, pp, c->ray_scratch, o);
+            e = hipGetLastError();
+        }
         cnt.resize(nr);
         if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), c->fr_count, nr * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) break;
-        bool over = false;
-        for (int q = 0; q < nr; ++q) over |= cnt[q] == -2;
+        bool over = false, pover = false;
+        for (int q = 0; q < nr; ++q) { over |= cnt[q] == -2; pover |= cnt[q] == -3; }
+        if (pover) {   // a path longer than its samples: four times the samples (a ray has at most 5e5)
+            if (pcap >= 500000) { e = hipErrorUnknown; break; }
+            c->fr_pcap = std::min<int64_t>(4 * pcap, 500000);
+            continue;
+        }
         if (over) {
             if (rcap >= (int64_t)NV) { e = hipErrorUnknown; break; }   // cannot happen: a row has at most NV entries
             c->fr_rcap = (int64_t)NV;
@@ -2736,7 +2878,7 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
         r0 += nr;
     }
     (void)hipFree(drcv);
-    if (e == hipErrorUnknown) return fail(c, FMM3D_E_DEVICE, "frechet: row staging overflow");
+    if (e == hipErrorUnknown) return fail(c, FMM3D_E_DEVICE, "frechet: row staging or path overflow");
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "frechet: %s", hipGetErrorString(e));
     if (short_cap) return fail(c, FMM3D_E_CAPACITY, "fmm3d_frechet: capacity %lld < %lld entries", (long long)cap, (long long)row_ptr[nrays]);
     return FMM3D_OK;
@@ -2987,6 +3129,9 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "ordered")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "ordered must be 0 or 1");
         c->ordered = iv;
+    } else if (!strcmp(key, "ordered_sweeps")) {
+        if (iv < 0 || iv > 1000000) return fail(c, FMM3D_E_ARG, "ordered_sweeps out of range");
+        c->ordered_until = iv;
     } else if (!strcmp(key, "spin_limit")) {
         if (iv < 0 || iv > 10000000) return fail(c, FMM3D_E_ARG, "spin_limit out of range");
         c->spin_limit = iv;

@@ -196,22 +196,29 @@ class Context:
         return out.reshape(self.nsrc, len(rcv))
 
     def frechet(self, rcv: np.ndarray, cap: int = 0) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
-        """(row_ptr int64 [nsrc*nrc+1], cols int32, vals float32)."""
+        """(row_ptr int64 [nsrc*nrc+1], cols int32, vals float32).
+
+        The arrays are sized from the previous call's entry count per row
+        (rows of one model and receiver set change little between calls),
+        retried once at the exact size when short; cols and vals are views
+        of the first row_ptr[-1] entries."""
         rcv = np.ascontiguousarray(rcv, dtype=np.float64)
         nrows = self.nsrc * len(rcv)
         if cap <= 0:
-            cap = max(1024, nrows * 512)
+            per = getattr(self, "_fr_per_row", 512.0)
+            cap = max(1024, int(nrows * per * 1.05) + 1024)
         for _ in range(2):
-            rp = np.zeros(nrows + 1, dtype=np.int64)
-            cols = np.zeros(cap, dtype=np.int32)
-            vals = np.zeros(cap, dtype=np.float32)
+            rp = np.empty(nrows + 1, dtype=np.int64)
+            cols = np.empty(cap, dtype=np.int32)
+            vals = np.empty(cap, dtype=np.float32)
             rc = self.L.fmm3d_frechet(self._h, len(rcv), rcv, rp, cols, vals, cap)
             if rc == E_CAPACITY:
                 cap = int(rp[-1])
                 continue
             self._check(rc)
             n = int(rp[-1])
-            return rp, cols[:n].copy(), vals[:n].copy()
+            self._fr_per_row = max(1.0, n / max(nrows, 1))
+            return rp, cols[:n], vals[:n]
         raise FmmError(E_CAPACITY, "frechet capacity retry failed")
 
     def set_option(self, key: str, value: float) -> None:
