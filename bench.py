@@ -77,7 +77,7 @@ def one_step(ctx, faces, rcv, prec):
     ctx.solve_tele(faces, lib.ORDER_MIXED, prec)
     st = ctx.stats()
     t = ctx.receivers(rcv)
-    rp, cols, vals = ctx.frechet(rcv)
+    rp, cols, vals = ctx.frechet(rcv, pinned=True)
     return t, st, int(rp[-1])
 
 

@@ -139,6 +139,15 @@ int fmm3d_receivers(fmm3d_ctx *ctx, int nrc, const double *rcv, float *t_out);
 int fmm3d_frechet(fmm3d_ctx *ctx, int nrc, const double *rcv, int64_t *row_ptr,
                   int32_t *cols, float *vals, int64_t cap);
 
+/* fmm3d_frechet with the rows in pinned host arrays owned by the context:
+ * on success *row_ptr (nsrc*nrc+1 entries), *cols and *vals (*nnz entries)
+ * point into them and stay valid until the next fmm3d_frechet_pinned call on
+ * ctx or fmm3d_destroy.  The device-to-host copies then go straight into the
+ * result; no counterpart in the reference (a performance variant of the row
+ * assembly above). */
+int fmm3d_frechet_pinned(fmm3d_ctx *ctx, int nrc, const double *rcv, const int64_t **row_ptr,
+                         const int32_t **cols, const float **vals, int64_t *nnz);
+
 /* Copy resident field `src` to the host as float64 (nnr, nnx, nnz), phi
  * fastest (the reference's ttn after the final exchange, F:6439-6448). */
 int fmm3d_get_field(fmm3d_ctx *ctx, int src, double *ttn);

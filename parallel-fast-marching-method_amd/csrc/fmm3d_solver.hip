@@ -410,8 +410,10 @@ template <> __device__ __forceinline__ float fast_rcp<float>(float x) { return _
 // octant quadratic after the substitution T = (X + D_L)/c_L on its leading
 // axis (fm3dt_parallel.f90:535-1004), so the two agree up to rounding.
 // Octants through a missing side are skipped, octants with no counted side
-// give nothing, as in the reference.  Values arrive shifted by the smallest
-// counted neighbour.
+// give nothing, as in the reference: the first carry A = NaN, the second
+// A = B = 0 and so T = 0 * inf = NaN, and the running minimum is fmin,
+// which passes over a NaN.  Values arrive shifted by the smallest counted
+// neighbour.
 // One node update from the gathered stencil (sides r-, r+, x-, x+, z-, z+).
 // FAST selects the axis-sum octant form; otherwise the reference's
 // leading-axis form in the CPU oracle's operation order.
@@ -440,7 +442,7 @@ __device__ __forceinline__ R col_update(int fom, R tc, const R v1[6], const R v2
         for (int q = 0; q < 6; ++q) {
             // selects on plain values keep this branch-free
             const R t1 = v1[q] - sh, t2 = v2[q] - sh;
-            const R num = sec[q] ? R(2) * t1 - R(0.5) * t2 : t1;
+            const R num = sec[q] ? fma(R(-0.5), t2, t1 + t1) : t1;
             const R al = ih[q / 2] * (cnt[q] ? (sec[q] ? R(1.5) : R(1)) : R(0));
             const R be = cnt[q] ? num * ih[q / 2] : R(0);
             ta[q] = in1[q] ? al * al : nan;
@@ -460,9 +462,7 @@ __device__ __forceinline__ R col_update(int fom, R tc, const R v1[6], const R v2
                     const R A = ta[i] + Axz, B = tbb[i] + Bxz, C = tc2[i] + Cxz;
                     R disc = fused_mul_sub(B, B, A * C);
                     disc = disc > R(0) ? disc : R(0);
-                    R T = (B + fast_sqrt(disc)) * fast_rcp(A);
-                    T = A > R(0) ? T : inf;
-                    best = T < best ? T : best;
+                    best = fmin(best, (B + fast_sqrt(disc)) * fast_rcp(A));
                 }
             }
     } else {
@@ -1464,6 +1464,11 @@ struct fmm3d_ctx {
     int eng_k = 1, eng_order = FMM3D_ORDER_MIXED, eng_pending = 0;
     std::vector<int> eng_active, eng_born;
     int* eng_lc = nullptr;         // pinned host copy of the per-slot last-change sweep
+    int64_t* pin_rp = nullptr;     // fmm3d_frechet_pinned's rows: pinned host arrays
+    int32_t* pin_cols = nullptr;
+    float* pin_vals = nullptr;
+    size_t pin_rows = 0;           // entries pin_rp holds
+    int64_t pin_cap = 0;           // entries pin_cols / pin_vals hold
     double* d_faces = nullptr;     // boundary faces of the last teleseismic solve
     size_t d_faces_n = 0;
     int64_t* eng_node = nullptr;   // device staging of an admission's fixed set
@@ -1691,6 +1696,9 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->fine) { fmm3d_destroy(c->fine); c->fine = nullptr; }
     if (c->eng_lc) (void)hipHostFree(c->eng_lc);
+    if (c->pin_rp) (void)hipHostFree(c->pin_rp);
+    if (c->pin_cols) (void)hipHostFree(c->pin_cols);
+    if (c->pin_vals) (void)hipHostFree(c->pin_vals);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     (void)hipFree(c->d_iter);
@@ -2882,6 +2890,43 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "frechet: %s", hipGetErrorString(e));
     if (short_cap) return fail(c, FMM3D_E_CAPACITY, "fmm3d_frechet: capacity %lld < %lld entries", (long long)cap, (long long)row_ptr[nrays]);
     return FMM3D_OK;
+}
+
+// fmm3d_frechet into the context's own pinned host arrays: the copies from
+// the device then run at the link's rate straight into the result, with no
+// staging through bounce buffers.  The arrays grow as needed (sized from
+// the previous call's entries per row, one retry at the exact size).
+extern "C" int fmm3d_frechet_pinned(fmm3d_ctx* c, int nrc, const double* rcv, const int64_t** row_ptr,
+                                    const int32_t** cols, const float** vals, int64_t* nnz)
+{
+    if (!c || nrc < 0 || (nrc > 0 && !rcv) || !row_ptr || !cols || !vals || !nnz)
+        return fail(c, FMM3D_E_ARG, "fmm3d_frechet_pinned: bad arguments");
+    if (c->sl.n <= 0) return fail(c, FMM3D_E_STATE, "fmm3d_frechet_pinned: no resident solution");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t rows = (size_t)c->sl.n * nrc + 1;
+    if (c->pin_rows < rows) {
+        if (c->pin_rp) (void)hipHostFree(c->pin_rp);
+        c->pin_rp = nullptr; c->pin_rows = 0;
+        HIPCHK(c, hipHostMalloc((void**)&c->pin_rp, rows * sizeof(int64_t), hipHostMallocDefault));
+        c->pin_rows = rows;
+    }
+    int64_t want = std::max<int64_t>(c->pin_cap, (int64_t)(rows - 1) * 512 + 1024);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (c->pin_cap < want) {
+            if (c->pin_cols) (void)hipHostFree(c->pin_cols);
+            if (c->pin_vals) (void)hipHostFree(c->pin_vals);
+            c->pin_cols = nullptr; c->pin_vals = nullptr; c->pin_cap = 0;
+            HIPCHK(c, hipHostMalloc((void**)&c->pin_cols, (size_t)want * sizeof(int32_t), hipHostMallocDefault));
+            HIPCHK(c, hipHostMalloc((void**)&c->pin_vals, (size_t)want * sizeof(float), hipHostMallocDefault));
+            c->pin_cap = want;
+        }
+        const int rc = fmm3d_frechet(c, nrc, rcv, c->pin_rp, c->pin_cols, c->pin_vals, c->pin_cap);
+        if (rc == FMM3D_E_CAPACITY) { want = c->pin_rp[rows - 1]; continue; }
+        if (rc) return rc;
+        *row_ptr = c->pin_rp; *cols = c->pin_cols; *vals = c->pin_vals; *nnz = c->pin_rp[rows - 1];
+        return FMM3D_OK;
+    }
+    return fail(c, FMM3D_E_CAPACITY, "fmm3d_frechet_pinned: capacity retry failed");
 }
 
 // ------------------------------------------------------------ engine

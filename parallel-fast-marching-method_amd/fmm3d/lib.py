@@ -27,7 +27,7 @@ EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model
            "fmm3d_solve_fixed", "fmm3d_solve_point",
            "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_result",
            "fmm3d_server_stats", "fmm3d_server_destroy",
-           "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_frechet", "fmm3d_get_field",
+           "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_frechet", "fmm3d_frechet_pinned", "fmm3d_get_field",
            "fmm3d_get_stats", "fmm3d_set_option")
 
 
@@ -101,6 +101,9 @@ def load(path: Optional[str] = None):
     L.fmm3d_solve_point.argtypes = [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     L.fmm3d_receivers.argtypes = [vp, C.c_int, dp, fp]
     L.fmm3d_frechet.argtypes = [vp, C.c_int, dp, lp, ip, fp, C.c_int64]
+    L.fmm3d_frechet_pinned.argtypes = [vp, C.c_int, dp, C.POINTER(C.POINTER(C.c_int64)),
+                                       C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_float)),
+                                       C.POINTER(C.c_int64)]
     L.fmm3d_get_field.argtypes = [vp, C.c_int, dp]
     L.fmm3d_get_stats.argtypes = [vp, C.POINTER(StatsC)]
     L.fmm3d_set_option.argtypes = [vp, C.c_char_p, C.c_double]
@@ -195,15 +198,26 @@ class Context:
         self._check(self.L.fmm3d_receivers(self._h, len(rcv), rcv, out))
         return out.reshape(self.nsrc, len(rcv))
 
-    def frechet(self, rcv: np.ndarray, cap: int = 0) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    def frechet(self, rcv: np.ndarray, cap: int = 0, pinned: bool = False) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
         """(row_ptr int64 [nsrc*nrc+1], cols int32, vals float32).
 
         The arrays are sized from the previous call's entry count per row
         (rows of one model and receiver set change little between calls),
         retried once at the exact size when short; cols and vals are views
-        of the first row_ptr[-1] entries."""
+        of the first row_ptr[-1] entries.  pinned=True returns views of the
+        context's pinned host arrays (fmm3d_frechet_pinned), valid until the
+        next pinned call on this context."""
         rcv = np.ascontiguousarray(rcv, dtype=np.float64)
         nrows = self.nsrc * len(rcv)
+        if pinned:
+            prp = C.POINTER(C.c_int64)(); pc = C.POINTER(C.c_int32)(); pv = C.POINTER(C.c_float)()
+            n = C.c_int64()
+            self._check(self.L.fmm3d_frechet_pinned(self._h, len(rcv), rcv, C.byref(prp), C.byref(pc),
+                                                    C.byref(pv), C.byref(n)))
+            rp = np.ctypeslib.as_array(prp, shape=(nrows + 1,))
+            if n.value == 0:
+                return rp, np.zeros(0, np.int32), np.zeros(0, np.float32)
+            return rp, np.ctypeslib.as_array(pc, shape=(n.value,)), np.ctypeslib.as_array(pv, shape=(n.value,))
         if cap <= 0:
             per = getattr(self, "_fr_per_row", 512.0)
             cap = max(1024, int(nrows * per * 1.05) + 1024)

@@ -139,3 +139,16 @@ def test_edges_and_errors():
     rc = c.L.fmm3d_frechet(c._h, 3, np.ascontiguousarray(rcv), rp2, np.zeros(8, np.int32), np.zeros(8, np.float32), 8)
     assert rc == lib.E_CAPACITY and rp2[-1] == n
     c.close()
+
+
+def test_pinned_frechet_rows_equal_the_copied_ones():
+    """fmm3d_frechet_pinned returns the rows of fmm3d_frechet bit for bit;
+    a second call (fewer receivers) reuses the context's arrays."""
+    gd = G.load(G.names()[1])
+    with ctx_for(gd) as c:
+        c.solve_tele(gd.faces, gd.fom, lib.PREC_FP32)
+        for rcv in (gd.rcv, gd.rcv[:5]):
+            rp, cols, vals = c.frechet(rcv)
+            prp, pcols, pvals = c.frechet(rcv, pinned=True)
+            assert np.array_equal(rp, prp)
+            assert np.array_equal(cols, pcols) and np.array_equal(vals, pvals)
