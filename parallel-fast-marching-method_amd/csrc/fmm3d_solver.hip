@@ -1283,6 +1283,15 @@ k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, co
     bx[0] = bmin[0]; bx[1] = bmax[0]; bx[2] = bmin[1]; bx[3] = bmax[1]; bx[4] = bmin[2]; bx[5] = bmax[2];
 }
 
+// Weight n of the uniform cubic B-spline at fraction t: bspline4's w[n] in
+// the same arithmetic.
+__device__ __forceinline__ double bspline_w(double t, int n)
+{
+    const double s = 1.0 - t, t2 = t * t, t3 = t2 * t;
+    const double num = n == 0 ? s * s * s : (n == 1 ? 4.0 - 6.0 * t2 + 3.0 * t3 : (n == 2 ? 1.0 + 3.0 * t + 3.0 * t2 - 3.0 * t3 : t3));
+    return num / 6.0;
+}
+
 // Lane j's double as a wavefront-uniform value.
 __device__ __forceinline__ double lane_double(double v, int j)
 {
@@ -1294,7 +1303,10 @@ __device__ __forceinline__ double lane_double(double v, int j)
 
 // Frechet rows from recorded ray paths, one wavefront per ray (F:7600-7764).
 // The samples come in 64 at a time, one per lane, and are broadcast one
-// after another.
+// after another.  A lane evaluates only the three weights its tap uses
+// (bspline4's arithmetic, weight by weight); the previous sample's weight
+// on an axis is reused while the lane's vertex offset on that axis stays
+// the same, and otherwise taken from the previous sample's fraction.
 // Each lane owns the vertices whose (r, theta, phi) indices are congruent
 // to its (lane/16, lane/4 % 4, lane % 4) mod 4, so the 64 taps of every
 // sub-segment land on 64 different lanes and every vertex is only ever
@@ -1318,7 +1330,10 @@ k_ray_frechet(Grid g, int nrays, RayPaths P, float* __restrict__ scratch, RayOut
         const RaySample* S = P.rec + (int64_t)q * P.cap;
         int64_t acc_at = -1;   // scratch index of the vertex held in acc, -1 none
         float acc = 0.0f;
-        double ui[4] = { 0, 0, 0, 0 }, vi[4] = { 0, 0, 0, 0 }, wi[4] = { 0, 0, 0, 0 }, vel = 1.0;
+        double vel = 1.0;
+        double pt[3] = { 0.0, 0.0, 0.0 };   // previous sample's B-spline fractions (r, theta, phi)
+        double pw[3] = { 0.0, 0.0, 0.0 };   // its weights at offsets pn[], when pn[] >= 0
+        int pn[3] = { -1, -1, -1 };
         RaySample mine;   // this lane's sample of the current batch of 64
         mine.r = mine.x = mine.z = mine.vel = mine.dinc = 0.0;
         mine.ivr = mine.ivx = mine.ivz = mine.kind = 0;
@@ -1334,21 +1349,28 @@ k_ray_frechet(Grid g, int nrays, RayPaths P, float* __restrict__ scratch, RayOut
             sm.vel = lane_double(mine.vel, jb); sm.dinc = lane_double(mine.dinc, jb);
             sm.ivr = __builtin_amdgcn_readlane(mine.ivr, jb); sm.ivx = __builtin_amdgcn_readlane(mine.ivx, jb);
             sm.ivz = __builtin_amdgcn_readlane(mine.ivz, jb); sm.kind = __builtin_amdgcn_readlane(mine.kind, jb);
-            double uio[4], vio[4], wio[4];
-            for (int k = 0; k < 4; ++k) { uio[k] = ui[k]; vio[k] = vi[k]; wio[k] = wi[k]; }
             const double velo = vel;
             vel = sm.vel;
-            bspline4(((g.gor - sm.r) - (sm.ivr - 1) * g.dvr) / g.dvr, ui);
-            bspline4(((sm.x - g.gox) - (sm.ivx - 1) * g.dvx) / g.dvx, vi);
-            bspline4(((sm.z - g.goz) - (sm.ivz - 1) * g.dvz) / g.dvz, wi);
-            if (sm.kind != 1) continue;
+            const double tc[3] = { ((g.gor - sm.r) - (sm.ivr - 1) * g.dvr) / g.dvr,
+                                   ((sm.x - g.gox) - (sm.ivx - 1) * g.dvx) / g.dvx,
+                                   ((sm.z - g.goz) - (sm.ivz - 1) * g.dvz) / g.dvz };
+            if (sm.kind != 1) {
+                for (int a = 0; a < 3; ++a) { pt[a] = tc[a]; pn[a] = -1; }
+                continue;
+            }
             // this lane's tap: vertex (iv - 1 + n) with n the lane's residue offset
             const int rr = sm.ivr - 1 + (((own_r - (sm.ivr - 1)) % 4 + 4) % 4);
             const int xx = sm.ivx - 1 + (((own_x - (sm.ivx - 1)) % 4 + 4) % 4);
             const int zz = sm.ivz - 1 + (((own_z - (sm.ivz - 1)) % 4 + 4) % 4);
-            const int n = rr - (sm.ivr - 1), m = xx - (sm.ivx - 1), l = zz - (sm.ivz - 1);
-            const double e1 = ui[n] * vi[m] * wi[l] / (vel * vel);
-            const double e0 = uio[n] * vio[m] * wio[l] / (velo * velo);
+            const int off[3] = { rr - (sm.ivr - 1), xx - (sm.ivx - 1), zz - (sm.ivz - 1) };
+            double wc[3], wp[3];
+            for (int a = 0; a < 3; ++a) {
+                wc[a] = bspline_w(tc[a], off[a]);
+                wp[a] = pn[a] == off[a] ? pw[a] : bspline_w(pt[a], off[a]);
+            }
+            const double e1 = wc[0] * wc[1] * wc[2] / (vel * vel);
+            const double e0 = wp[0] * wp[1] * wp[2] / (velo * velo);
+            for (int a = 0; a < 3; ++a) { pt[a] = tc[a]; pw[a] = wc[a]; pn[a] = off[a]; }
             const double inc = -(e1 + e0) * sm.dinc / 2.0;
             if (zz >= 0 && zz < NVZ && xx >= 0 && xx < NVX && rr >= 0 && rr < NVR) {
                 const int64_t at = ((int64_t)zz * NVX + xx) * NVR + rr;
