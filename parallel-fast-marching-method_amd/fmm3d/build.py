@@ -15,6 +15,7 @@ HOST = os.path.join(ROOT, "host")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libfmm3d_hip.so")
 EXE = os.path.join(ROOT, "host", "fm3dt_gpu")
+INVERT_EXE = os.path.join(ROOT, "host", "fm3dt_invert")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
@@ -45,17 +46,20 @@ def build_hip(verbose: bool = False) -> str:
 
 
 def build_host() -> str:
-    """Fortran drop-in host (ISO_C_BINDING onto libfmm3d_hip.so)."""
-    src = os.path.join(HOST, "fm3dt_gpu.f90")
-    if not os.path.exists(src) or not os.path.exists(FLANG):
+    """Fortran hosts (ISO_C_BINDING onto libfmm3d_hip.so): the drop-in
+    fm3dt_gpu and the resident inversion loop fm3dt_invert, both on the
+    shared module file fm3dt_core.f90."""
+    core = os.path.join(HOST, "fm3dt_core.f90")
+    if not os.path.exists(core) or not os.path.exists(FLANG):
         return ""
     moddir = os.path.join(HOST, "_mod")
     os.makedirs(moddir, exist_ok=True)
-    cmd = [FLANG, "-O2", "-J", moddir, src, "-o", EXE, "-L" + PKG, "-lfmm3d_hip",
-           "-Wl,-rpath,$ORIGIN/../fmm3d", "-Wl,-rpath," + PKG]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
+    for main, exe in (("fm3dt_gpu.f90", EXE), ("fm3dt_invert.f90", INVERT_EXE)):
+        cmd = [FLANG, "-O2", "-J", moddir, core, os.path.join(HOST, main), "-o", exe, "-L" + PKG, "-lfmm3d_hip",
+               "-Wl,-rpath,$ORIGIN/../fmm3d", "-Wl,-rpath," + PKG]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("flang failed:\n" + r.stdout + r.stderr)
     return EXE
 
 

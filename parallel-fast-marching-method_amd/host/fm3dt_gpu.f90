@@ -8,242 +8,28 @@
 ! records (column, value) after a (count, receiver) header per row), as
 ! PROGRAM fmmin3d does (fm3dt_parallel.f90:6015-6662).  The solve, the
 ! receiver interpolation and the rays/Frechet rows run on one GPU through
-! the C ABI of libfmm3d_hip.so (include/fmm3d.h).
+! the C ABI of libfmm3d_hip.so (include/fmm3d.h); the file handling is in
+! fm3dt_core.f90.
 !
 ! Environment: FMM3D_DEVICE (default 0), FMM3D_PRECISION=fp32 for the
 ! float32 solver (default float64, the reference's arithmetic),
 ! FMM3D_BATCH (sources per device call, default 64).
 !-----------------------------------------------------------------------
-MODULE fmm3d_c
-   USE ISO_C_BINDING
-   IMPLICIT NONE
-   TYPE, BIND(C) :: fmm3d_grid_desc
-      INTEGER(C_INT) :: nvr, nvx, nvz
-      REAL(C_DOUBLE) :: gor, goxd, gozd, dvr, dvxd, dvzd
-      INTEGER(C_INT) :: nnr, nnx, nnz
-      REAL(C_DOUBLE) :: rgor, rgoxd, rgozd, dnr, dnxd, dnzd
-      REAL(C_DOUBLE) :: earth
-   END TYPE fmm3d_grid_desc
-   INTERFACE
-      INTEGER(C_INT) FUNCTION fmm3d_create(g, device, ctx) BIND(C, NAME='fmm3d_create')
-         IMPORT :: C_INT, C_PTR, fmm3d_grid_desc
-         TYPE(fmm3d_grid_desc), INTENT(IN) :: g
-         INTEGER(C_INT), VALUE :: device
-         TYPE(C_PTR) :: ctx
-      END FUNCTION
-      INTEGER(C_INT) FUNCTION fmm3d_destroy(ctx) BIND(C, NAME='fmm3d_destroy')
-         IMPORT :: C_INT, C_PTR
-         TYPE(C_PTR), VALUE :: ctx
-      END FUNCTION
-      TYPE(C_PTR) FUNCTION fmm3d_last_error(ctx) BIND(C, NAME='fmm3d_last_error')
-         IMPORT :: C_PTR
-         TYPE(C_PTR), VALUE :: ctx
-      END FUNCTION
-      INTEGER(C_INT) FUNCTION fmm3d_set_model_vtx(ctx, velv) BIND(C, NAME='fmm3d_set_model_vtx')
-         IMPORT :: C_INT, C_PTR, C_DOUBLE
-         TYPE(C_PTR), VALUE :: ctx
-         REAL(C_DOUBLE), INTENT(IN) :: velv(*)
-      END FUNCTION
-      INTEGER(C_INT) FUNCTION fmm3d_solve_tele(ctx, nsrc, faces, order, prec, flags) BIND(C, NAME='fmm3d_solve_tele')
-         IMPORT :: C_INT, C_PTR, C_DOUBLE
-         TYPE(C_PTR), VALUE :: ctx
-         INTEGER(C_INT), VALUE :: nsrc, order, prec, flags
-         REAL(C_DOUBLE), INTENT(IN) :: faces(*)
-      END FUNCTION
-      INTEGER(C_INT) FUNCTION fmm3d_receivers(ctx, nrc, rcv, t_out) BIND(C, NAME='fmm3d_receivers')
-         IMPORT :: C_INT, C_PTR, C_DOUBLE, C_FLOAT
-         TYPE(C_PTR), VALUE :: ctx
-         INTEGER(C_INT), VALUE :: nrc
-         REAL(C_DOUBLE), INTENT(IN) :: rcv(*)
-         REAL(C_FLOAT) :: t_out(*)
-      END FUNCTION
-      INTEGER(C_INT) FUNCTION fmm3d_frechet(ctx, nrc, rcv, row_ptr, cols, vals, cap) BIND(C, NAME='fmm3d_frechet')
-         IMPORT :: C_INT, C_PTR, C_DOUBLE, C_FLOAT, C_INT32_T, C_INT64_T
-         TYPE(C_PTR), VALUE :: ctx
-         INTEGER(C_INT), VALUE :: nrc
-         REAL(C_DOUBLE), INTENT(IN) :: rcv(*)
-         INTEGER(C_INT64_T) :: row_ptr(*)
-         INTEGER(C_INT32_T) :: cols(*)
-         REAL(C_FLOAT) :: vals(*)
-         INTEGER(C_INT64_T), VALUE :: cap
-      END FUNCTION
-      INTEGER(C_SIZE_T) FUNCTION c_strlen(s) BIND(C, NAME='strlen')
-         IMPORT :: C_SIZE_T, C_PTR
-         TYPE(C_PTR), VALUE :: s
-      END FUNCTION
-   END INTERFACE
-CONTAINS
-   SUBROUTINE check(rc, ctx, what)
-      INTEGER(C_INT), INTENT(IN) :: rc
-      TYPE(C_PTR), INTENT(IN) :: ctx
-      CHARACTER(LEN=*), INTENT(IN) :: what
-      TYPE(C_PTR) :: msg
-      CHARACTER(KIND=C_CHAR), POINTER :: chars(:)
-      INTEGER :: n, i
-      CHARACTER(LEN=512) :: text
-      IF (rc == 0) RETURN
-      msg = fmm3d_last_error(ctx)
-      n = INT(c_strlen(msg))
-      CALL C_F_POINTER(msg, chars, [n])
-      text = ' '
-      DO i = 1, MIN(n, 512)
-         text(i:i) = chars(i)
-      END DO
-      WRITE(6,*) 'ERROR in ', what, ' (status ', rc, '): ', TRIM(text)
-      STOP 1
-   END SUBROUTINE check
-END MODULE fmm3d_c
-
 PROGRAM fm3dt_gpu
    USE ISO_C_BINDING
    USE fmm3d_c
+   USE fm3dt_run
    IMPLICIT NONE
-   REAL(KIND=C_DOUBLE), PARAMETER :: pi = 3.1415926535898   ! default-real literal, as the reference
-   CHARACTER(LEN=20) :: sources, itimes, receivers, gridv, rtravel, frechet, cdum
-   CHARACTER(LEN=8) :: pht
-   CHARACTER(LEN=32) :: env
-   INTEGER :: fom, ltfr, fsrt, cfd, nra, idum, nsrc, nrc, i, j, k, ii, s0, nb, ios, batch, dev, prec
-   INTEGER(C_INT64_T) :: cap, r, nrows
-   REAL(C_DOUBLE) :: earth, cslat, cslong, la, lo, dp
+   TYPE(run_config) :: cfg
    TYPE(fmm3d_grid_desc) :: g
    TYPE(C_PTR) :: ctx
-   REAL(C_DOUBLE), ALLOCATABLE :: velv(:), faces(:), rcv(:)
-   REAL(C_FLOAT), ALLOCATABLE :: tt(:), vals(:)
-   INTEGER(C_INT64_T), ALLOCATABLE :: rp(:)
-   INTEGER(C_INT32_T), ALLOCATABLE :: cols(:)
-   INTEGER(8) :: face_rec
-   INTEGER :: itu
+   REAL(C_DOUBLE), ALLOCATABLE :: velv(:)
 
-   ! run configuration (fm3dt_parallel.f90:6082-6102)
-   OPEN(UNIT=10, FILE='fm3dt_parallel.in', STATUS='old')
-   READ(10, '(a20)') cdum
-   READ(10, '(a20)') cdum
-   READ(10, '(a20)') cdum
-   READ(10, '(a20)') sources
-   READ(10, '(a20)') itimes
-   READ(10, '(a20)') receivers
-   READ(10, '(a20)') gridv
-   READ(10, *) earth
-   READ(10, *) fom
-   READ(10, *) ltfr
-   READ(10, *) cslat, cslong
-   READ(10, '(a20)') cdum
-   READ(10, '(a20)') cdum
-   READ(10, '(a20)') cdum
-   READ(10, *) fsrt
-   READ(10, '(a20)') rtravel
-   READ(10, *) cfd
-   READ(10, '(a20)') frechet
-   CLOSE(10)
-
-   ! model (fm3dt_parallel.f90:6103-6126): header verbatim, values r fastest
-   OPEN(UNIT=10, FILE=gridv, STATUS='old')
-   READ(10, *) g%nvr, g%nvx, g%nvz
-   READ(10, *) g%gor, g%goxd, g%gozd
-   READ(10, *) g%dvr, g%dvxd, g%dvzd
-   READ(10, *) g%nnr, g%nnx, g%nnz
-   READ(10, *) g%rgor, g%rgoxd, g%rgozd
-   READ(10, *) g%dnr, g%dnxd, g%dnzd
-   g%earth = earth
-   ALLOCATE(velv((g%nvr + 2) * (g%nvx + 2) * (g%nvz + 2)))
-   DO i = 1, SIZE(velv)
-      READ(10, *) velv(i)
-   END DO
-   CLOSE(10)
-
-   dev = 0
-   CALL GET_ENVIRONMENT_VARIABLE('FMM3D_DEVICE', env, STATUS=ios)
-   IF (ios == 0) READ(env, *) dev
-   batch = 64
-   CALL GET_ENVIRONMENT_VARIABLE('FMM3D_BATCH', env, STATUS=ios)
-   IF (ios == 0) READ(env, *) batch
-   prec = 0
-   CALL GET_ENVIRONMENT_VARIABLE('FMM3D_PRECISION', env, STATUS=ios)
-   IF (ios == 0 .AND. TRIM(env) == 'fp32') prec = 1
-
+   CALL read_config(cfg)
+   CALL read_model(cfg, g, velv)
    ctx = C_NULL_PTR
-   CALL check(fmm3d_create(g, INT(dev, C_INT), ctx), C_NULL_PTR, 'fmm3d_create')
+   CALL check(fmm3d_create(g, INT(cfg%dev, C_INT), ctx), C_NULL_PTR, 'fmm3d_create')
    CALL check(fmm3d_set_model_vtx(ctx, velv), ctx, 'fmm3d_set_model_vtx')
-
-   OPEN(UNIT=60, FILE=sources, STATUS='old')
-   READ(60, *) nra
-   IF (fsrt == 1) THEN
-      OPEN(UNIT=70, FILE=receivers, STATUS='old')
-      READ(70, *) idum
-      IF (idum /= nra) THEN
-         WRITE(6, *) 'ERROR!!!'
-         WRITE(6, *) 'Source and receiver files are'
-         WRITE(6, *) 'inconsistent!!!!'
-         WRITE(6, *) 'First line of each file should'
-         WRITE(6, *) 'be identical!!!'
-         WRITE(6, *) 'TERMINATING PROGRAM!!!'
-         STOP
-      END IF
-   END IF
-   IF (cfd == 1) OPEN(UNIT=50, FILE=frechet, FORM='unformatted', STATUS='replace')
-   OPEN(NEWUNIT=itu, FILE=itimes, ACCESS='stream', FORM='unformatted', STATUS='old')
-   face_rec = 0
-
-   DO ii = 1, nra
-      READ(60, *) nsrc
-      DO i = 1, nsrc                      ! source positions are read, not used (F:6275-6285)
-         READ(60, *) la, lo, dp
-         READ(60, '(a8)') pht
-      END DO
-      READ(70, *) nrc
-      ALLOCATE(rcv(3 * nrc))
-      DO i = 1, nrc                       ! (height, colat, lon) as F:6293-6312
-         READ(70, *) rcv(3 * i - 2), la, lo
-         rcv(3 * i - 1) = (90.0 - la) * pi / 180.0
-         rcv(3 * i) = lo * pi / 180.0
-      END DO
-      IF (ii == 1) THEN
-         OPEN(UNIT=20, FILE=rtravel, STATUS='replace')
-      ELSE
-         OPEN(UNIT=20, FILE=rtravel, POSITION='append')
-      END IF
-      DO s0 = 1, nsrc, batch
-         nb = MIN(batch, nsrc - s0 + 1)
-         ALLOCATE(faces(INT(g%nnx, 8) * g%nnz * nb), tt(nrc * nb))
-         READ(itu, POS=1 + face_rec * 8 * INT(g%nnx, 8) * g%nnz) faces
-         face_rec = face_rec + nb
-         CALL check(fmm3d_solve_tele(ctx, INT(nb, C_INT), faces, INT(fom, C_INT), INT(prec, C_INT), 0_C_INT), &
-                    ctx, 'fmm3d_solve_tele')
-         IF (fsrt == 1) THEN
-            CALL check(fmm3d_receivers(ctx, INT(nrc, C_INT), rcv, tt), ctx, 'fmm3d_receivers')
-            DO i = 1, nrc * nb
-               WRITE(20, *) tt(i)
-            END DO
-         END IF
-         IF (cfd == 1) THEN
-            nrows = INT(nrc, 8) * nb
-            ALLOCATE(rp(nrows + 1))
-            cap = nrows * 512
-            DO k = 1, 2
-               ALLOCATE(cols(MAX(cap, 1_8)), vals(MAX(cap, 1_8)))
-               ios = fmm3d_frechet(ctx, INT(nrc, C_INT), rcv, rp, cols, vals, cap)
-               IF (ios /= -6) EXIT
-               cap = rp(nrows + 1)
-               DEALLOCATE(cols, vals)
-            END DO
-            CALL check(INT(ios, C_INT), ctx, 'fmm3d_frechet')
-            DO r = 1, nrows
-               j = INT(MOD(r - 1, INT(nrc, 8))) + 1
-               WRITE(50) INT(rp(r + 1) - rp(r), 4), REAL(j, 4)
-               DO i = INT(rp(r)) + 1, INT(rp(r + 1))
-                  WRITE(50) cols(i), vals(i)
-               END DO
-            END DO
-            DEALLOCATE(rp, cols, vals)
-         END IF
-         DEALLOCATE(faces, tt)
-      END DO
-      CLOSE(20)
-      DEALLOCATE(rcv)
-   END DO
-   CLOSE(60)
-   IF (fsrt == 1) CLOSE(70)
-   IF (cfd == 1) CLOSE(50)
-   CLOSE(itu)
+   CALL fmm_pass(cfg, ctx, g)
    CALL check(fmm3d_destroy(ctx), C_NULL_PTR, 'fmm3d_destroy')
 END PROGRAM fm3dt_gpu

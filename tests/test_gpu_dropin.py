@@ -35,6 +35,10 @@ def test_dropin_writes_the_reference_file_contract(tmp_path):
     for k, (rid, rc, rv) in enumerate(rows):
         assert rid == k % 12 + 1
         assert np.array_equal(rc, cols[rp[k]:rp[k + 1]]) and np.array_equal(rv, vals[rp[k]:rp[k + 1]])
+    # byte for byte the file sequential unformatted writes give
+    ref = tmp_path / "frechet_ref.out"
+    io.write_frechet(str(ref), [(k % 12 + 1, cols[rp[k]:rp[k + 1]], vals[rp[k]:rp[k + 1]]) for k in range(len(rows))])
+    assert (tmp_path / "frechet.out").read_bytes() == ref.read_bytes()
 
 
 def test_dropin_rejects_inconsistent_array_counts(tmp_path):
