@@ -3,16 +3,19 @@
 per second at 201^3, batched float32 against the plain float64 GPU solver,
 with every source-solve carrying its receiver times and Frechet rows.
 
-A step is one batch: `--batch` plane-wave sources (bottom-face injection)
-on the 201^3 grid (0-600 km, 15x15 deg, 25^3 B-spline vertices, +-3 %),
-solved, then 300 receiver times and 300 rays with Frechet rows per source.
-The model is resident; boundary faces go in and rows come out through the
-C ABI each step (the PCIe share is small and is included).
+A step is one batch: `--batch` plane-wave sources on the 201^3 grid (0-600
+km, 15x15 deg, 25^3 B-spline vertices, +-3 %), each injected on the bottom
+face and on the two side faces it enters through with 1-D (tau-p) boundary
+times, solved, then 300 receiver times and 300 rays with Frechet rows per
+source.  The model is resident; boundary faces go in and rows come out
+through the C ABI each step (the PCIe share is small and is included).
 
 Prints one JSON line.  Extra fields: the plain float64 solver's throughput
 on the same sources (one source per call), the float32 error against float64
 at the receivers, per-step latency percentiles, the relaxation kernel's
-roofline figure and a CPU baseline (the oracle's heap march, one core).
+roofline figure, the same figures with bottom-face injection only (the
+reference's own boundary condition), the serving loop, and a CPU baseline
+(the oracle's heap march from the same fixed nodes, one core).
 """
 from __future__ import annotations
 
@@ -73,8 +76,35 @@ def copy_calibration(mib):
     del x, y
 
 
-def one_step(ctx, faces, rcv, prec):
-    ctx.solve_tele(faces, lib.ORDER_MIXED, prec)
+class Workload:
+    """Boundary data of the batch: bottom faces, and with sides the theta and
+    phi side faces the waves enter through (fmm3d_solve_tele_sides)."""
+
+    def __init__(self, g, srcs, sides):
+        parts = [synth.tele_faces(g, s) for s in srcs]
+        self.faces = np.stack([q[0] for q in parts])
+        self.sides = sides
+        if sides:
+            self.xf = np.stack([q[1] for q in parts])
+            self.zf = np.stack([q[2] for q in parts])
+            self.side = np.stack([q[3] for q in parts])
+
+    def take(self, lo, hi):
+        w = Workload.__new__(Workload)
+        w.faces, w.sides = self.faces[lo:hi], self.sides
+        if self.sides:
+            w.xf, w.zf, w.side = self.xf[lo:hi], self.zf[lo:hi], self.side[lo:hi]
+        return w
+
+    def solve(self, ctx, prec):
+        if self.sides:
+            ctx.solve_tele_sides(self.faces, self.xf, self.zf, self.side, lib.ORDER_MIXED, prec)
+        else:
+            ctx.solve_tele(self.faces, lib.ORDER_MIXED, prec)
+
+
+def one_step(ctx, work, rcv, prec):
+    work.solve(ctx, prec)
     st = ctx.stats()
     t = ctx.receivers(rcv)
     rp, cols, vals = ctx.frechet(rcv, pinned=True)
@@ -83,9 +113,10 @@ def one_step(ctx, faces, rcv, prec):
 
 def serving(g, velv, faces, rcv, seconds, t_src):
     """The serving loop under Poisson arrivals: requests of 4 float32 plane-wave
-    sources (receiver times only) on the resident 201^3 model, offered at 70 %
-    of the batched throughput; p50/p99 request latency and the throughput the
-    server sustains.  Results are checked against the batched solve."""
+    sources (bottom faces, receiver times only) on the resident 201^3 model,
+    offered at 70 % of the batched bottom-face throughput; p50/p99 request
+    latency and the throughput the server sustains.  Results are checked
+    against the batched solve of the same sources."""
     rng = np.random.default_rng(5)
     per = 4
     rate = 0.7 * (1.0 / t_src) / per              # requests per second
@@ -120,19 +151,28 @@ def serving(g, velv, faces, rcv, seconds, t_src):
             "source_solves_per_s": round(n * per / span, 2), "server_hbm_gb": round(st["hbm_bytes"] / 1e9, 2)}, tickets, errs
 
 
-def cpu_baseline(g, velv, faces, rcv, n):
-    """The oracle's heap march (global mode) + receivers + rays/Frechet, one
-    core, on `n` sources of the same workload."""
+def cpu_baseline(g, velv, srcs, rcv, n):
+    """The oracle's heap march (global mode) from the same fixed nodes
+    (bottom and entry side faces) + receivers + rays/Frechet, one core, on
+    `n` sources of the same workload."""
     from oracle import oracle as O
     veln = O.gridder(g, velv)
+    sets = []
+    for s in srcs[:n]:
+        node, tv = synth.tele_boundary_nodes(g, s)
+        fixed = np.zeros(g.nnr * g.nnx * g.nnz, np.uint8)
+        tfix = np.zeros(fixed.shape)
+        fixed[node] = 1
+        tfix[node] = tv
+        sets.append((fixed.reshape(g.nnr, g.nnx, g.nnz), tfix.reshape(g.nnr, g.nnx, g.nnz)))
     t0 = time.perf_counter()
-    for s in range(n):
-        T = O.fmm_tele(g, veln, faces[s], 1)
+    for fixed, tfix in sets:
+        T = O.fmm_fixed(g, veln, fixed, tfix, 1)
         O.srtimes(g, T, rcv)
         O.rpaths(g, veln, T, rcv)
     dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "source-solves/s", "cores": 1, "kind": "port",
-            "sample": f"{n} source-solve(s) of the 201^3 workload (heap march + {len(rcv)} receivers + rays/Frechet), "
+            "sample": f"{n} source-solve(s) of the 201^3 workload (heap march from bottom + side faces + {len(rcv)} receivers + rays/Frechet), "
                       f"{dt:.1f} s on one host core"}
 
 
@@ -146,45 +186,47 @@ def main():
         copy_calibration(a.copy_mib)
     g, velv, srcs, rcv_deg = synth.config_201(nsrc=a.batch, nrc=a.nrc, seed=2)
     rcv = synth.receivers_rad(rcv_deg)
-    faces = np.stack([synth.tele_bottom_face(g, s) for s in srcs])
     ctx = lib.Context(g)
     ctx.set_model(velv)
     ctx.set_option("timing", 1)
 
-    # accuracy of the batched float32 path against float64 (outside the timed region)
-    t64, _, _ = one_step(ctx, faces, rcv, lib.PREC_FP64)
-    t32, _, _ = one_step(ctx, faces, rcv, lib.PREC_FP32)
-    err = float(np.nanmax(np.abs(t32.astype(np.float64) - t64)))
+    def measure(work, steps, warmup, plain_n):
+        """float32 error against float64, the timed float32 batches, and the
+        plain float64 solver (one source per call) on the same sources"""
+        t64, _, _ = one_step(ctx, work, rcv, lib.PREC_FP64)
+        t32, _, _ = one_step(ctx, work, rcv, lib.PREC_FP32)
+        r = {"t32": t32, "err": float(np.nanmax(np.abs(t32.astype(np.float64) - t64))), "lat": [],
+             "upd": 0, "nvis": 0, "kms": 0.0, "launches": 0, "nnz": 0}
+        for _ in range(warmup):
+            one_step(ctx, work, rcv, lib.PREC_FP32)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s0 = time.perf_counter()
+            _, st, n = one_step(ctx, work, rcv, lib.PREC_FP32)
+            r["lat"].append(1e3 * (time.perf_counter() - s0))
+            r["upd"] += st["node_updates"]
+            r["nvis"] += st["node_visits"]
+            r["kms"] += st["kernel_ms"]
+            r["launches"] += st["kernel_launches"]
+            r["nnz"] += n
+        r["elapsed"] = time.perf_counter() - t0
+        r["value"] = a.batch * steps / r["elapsed"]
+        plain_t = []
+        for s in range(plain_n):
+            s0 = time.perf_counter()
+            one_step(ctx, work.take(s, s + 1), rcv, lib.PREC_FP64)
+            plain_t.append(time.perf_counter() - s0)
+        r["plain"] = 1.0 / float(np.mean(plain_t)) if plain_t else None
+        return r
 
-    # batched float32: the measured configuration
-    for _ in range(a.warmup):
-        one_step(ctx, faces, rcv, lib.PREC_FP32)
-    lat = []
-    upd = 0
-    nvis = 0
-    kms = 0.0
-    launches = 0
-    nnz = 0
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        s0 = time.perf_counter()
-        _, st, n = one_step(ctx, faces, rcv, lib.PREC_FP32)
-        lat.append(1e3 * (time.perf_counter() - s0))
-        upd += st["node_updates"]
-        nvis += st["node_visits"]
-        kms += st["kernel_ms"]
-        launches += st["kernel_launches"]
-        nnz += n
-    elapsed = time.perf_counter() - t0
-    value = a.batch * a.steps / elapsed
-
-    # plain float64 GPU solver, one source per call
-    plain_t = []
-    for s in range(a.plain):
-        s0 = time.perf_counter()
-        one_step(ctx, faces[s:s + 1], rcv, lib.PREC_FP64)
-        plain_t.append(time.perf_counter() - s0)
-    plain = 1.0 / float(np.mean(plain_t)) if plain_t else None
+    # the measured configuration: bottom + entry side faces
+    side = measure(Workload(g, srcs, True), a.steps, a.warmup, a.plain)
+    # the reference's own boundary condition: bottom faces only
+    bottom_work = Workload(g, srcs, False)
+    bottom = measure(bottom_work, a.steps, a.warmup, a.plain)
+    value, elapsed, err, lat, plain = side["value"], side["elapsed"], side["err"], side["lat"], side["plain"]
+    upd, nvis, kms, launches, nnz = side["upd"], side["nvis"], side["kms"], side["launches"], side["nnz"]
+    faces = bottom_work.faces
 
     avg_launch_ms = kms / max(launches, 1)
     bytes_per_launch = BYTES_PER_NODE_VISIT * nvis / max(launches, 1)
@@ -203,7 +245,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"teleseismic 201^3 nodes (0-600 km, 15x15 deg), 25^3 B-spline vertices +-3%, "
-                               f"{a.batch} plane-wave sources per batch (bottom injection), {a.nrc} receivers, "
+                               f"{a.batch} plane-wave sources per batch injected on the bottom and entry side faces "
+                               f"(1-D tau-p times), {a.nrc} receivers, "
                                f"mixed-order stencils, receiver times + rays/Frechet rows",
                    "grid": [g.nnr, g.nnx, g.nnz], "batch": a.batch, "receivers": a.nrc},
         "plain_fp64_source_solves_per_s": round(plain, 3) if plain else None,
@@ -220,18 +263,23 @@ def main():
                      "node_visits_per_launch": int(nvis / max(launches, 1)),
                      "algorithmic_bytes_per_node_visit": BYTES_PER_NODE_VISIT,
                      "node_updates_per_s": round(upd / (kms * 1e-3), 1) if kms > 0 else None},
+        "bottom_faces_only": {"value": round(bottom["value"], 3),
+                              "ms_per_step": round(1e3 * bottom["elapsed"] / a.steps, 2),
+                              "plain_fp64_source_solves_per_s": round(bottom["plain"], 3) if bottom["plain"] else None,
+                              "ratio_vs_plain_fp64": round(bottom["value"] / bottom["plain"], 2) if bottom["plain"] else None,
+                              "max_receiver_err_s_vs_fp64": bottom["err"]},
     }
     ctx.close()
     if a.serve_s > 0:
-        sv, tk, outs = serving(g, velv, faces, rcv, a.serve_s, elapsed / a.steps / a.batch * 1.0)
+        sv, tk, outs = serving(g, velv, faces, rcv, a.serve_s, bottom["elapsed"] / a.steps / a.batch)
         # every served request against the batched float32 solve of its sources
         err = 0.0
         for (t, idx), o in zip(tk, outs):
-            err = max(err, float(np.nanmax(np.abs(o.astype(np.float64) - t32[idx]))))
+            err = max(err, float(np.nanmax(np.abs(o.astype(np.float64) - bottom["t32"][idx]))))
         sv["max_abs_diff_vs_batch_s"] = err
         out["serving"] = sv
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(g, velv, faces, rcv, a.cpu_sources)
+        out["cpu_baseline"] = cpu_baseline(g, velv, srcs, rcv, a.cpu_sources)
     print(json.dumps(out), flush=True)
 
 

@@ -100,6 +100,20 @@ int fmm3d_set_model_vtx(fmm3d_ctx *ctx, const double *velv);
 int fmm3d_solve_tele(fmm3d_ctx *ctx, int nsrc, const double *faces, int order,
                      int precision, int flags);
 
+/* Teleseismic solves with the plane wave injected on the bottom face and on
+ * one theta and one phi side face per source (the faces it enters through),
+ * all from 1-D boundary times.  No counterpart in the reference, which
+ * reads bottom faces only (itimes.out, fm3dt_parallel.f90:6894-6961); the
+ * discrete system and the stop rule are those of fmm3d_solve_tele.
+ *   faces: nsrc bottom faces, nnx*nnz float64, phi fastest (itimes.out);
+ *   xface: nsrc theta faces, nnr*nnz float64 (ir = 0 at the top), phi fastest;
+ *   zface: nsrc phi faces, nnr*nnx float64, theta fastest;
+ *   side:  2*nsrc ints, per source (theta side, phi side): -1 none, 0 the
+ *          face at index 0, 1 the face at index nnx-1 (nnz-1).
+ * Values > 0 are fixed, the rest unknown, on every face.  flags: 0. */
+int fmm3d_solve_tele_sides(fmm3d_ctx *ctx, int nsrc, const double *faces, const double *xface,
+                           const double *zface, const int *side, int order, int precision, int flags);
+
 /* Solves from general fixed (Dirichlet) node sets, one per source: the
  * generalisation of the bottom-face injection above to any nodes, e.g.
  * bottom and side faces of a teleseismic plane wave or the refined

@@ -1493,6 +1493,9 @@ struct fmm3d_ctx {
     int64_t pin_cap = 0;           // entries pin_cols / pin_vals hold
     double* d_faces = nullptr;     // boundary faces of the last teleseismic solve
     size_t d_faces_n = 0;
+    double* d_sides = nullptr;     // side faces of the last teleseismic solve: theta faces, then phi faces
+    int* d_side_of = nullptr;      // per source: theta side, phi side (-1 none, 0 low index, 1 high)
+    size_t d_sides_n = 0, d_side_of_n = 0;
     int64_t* eng_node = nullptr;   // device staging of an admission's fixed set
     double* eng_tv = nullptr;
     size_t eng_stage = 0;
@@ -1725,6 +1728,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     if (c->graph) (void)hipGraphDestroy(c->graph);
     (void)hipFree(c->d_iter);
     (void)hipFree(c->d_faces);
+    (void)hipFree(c->d_sides); (void)hipFree(c->d_side_of);
     (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv);
     free_slots(c->sl);
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
@@ -1846,6 +1850,42 @@ __global__ void k_fixbits_tele(int nnr, int nnx, int nnz, int nslots, const doub
             if (faces[s * NL + c % NL] > 0.0) bits |= 1u << b;
         }
         fix[w] = bits;
+    }
+}
+
+// Side-face injection after the bottom face: values > 0 on a slot's theta
+// face (ix = 0 or nnx-1, every ir and iz; (nnr, nnz) phi fastest) and phi
+// face (iz = 0 or nnz-1, every ir and ix; (nnr, nnx) theta fastest) become
+// fixed nodes, like the bottom face's (float32: minus the slot's offset).  A
+// side of -1 injects nothing.  Bits of one word come from several threads,
+// hence atomicOr.
+template <typename R>
+__global__ void k_init_sides(int nnr, int nnx, int nnz, int nslots, const double* __restrict__ xf,
+                             const double* __restrict__ zf, const int* __restrict__ side,
+                             const double* __restrict__ toff, R* __restrict__ T, uint32_t* __restrict__ fix)
+{
+    const size_t N = (size_t)nnr * nnx * nnz, NW = (N + 31) / 32;
+    const size_t XF = (size_t)nnr * nnz, ZF = (size_t)nnr * nnx, per = XF + ZF;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < per * nslots; e += (size_t)gridDim.x * blockDim.x) {
+        const size_t s = e / per, k = e % per;
+        size_t ir, ix, iz;
+        double v;
+        if (k < XF) {
+            const int xs = side[2 * s];
+            if (xs < 0) continue;
+            ir = k / nnz; iz = k % nnz; ix = xs ? nnx - 1 : 0;
+            v = xf[s * XF + k];
+        } else {
+            const size_t k2 = k - XF;
+            const int zs = side[2 * s + 1];
+            if (zs < 0) continue;
+            ir = k2 / nnx; ix = k2 % nnx; iz = zs ? nnz - 1 : 0;
+            v = zf[s * ZF + k2];
+        }
+        if (!(v > 0.0)) continue;
+        const size_t c = (ir * nnx + ix) * nnz + iz;
+        T[s * N + c] = std::is_same<R, double>::value ? (R)v : (R)(v - toff[s]);
+        atomicOr(fix + s * NW + (c >> 5), 1u << (c & 31));
     }
 }
 
@@ -2425,30 +2465,49 @@ static void lateral_flow(const Grid& g, const double* face, int out[2])
     flow_code(gx, gz, out);
 }
 
-extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int order, int precision, int flags)
+// Smallest positive value of n doubles and whether any is NaN (a
+// branch-free, vectorisable scan).
+static double positive_min(const double* f, size_t n, int* nan)
 {
-    if (!c) return fail(c, FMM3D_E_ARG, "fmm3d_solve_tele: null context");
-    if (nsrc <= 0 || !faces) return fail(c, FMM3D_E_ARG, "fmm3d_solve_tele: need nsrc > 0 and faces");
-    if (order != FMM3D_ORDER_FIRST && order != FMM3D_ORDER_MIXED) return fail(c, FMM3D_E_ARG, "fmm3d_solve_tele: bad order %d", order);
-    if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "fmm3d_solve_tele: bad precision %d", precision);
-    if (flags != 0) return fail(c, FMM3D_E_ARG, "fmm3d_solve_tele: flags must be 0");
-    if (!c->have_model) return fail(c, FMM3D_E_STATE, "fmm3d_solve_tele: no velocity model (call fmm3d_set_model_vtx)");
+    double m = __builtin_huge_val();
+    int bad = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const double v = f[i];
+        bad |= (v != v);
+        m = (v > 0.0 && v < m) ? v : m;
+    }
+    *nan |= bad;
+    return m;
+}
+
+// Teleseismic solves from bottom faces and, when xf is given, one theta
+// and one phi side face per source (side[2s], side[2s+1]: -1 none, 0 the
+// low-index face, 1 the high-index one).
+static int solve_tele_impl(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, const double* xf,
+                           const double* zf, const int* side, int order, int precision, int flags)
+{
+    if (!c) return fail(c, FMM3D_E_ARG, "%s: null context", fn);
+    if (nsrc <= 0 || !faces) return fail(c, FMM3D_E_ARG, "%s: need nsrc > 0 and faces", fn);
+    if (xf && (!zf || !side)) return fail(c, FMM3D_E_ARG, "%s: need both side faces and the side indices", fn);
+    if (order != FMM3D_ORDER_FIRST && order != FMM3D_ORDER_MIXED) return fail(c, FMM3D_E_ARG, "%s: bad order %d", fn, order);
+    if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "%s: bad precision %d", fn, precision);
+    if (flags != 0) return fail(c, FMM3D_E_ARG, "%s: flags must be 0", fn);
+    if (!c->have_model) return fail(c, FMM3D_E_STATE, "%s: no velocity model (call fmm3d_set_model_vtx)", fn);
     HIPCHK(c, hipSetDevice(c->device));
     const Grid& g = c->g;
-    const size_t F = (size_t)g.nnx * g.nnz;
+    const size_t F = (size_t)g.nnx * g.nnz, XF = (size_t)g.nnr * g.nnz, ZF = (size_t)g.nnr * g.nnx;
     // boundary minimum per source: the float32 offset (and a sanity check)
     std::vector<double> toff(nsrc, 0.0);
     for (int s = 0; s < nsrc; ++s) {
-        // branch-free scan (vectorisable): smallest positive value, and any NaN
-        const double* f = faces + (size_t)s * F;
-        double m = __builtin_huge_val();
         int nan = 0;
-        for (size_t i = 0; i < F; ++i) {
-            const double v = f[i];
-            nan |= (v != v);
-            m = (v > 0.0 && v < m) ? v : m;
+        double m = positive_min(faces + (size_t)s * F, F, &nan);
+        if (xf) {
+            if (side[2 * s] < -1 || side[2 * s] > 1 || side[2 * s + 1] < -1 || side[2 * s + 1] > 1)
+                return fail(c, FMM3D_E_ARG, "%s: side indices of source %d must be -1, 0 or 1", fn, s);
+            if (side[2 * s] >= 0) m = std::min(m, positive_min(xf + (size_t)s * XF, XF, &nan));
+            if (side[2 * s + 1] >= 0) m = std::min(m, positive_min(zf + (size_t)s * ZF, ZF, &nan));
         }
-        if (nan) return fail(c, FMM3D_E_ARG, "fmm3d_solve_tele: NaN boundary time (source %d)", s);
+        if (nan) return fail(c, FMM3D_E_ARG, "%s: NaN boundary time (source %d)", fn, s);
         toff[s] = isfinite(m) ? m : 0.0;
     }
     int rc = ensure_slots(c, nsrc, precision);
@@ -2471,6 +2530,32 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     else
         k_init_tele<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff);
     k_fixbits_tele<<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, s.fix);
+    if (xf) {
+        const size_t nside = (size_t)nsrc * (XF + ZF);
+        if (c->d_sides_n < nside) {
+            (void)hipFree(c->d_sides);
+            c->d_sides = nullptr;
+            c->d_sides_n = 0;
+            HIPCHK(c, dalloc(&c->d_sides, nside * sizeof(double)));
+            c->d_sides_n = nside;
+        }
+        if (c->d_side_of_n < (size_t)nsrc * 2) {
+            (void)hipFree(c->d_side_of);
+            c->d_side_of = nullptr;
+            c->d_side_of_n = 0;
+            HIPCHK(c, dalloc(&c->d_side_of, (size_t)nsrc * 2 * sizeof(int)));
+            c->d_side_of_n = (size_t)nsrc * 2;
+        }
+        double* dx = c->d_sides;
+        double* dz = c->d_sides + (size_t)nsrc * XF;
+        HIPCHK(c, hipMemcpyAsync(dx, xf, (size_t)nsrc * XF * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dz, zf, (size_t)nsrc * ZF * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_side_of, side, (size_t)nsrc * 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        if (precision == FMM3D_PREC_FP64)
+            k_init_sides<double><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dx, dz, c->d_side_of, s.toff, (double*)s.T, s.fix);
+        else
+            k_init_sides<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dx, dz, c->d_side_of, s.toff, (float*)s.T, s.fix);
+    }
     if (c->strategy >= 1) {
         std::vector<int> flow((size_t)nsrc * 2);
         for (int q = 0; q < nsrc; ++q) lateral_flow(g, faces + (size_t)q * F, &flow[2 * q]);
@@ -2481,24 +2566,58 @@ extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int
     return solve_run(c, nsrc, order, precision);
 }
 
-// Lateral flow signs for a general fixed set: the (theta, phi) slope of a
-// least-squares plane through the fixed times over their node positions.
+extern "C" int fmm3d_solve_tele(fmm3d_ctx* c, int nsrc, const double* faces, int order, int precision, int flags)
+{
+    return solve_tele_impl(c, "fmm3d_solve_tele", nsrc, faces, nullptr, nullptr, nullptr, order, precision, flags);
+}
+
+extern "C" int fmm3d_solve_tele_sides(fmm3d_ctx* c, int nsrc, const double* faces, const double* xface,
+                                      const double* zface, const int* side, int order, int precision, int flags)
+{
+    return solve_tele_impl(c, "fmm3d_solve_tele_sides", nsrc, faces, xface, zface, side, order, precision, flags);
+}
+
+// Lateral flow signs for a general fixed set: the (theta, phi) slopes of a
+// least-squares fit t = a + gx*ix + gz*iz + gr*ir through the fixed times
+// over their node positions.  The depth term keeps the strong vertical
+// trend of side faces (times growing upward along the whole face) out of
+// the lateral slopes; a set within one layer fits (theta, phi) alone.
 static void fixed_flow(const Grid& g, const int64_t* node, const double* tv, int64_t n, int out[2])
 {
-    double m = 0, sx = 0, sz = 0, sxx = 0, szz = 0, sxz = 0, st = 0, sxt = 0, szt = 0;
     const int64_t NL = (int64_t)g.nnx * g.nnz;
+    double m = 0, s[3] = { 0, 0, 0 }, st = 0;
     for (int64_t i = 0; i < n; ++i) {
         const int64_t f = node[i] % NL;
-        const double ix = (double)(f / g.nnz), iz = (double)(f % g.nnz), v = tv[i];
-        m += 1; sx += ix; sz += iz; sxx += ix * ix; szz += iz * iz; sxz += ix * iz;
-        st += v; sxt += ix * v; szt += iz * v;
+        const double p[3] = { (double)(f / g.nnz), (double)(f % g.nnz), (double)(node[i] / NL) };
+        m += 1; st += tv[i];
+        for (int a = 0; a < 3; ++a) s[a] += p[a];
     }
     double gx = 0.0, gz = 0.0;
     if (m >= 3) {
-        const double cxx = sxx - sx * sx / m, czz = szz - sz * sz / m, cxz = sxz - sx * sz / m;
-        const double cxt = sxt - sx * st / m, czt = szt - sz * st / m;
-        const double det = cxx * czz - cxz * cxz;
-        if (det > 0.0) { gx = (cxt * czz - czt * cxz) / det; gz = (czt * cxx - cxt * cxz) / det; }
+        double C[3][3] = {}, b[3] = {};
+        const double mu[3] = { s[0] / m, s[1] / m, s[2] / m }, mt = st / m;
+        for (int64_t i = 0; i < n; ++i) {
+            const int64_t f = node[i] % NL;
+            const double d[3] = { (double)(f / g.nnz) - mu[0], (double)(f % g.nnz) - mu[1], (double)(node[i] / NL) - mu[2] };
+            const double dt = tv[i] - mt;
+            for (int a = 0; a < 3; ++a) {
+                b[a] += d[a] * dt;
+                for (int c2 = 0; c2 < 3; ++c2) C[a][c2] += d[a] * d[c2];
+            }
+        }
+        const double det3 = C[0][0] * (C[1][1] * C[2][2] - C[1][2] * C[2][1]) - C[0][1] * (C[1][0] * C[2][2] - C[1][2] * C[2][0]) +
+                            C[0][2] * (C[1][0] * C[2][1] - C[1][1] * C[2][0]);
+        const double det2 = C[0][0] * C[1][1] - C[0][1] * C[1][0];
+        if (C[2][2] > 0.0 && fabs(det3) > 1e-12 * fabs(C[0][0] * C[1][1] * C[2][2])) {
+            // Cramer's rule on the 3 x 3 normal equations
+            gx = (b[0] * (C[1][1] * C[2][2] - C[1][2] * C[2][1]) - C[0][1] * (b[1] * C[2][2] - C[1][2] * b[2]) +
+                  C[0][2] * (b[1] * C[2][1] - C[1][1] * b[2])) / det3;
+            gz = (C[0][0] * (b[1] * C[2][2] - C[1][2] * b[2]) - b[0] * (C[1][0] * C[2][2] - C[1][2] * C[2][0]) +
+                  C[0][2] * (C[1][0] * b[2] - b[1] * C[2][0])) / det3;
+        } else if (det2 > 0.0) {
+            gx = (b[0] * C[1][1] - b[1] * C[0][1]) / det2;
+            gz = (b[1] * C[0][0] - b[0] * C[1][0]) / det2;
+        }
     }
     flow_code(gx, gz, out);
 }

@@ -24,7 +24,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 WALK_ALTERNATE = 1
 
 EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model_vtx",
-           "fmm3d_solve_fixed", "fmm3d_solve_point",
+           "fmm3d_solve_fixed", "fmm3d_solve_point", "fmm3d_solve_tele_sides",
            "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_result",
            "fmm3d_server_stats", "fmm3d_server_destroy",
            "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_frechet", "fmm3d_frechet_pinned", "fmm3d_get_field",
@@ -91,6 +91,8 @@ def load(path: Optional[str] = None):
     L.fmm3d_last_error.restype = C.c_char_p
     L.fmm3d_set_model_vtx.argtypes = [vp, dp]
     L.fmm3d_solve_tele.argtypes = [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int]
+    L.fmm3d_solve_tele_sides.argtypes = [vp, C.c_int, dp, dp, dp, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
+                                         C.c_int, C.c_int, C.c_int]
     L.fmm3d_server_create.argtypes = [C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
     L.fmm3d_server_submit.argtypes = [vp, C.POINTER(RequestC), C.POINTER(C.c_int64)]
     L.fmm3d_server_wait.argtypes = [vp, C.c_int64, C.c_double]
@@ -166,6 +168,23 @@ class Context:
             raise ValueError(f"faces must be (nsrc, {self.g.nnx}, {self.g.nnz}), got {faces.shape}")
         self._check(self.L.fmm3d_solve_tele(self._h, faces.shape[0], faces, order, precision, 0))
         self.nsrc = faces.shape[0]
+
+    def solve_tele_sides(self, faces: np.ndarray, xface: np.ndarray, zface: np.ndarray, side: np.ndarray,
+                         order: int = ORDER_MIXED, precision: int = PREC_FP64) -> None:
+        """Bottom faces (nsrc, nnx, nnz) plus one theta face (nsrc, nnr, nnz) and
+        one phi face (nsrc, nnr, nnx) per source; side (nsrc, 2): which theta
+        and phi face (-1 none, 0 low index, 1 high index)."""
+        faces = np.ascontiguousarray(faces, dtype=np.float64)
+        xface = np.ascontiguousarray(xface, dtype=np.float64)
+        zface = np.ascontiguousarray(zface, dtype=np.float64)
+        side = np.ascontiguousarray(side, dtype=np.int32)
+        n = faces.shape[0]
+        g = self.g
+        if (faces.shape != (n, g.nnx, g.nnz) or xface.shape != (n, g.nnr, g.nnz) or zface.shape != (n, g.nnr, g.nnx)
+                or side.shape != (n, 2)):
+            raise ValueError("solve_tele_sides: face arrays of inconsistent shapes")
+        self._check(self.L.fmm3d_solve_tele_sides(self._h, n, faces, xface, zface, side, order, precision, 0))
+        self.nsrc = n
 
     def solve_fixed(self, sets, order: int = ORDER_MIXED, precision: int = PREC_FP64, alternate: bool = False) -> None:
         """sets: one (node_index array, time array) pair per source."""

@@ -274,6 +274,19 @@ def tele_boundary_nodes(g: GridDesc, s: TeleSource, sides: bool = True):
     return idx.astype(np.int64), T.ravel()[idx].astype(np.float64)
 
 
+def tele_faces(g: GridDesc, s: TeleSource):
+    """The fixed set of tele_boundary_nodes as face arrays for
+    fmm3d_solve_tele_sides: (bottom (nnx, nnz), theta face (nnr, nnz), phi
+    face (nnr, nnx), side (theta, phi) with 0 the low-index face and 1 the
+    high-index one), 1-D-model times."""
+    T = tele_field_1d(g, s)
+    bot = T[-1]
+    xs = 0 if bot[0, :].mean() <= bot[-1, :].mean() else 1
+    zs = 0 if bot[:, 0].mean() <= bot[:, -1].mean() else 1
+    return (bot.copy(), T[:, -1 if xs else 0, :].copy(), T[:, :, -1 if zs else 0].copy(),
+            np.array([xs, zs], np.int32))
+
+
 def point_source_rad(g: GridDesc, src: "LocalSource"):
     """(height km, colatitude rad, longitude rad) of a local source."""
     return np.array([-src.depth_km, (90.0 - src.lat) * PI_REF / 180.0, src.lon * PI_REF / 180.0])

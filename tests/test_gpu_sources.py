@@ -103,3 +103,34 @@ def test_source_errors():
         with pytest.raises(lib.FmmError) as e:
             c.solve_fixed([(np.array([5]), np.array([np.nan]))])
         assert e.value.code == lib.E_ARG
+
+
+def test_side_faces_as_face_arrays_equal_the_fixed_set():
+    """fmm3d_solve_tele_sides (bottom + one theta and one phi face as arrays)
+    relaxes the same fixed set as fmm3d_solve_fixed with the node lists of
+    tele_boundary_nodes: float64 fields within 1e-9 s (different patch
+    orders, same fixed point), float32 within 1 ms of them."""
+    g, velv, srcs, rcv = synth.tele_case((31, 33, 35), 400.0, (10.0, 10.0), (10, 11, 12), 0.05, 4, 12, 41)
+    parts = [synth.tele_faces(g, s) for s in srcs]
+    faces = np.stack([p[0] for p in parts]); xf = np.stack([p[1] for p in parts])
+    zf = np.stack([p[2] for p in parts]); side = np.stack([p[3] for p in parts])
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        c.solve_fixed([synth.tele_boundary_nodes(g, s) for s in srcs], lib.ORDER_MIXED, lib.PREC_FP64)
+        ref = [c.field(k) for k in range(len(srcs))]
+        c.solve_tele_sides(faces, xf, zf, side, lib.ORDER_MIXED, lib.PREC_FP64)
+        for k in range(len(srcs)):
+            assert float(np.abs(c.field(k) - ref[k]).max()) <= 1e-9
+        c.solve_tele_sides(faces, xf, zf, side, lib.ORDER_MIXED, lib.PREC_FP32)
+        for k in range(len(srcs)):
+            assert float(np.abs(c.field(k) - ref[k]).max()) <= 1e-3
+        # a face left out (-1) is not injected: the result is the bottom-only solve
+        none = np.full_like(side, -1)
+        c.solve_tele_sides(faces, xf, zf, none, lib.ORDER_MIXED, lib.PREC_FP64)
+        a = c.field(0)
+        c.solve_tele(faces[:1], lib.ORDER_MIXED, lib.PREC_FP64)
+        assert float(np.abs(a - c.field(0)).max()) <= 1e-9
+        bad = side.copy(); bad[0, 0] = 2
+        with pytest.raises(lib.FmmError) as e:
+            c.solve_tele_sides(faces, xf, zf, bad)
+        assert e.value.code == lib.E_ARG
