@@ -138,6 +138,20 @@ int fmm3d_solve_fixed(fmm3d_ctx *ctx, int nsrc, const int64_t *offs, const int64
 int fmm3d_solve_point(fmm3d_ctx *ctx, int nsrc, const double *src, int refine, int radius,
                       int order, int precision, int flags);
 
+/* One batch of teleseismic and point sources relaxed together: slots
+ * 0 .. ntele-1 hold the teleseismic sources (faces, xface, zface, side as
+ * for fmm3d_solve_tele_sides; xface null for bottom faces only) and slots
+ * ntele .. ntele+npoint-1 the point sources (src as for fmm3d_solve_point,
+ * source q refined by `refine` over radius[q] nodes).  Teleseismic slots
+ * walk upward in flow order, point slots alternate their walk, and every
+ * slot leaves the sweeps once a sweep changed none of its nodes; the
+ * result per slot is that of the single-kind calls.  No counterpart in the
+ * reference (one teleseismic source at a time, fm3dt_parallel.f90:6371).
+ * flags: 0. */
+int fmm3d_solve_mixed(fmm3d_ctx *ctx, int ntele, const double *faces, const double *xface,
+                      const double *zface, const int *side, int npoint, const double *src,
+                      int refine, const int *radius, int order, int precision, int flags);
+
 /* Receiver times for every resident source (srtimes,
  * fm3dt_parallel.f90:7040-7248): t_out is nsrc*nrc float32, source-major.
  * A receiver outside the grid yields NaN. */
@@ -194,7 +208,11 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *   "ordered_sweeps" column walk: only sweeps up to this one wait on their
  *                   upstream patches (default 0: every sweep)
  *   "spin_limit"    polls before such a wait is given up (default 20000)
- *   "graph"         1 (default): replay 4-sweep chunks from a captured hipGraph
+ *   "check_every"   column walk: sweeps between convergence checks (1..8,
+ *                   default 4); at a check, every source whose last sweep
+ *                   changed nothing leaves the remaining sweeps
+ *   "graph"         1 (default): replay each chunk of check_every sweeps
+ *                   from a captured hipGraph
  *   "autotune"      1: the first solve of a kind (precision, order, walk,
  *                   batch size class) times patch shapes 8x8/8x16/16x8/16x16
  *                   with 8 or 16 passes on the real batch and keeps the

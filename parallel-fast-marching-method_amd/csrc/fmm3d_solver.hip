@@ -1472,13 +1472,15 @@ struct fmm3d_ctx {
     int plane_sweeps = 0;          // hyperplane walk: sweeps before the column walk takes over (0: all)
     int use_graph = 1;             // replay sweep chunks from a captured hipGraph
     int ordered_until = 0;         // column walk: sweeps up to this one wait on upstream patches (0: all)
+    int check_every = 4;           // column walk: sweeps per convergence check (one graph replay)
     int autotune = 0;              // time candidate patch shapes / pass caps at first use
     std::vector<std::array<int, 8>> tuned;   // key (prec, order, walk, batch class) -> (px, pz, passes)
     int* d_iter = nullptr;         // device sweep counter (graph replays)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<char> g_sig;       // kernel arguments, grid and type the graph was captured with
-    int walk_alt = 0;              // this solve: alternate the column walk direction per sweep
+    int walk_alt = 0;              // this solve: alternate the column walk direction per sweep (2: per slot)
+    std::vector<int> walk_slots;   // per slot walk flags when walk_alt == 2 (mixed batches)
     std::vector<double> velv_host; // the vertex model (for refined sub-grids)
     fmm3d_ctx* fine = nullptr;     // refined source neighbourhood (point sources)
     // continuous-batching engine
@@ -2208,7 +2210,8 @@ static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k, bo
     if (c->strategy != 2 && !column_supported(c->col_px, c->col_pz))
         return fail(c, FMM3D_E_ARG, "solver: unsupported column patch %dx%d", c->col_px, c->col_pz);
     const int kind = sweep_kernel(c, k, engine);
-    if (!c->use_graph || n != FMM3D_CHECK || c->trace || kind != sweep_kernel(c, k + n - 1, engine)) {
+    if (!c->use_graph || n != (engine ? FMM3D_CHECK : c->check_every) || c->trace ||
+        kind != sweep_kernel(c, k + n - 1, engine)) {
         a.iterp = nullptr;
         for (int j = 0; j < n; ++j) {
             a.iter = k + j;
@@ -2304,15 +2307,26 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     HIPCHK(c, hipGetLastError());
     int k = 1, last = 0;
     bool done = false;
+    std::vector<int> onv(nsrc, 1);
     while (!done) {
-        const int chunk = c->max_sweeps > 0 ? std::min(FMM3D_CHECK, c->max_sweeps - k + 1) : FMM3D_CHECK;
+        const int chunk = c->max_sweeps > 0 ? std::min(c->check_every, c->max_sweeps - k + 1) : c->check_every;
         int rcl = launch_chunk<R, FAST, SHIFT>(c, a, blocks, chunk, k);
         if (rcl) return rcl;
         k += chunk;
         HIPCHK(c, hipMemcpyAsync(lc.data(), s.lastchg, nsrc * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         int mx = 0;
-        for (int q = 0; q < nsrc; ++q) mx = std::max(mx, lc[q]);
+        bool leave = false;
+        for (int q = 0; q < nsrc; ++q) {
+            mx = std::max(mx, lc[q]);
+            // sweep k-1 changed no node of slot q: it is at its fixed point
+            // and leaves the remaining sweeps (its blocks return at once)
+            if (onv[q] && lc[q] < k - 1) { onv[q] = 0; leave = true; }
+        }
+        if (leave && mx >= k - 1) {
+            HIPCHK(c, hipMemcpyAsync(s.on, onv.data(), nsrc * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
         if (c->trace) {
             fprintf(stderr, "fmm3d trace: lastchg");
             for (int q = 0; q < nsrc; ++q) fprintf(stderr, " %d", lc[q]);
@@ -2394,7 +2408,10 @@ static int solve_run_once(fmm3d_ctx* c, int nsrc, int order, int precision)
 {
     Slots& s = c->sl;
     k_fill_real<double><<<(3 * nsrc + 255) / 256, 256, 0, c->stream>>>(s.srcpos, (size_t)3 * nsrc, __builtin_nan(""));
-    k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.walkalt, nsrc, c->walk_alt);
+    if (c->walk_alt == 2 && (int)c->walk_slots.size() == nsrc)
+        HIPCHK(c, hipMemcpyAsync(s.walkalt, c->walk_slots.data(), nsrc * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    else
+        k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.walkalt, nsrc, c->walk_alt ? 1 : 0);
     HIPCHK(c, hipGetLastError());
     c->stats = fmm3d_stats{};
     c->stats.tile_r = c->brick_sl; c->stats.tile_x = c->tile_x; c->stats.tile_z = c->tile_z;
@@ -2480,24 +2497,17 @@ static double positive_min(const double* f, size_t n, int* nan)
     return m;
 }
 
-// Teleseismic solves from bottom faces and, when xf is given, one theta
-// and one phi side face per source (side[2s], side[2s+1]: -1 none, 0 the
-// low-index face, 1 the high-index one).
-static int solve_tele_impl(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, const double* xf,
-                           const double* zf, const int* side, int order, int precision, int flags)
+// Argument checks of teleseismic face arrays and the float32 offset of
+// each source (its boundary minimum).  side[2s], side[2s+1]: -1 none, 0
+// the low-index face, 1 the high-index one.
+static int tele_offsets(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, const double* xf,
+                        const double* zf, const int* side, std::vector<double>& toff)
 {
-    if (!c) return fail(c, FMM3D_E_ARG, "%s: null context", fn);
     if (nsrc <= 0 || !faces) return fail(c, FMM3D_E_ARG, "%s: need nsrc > 0 and faces", fn);
     if (xf && (!zf || !side)) return fail(c, FMM3D_E_ARG, "%s: need both side faces and the side indices", fn);
-    if (order != FMM3D_ORDER_FIRST && order != FMM3D_ORDER_MIXED) return fail(c, FMM3D_E_ARG, "%s: bad order %d", fn, order);
-    if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "%s: bad precision %d", fn, precision);
-    if (flags != 0) return fail(c, FMM3D_E_ARG, "%s: flags must be 0", fn);
-    if (!c->have_model) return fail(c, FMM3D_E_STATE, "%s: no velocity model (call fmm3d_set_model_vtx)", fn);
-    HIPCHK(c, hipSetDevice(c->device));
     const Grid& g = c->g;
     const size_t F = (size_t)g.nnx * g.nnz, XF = (size_t)g.nnr * g.nnz, ZF = (size_t)g.nnr * g.nnx;
-    // boundary minimum per source: the float32 offset (and a sanity check)
-    std::vector<double> toff(nsrc, 0.0);
+    toff.assign(nsrc, 0.0);
     for (int s = 0; s < nsrc; ++s) {
         int nan = 0;
         double m = positive_min(faces + (size_t)s * F, F, &nan);
@@ -2510,11 +2520,17 @@ static int solve_tele_impl(fmm3d_ctx* c, const char* fn, int nsrc, const double*
         if (nan) return fail(c, FMM3D_E_ARG, "%s: NaN boundary time (source %d)", fn, s);
         toff[s] = isfinite(m) ? m : 0.0;
     }
-    int rc = ensure_slots(c, nsrc, precision);
-    if (rc) return rc;
+    return FMM3D_OK;
+}
+
+// Initial fields, fixed bits, offsets and lateral flows of slots
+// 0 .. nsrc-1 from teleseismic faces (slots must be allocated).
+static int tele_upload(fmm3d_ctx* c, int nsrc, const double* faces, const double* xf, const double* zf,
+                       const int* side, int precision, const std::vector<double>& toff)
+{
+    const Grid& g = c->g;
     Slots& s = c->sl;
-    s.n = 0;
-    c->walk_alt = 0;
+    const size_t F = (size_t)g.nnx * g.nnz, XF = (size_t)g.nnr * g.nnz, ZF = (size_t)g.nnr * g.nnx;
     if (c->d_faces_n < (size_t)nsrc * F) {
         (void)hipFree(c->d_faces);
         c->d_faces = nullptr;
@@ -2556,13 +2572,36 @@ static int solve_tele_impl(fmm3d_ctx* c, const char* fn, int nsrc, const double*
         else
             k_init_sides<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dx, dz, c->d_side_of, s.toff, (float*)s.T, s.fix);
     }
-    if (c->strategy >= 1) {
-        std::vector<int> flow((size_t)nsrc * 2);
-        for (int q = 0; q < nsrc; ++q) lateral_flow(g, faces + (size_t)q * F, &flow[2 * q]);
-        HIPCHK(c, hipMemcpyAsync(s.flow, flow.data(), flow.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    }
+    std::vector<int> flow((size_t)nsrc * 2);
+    for (int q = 0; q < nsrc; ++q) lateral_flow(g, faces + (size_t)q * F, &flow[2 * q]);
+    HIPCHK(c, hipMemcpyAsync(s.flow, flow.data(), flow.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // the flow vector is pageable host memory: the copy must finish before it goes
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return fail(c, FMM3D_E_DEVICE, "init: %s", hipGetErrorString(le));
+    return FMM3D_OK;
+}
+
+// Teleseismic solves from bottom faces and, when xf is given, one theta
+// and one phi side face per source.
+static int solve_tele_impl(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, const double* xf,
+                           const double* zf, const int* side, int order, int precision, int flags)
+{
+    if (!c) return fail(c, FMM3D_E_ARG, "%s: null context", fn);
+    if (order != FMM3D_ORDER_FIRST && order != FMM3D_ORDER_MIXED) return fail(c, FMM3D_E_ARG, "%s: bad order %d", fn, order);
+    if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "%s: bad precision %d", fn, precision);
+    if (flags != 0) return fail(c, FMM3D_E_ARG, "%s: flags must be 0", fn);
+    if (!c->have_model) return fail(c, FMM3D_E_STATE, "%s: no velocity model (call fmm3d_set_model_vtx)", fn);
+    std::vector<double> toff;
+    int rc = tele_offsets(c, fn, nsrc, faces, xf, zf, side, toff);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    rc = ensure_slots(c, nsrc, precision);
+    if (rc) return rc;
+    c->sl.n = 0;
+    c->walk_alt = 0;
+    rc = tele_upload(c, nsrc, faces, xf, zf, side, precision, toff);
+    if (rc) return rc;
     return solve_run(c, nsrc, order, precision);
 }
 
@@ -2622,40 +2661,45 @@ static void fixed_flow(const Grid& g, const int64_t* node, const double* tv, int
     flow_code(gx, gz, out);
 }
 
-extern "C" int fmm3d_solve_fixed(fmm3d_ctx* c, int nsrc, const int64_t* offs, const int64_t* node, const double* tv,
-                                 int order, int precision, int flags)
+// Argument checks of fixed node sets, and per source the float32 offset
+// (the smallest fixed time) and the lateral flow code.
+static int fixed_prepare(fmm3d_ctx* c, const char* fn, int nsrc, const int64_t* offs, const int64_t* node,
+                         const double* tv, std::vector<double>& toff, std::vector<int>& flow)
 {
-    if (!c) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: null context");
-    if (nsrc <= 0 || !offs) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: need nsrc > 0 and offsets");
-    if (order != FMM3D_ORDER_FIRST && order != FMM3D_ORDER_MIXED) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: bad order %d", order);
-    if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: bad precision %d", precision);
-    if (flags & ~FMM3D_WALK_ALTERNATE) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: unknown flags 0x%x", flags);
-    if (!c->have_model) return fail(c, FMM3D_E_STATE, "fmm3d_solve_fixed: no velocity model (call fmm3d_set_model_vtx)");
-    if (offs[0] != 0) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: offs[0] must be 0");
-    const int64_t total = offs[nsrc];
-    if (total > 0 && (!node || !tv)) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: null node or time array");
-    std::vector<double> toff(nsrc, 0.0);
-    std::vector<int> flow((size_t)nsrc * 2, 1);
+    if (nsrc <= 0 || !offs) return fail(c, FMM3D_E_ARG, "%s: need nsrc > 0 and offsets", fn);
+    if (offs[0] != 0) return fail(c, FMM3D_E_ARG, "%s: offs[0] must be 0", fn);
+    if (offs[nsrc] > 0 && (!node || !tv)) return fail(c, FMM3D_E_ARG, "%s: null node or time array", fn);
+    toff.assign(nsrc, 0.0);
+    flow.assign((size_t)nsrc * 2, 1);
     const Grid& g = c->g;
     for (int q = 0; q < nsrc; ++q) {
-        if (offs[q + 1] < offs[q]) return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: offsets decrease at source %d", q);
+        if (offs[q + 1] < offs[q]) return fail(c, FMM3D_E_ARG, "%s: offsets decrease at source %d", fn, q);
         double m = __builtin_huge_val();
         for (int64_t i = offs[q]; i < offs[q + 1]; ++i) {
             if (node[i] < 0 || (size_t)node[i] >= c->N)
-                return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: node %lld out of range (source %d)", (long long)node[i], q);
+                return fail(c, FMM3D_E_ARG, "%s: node %lld out of range (source %d)", fn, (long long)node[i], q);
             if (!(tv[i] >= 0.0) || !isfinite(tv[i]))
-                return fail(c, FMM3D_E_ARG, "fmm3d_solve_fixed: time %g at node %lld must be finite and >= 0", tv[i], (long long)node[i]);
+                return fail(c, FMM3D_E_ARG, "%s: time %g at node %lld must be finite and >= 0", fn, tv[i], (long long)node[i]);
             m = std::min(m, tv[i]);
         }
         toff[q] = isfinite(m) ? m : 0.0;
         fixed_flow(g, node + offs[q], tv + offs[q], offs[q + 1] - offs[q], &flow[2 * q]);
     }
-    HIPCHK(c, hipSetDevice(c->device));
-    int rc = ensure_slots(c, nsrc, precision);
-    if (rc) return rc;
+    return FMM3D_OK;
+}
+
+// Initial fields, fixed bits, offsets and flows of slots slot0 ..
+// slot0+nsrc-1 from fixed node sets (slots must be allocated).
+static int fixed_upload(fmm3d_ctx* c, const char* fn, int slot0, int nsrc, const int64_t* offs, const int64_t* node,
+                        const double* tv, int precision, const std::vector<double>& toff, const std::vector<int>& flow)
+{
     Slots& s = c->sl;
-    s.n = 0;
-    c->walk_alt = (flags & FMM3D_WALK_ALTERNATE) ? 1 : 0;
+    const int64_t total = offs[nsrc];
+    const size_t N = c->N, NW = (N + 31) / 32;
+    const size_t es = precision == FMM3D_PREC_FP64 ? sizeof(double) : sizeof(float);
+    void* T0 = (char*)s.T + (size_t)slot0 * N * es;
+    uint32_t* fix0 = s.fix + (size_t)slot0 * NW;
+    double* toff0 = s.toff + slot0;
     int64_t* doffs = nullptr;
     int64_t* dnode = nullptr;
     double* dtv = nullptr;
@@ -2665,29 +2709,51 @@ extern "C" int fmm3d_solve_fixed(fmm3d_ctx* c, int nsrc, const int64_t* offs, co
     if (e == hipSuccess && total > 0) e = dalloc(&dtv, (size_t)total * sizeof(double));
     if (e == hipSuccess && total > 0) e = hipMemcpyAsync(dnode, node, (size_t)total * sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && total > 0) e = hipMemcpyAsync(dtv, tv, (size_t)total * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(s.toff, toff.data(), nsrc * sizeof(double), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(s.flow, flow.data(), flow.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(toff0, toff.data(), nsrc * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.flow + 2 * slot0, flow.data(), flow.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        const size_t n = (size_t)nsrc * c->N, nw = (size_t)nsrc * ((c->N + 31) / 32);
-        if (precision == FMM3D_PREC_FP64) k_fill_real<double><<<4096, 256, 0, c->stream>>>((double*)s.T, n, Lim<double>::inf());
-        else k_fill_real<float><<<4096, 256, 0, c->stream>>>((float*)s.T, n, Lim<float>::inf());
-        k_fill_u32<<<4096, 256, 0, c->stream>>>(s.fix, nw);
+        const size_t n = (size_t)nsrc * N, nw = (size_t)nsrc * NW;
+        if (precision == FMM3D_PREC_FP64) k_fill_real<double><<<4096, 256, 0, c->stream>>>((double*)T0, n, Lim<double>::inf());
+        else k_fill_real<float><<<4096, 256, 0, c->stream>>>((float*)T0, n, Lim<float>::inf());
+        k_fill_u32<<<4096, 256, 0, c->stream>>>(fix0, nw);
         if (total > 0) {
             const int nb = (int)std::min<int64_t>(4096, (total + 255) / 256);
             if (precision == FMM3D_PREC_FP64)
-                k_scatter_fixed<double><<<nb, 256, 0, c->stream>>>(nsrc, c->N, doffs, dnode, dtv, s.toff, (double*)s.T, s.fix);
+                k_scatter_fixed<double><<<nb, 256, 0, c->stream>>>(nsrc, N, doffs, dnode, dtv, toff0, (double*)T0, fix0);
             else
-                k_scatter_fixed<float><<<nb, 256, 0, c->stream>>>(nsrc, c->N, doffs, dnode, dtv, s.toff, (float*)s.T, s.fix);
+                k_scatter_fixed<float><<<nb, 256, 0, c->stream>>>(nsrc, N, doffs, dnode, dtv, toff0, (float*)T0, fix0);
         }
         e = hipGetLastError();
     }
-    if (e != hipSuccess) {
-        (void)hipFree(doffs); (void)hipFree(dnode); (void)hipFree(dtv);
-        return fail(c, FMM3D_E_DEVICE, "fmm3d_solve_fixed: %s", hipGetErrorString(e));
-    }
-    rc = solve_run(c, nsrc, order, precision);
+    // the staging buffers and the host vectors must outlive the copies and the scatter
+    const hipError_t es2 = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = es2;
     (void)hipFree(doffs); (void)hipFree(dnode); (void)hipFree(dtv);
-    return rc;
+    if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "%s: %s", fn, hipGetErrorString(e));
+    return FMM3D_OK;
+}
+
+extern "C" int fmm3d_solve_fixed(fmm3d_ctx* c, int nsrc, const int64_t* offs, const int64_t* node, const double* tv,
+                                 int order, int precision, int flags)
+{
+    const char* fn = "fmm3d_solve_fixed";
+    if (!c) return fail(c, FMM3D_E_ARG, "%s: null context", fn);
+    if (order != FMM3D_ORDER_FIRST && order != FMM3D_ORDER_MIXED) return fail(c, FMM3D_E_ARG, "%s: bad order %d", fn, order);
+    if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "%s: bad precision %d", fn, precision);
+    if (flags & ~FMM3D_WALK_ALTERNATE) return fail(c, FMM3D_E_ARG, "%s: unknown flags 0x%x", fn, flags);
+    if (!c->have_model) return fail(c, FMM3D_E_STATE, "%s: no velocity model (call fmm3d_set_model_vtx)", fn);
+    std::vector<double> toff;
+    std::vector<int> flow;
+    int rc = fixed_prepare(c, fn, nsrc, offs, node, tv, toff, flow);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    rc = ensure_slots(c, nsrc, precision);
+    if (rc) return rc;
+    c->sl.n = 0;
+    c->walk_alt = (flags & FMM3D_WALK_ALTERNATE) ? 1 : 0;
+    rc = fixed_upload(c, fn, 0, nsrc, offs, node, tv, precision, toff, flow);
+    if (rc) return rc;
+    return solve_run(c, nsrc, order, precision);
 }
 
 // B-spline velocity at one point, in the gridder's arithmetic (k_gridder);
@@ -2815,6 +2881,24 @@ static int point_fixed_set(fmm3d_ctx* c, const double* src3, int refine, int rad
     return FMM3D_OK;
 }
 
+// Fixed sets of point sources, source q refined over radius[q] nodes.
+static int point_sets(fmm3d_ctx* c, const char* fn, int nsrc, const double* src, int refine, const int* radius,
+                      int order, std::vector<int64_t>& offs, std::vector<int64_t>& node, std::vector<double>& tv)
+{
+    if (nsrc <= 0 || !src || !radius) return fail(c, FMM3D_E_ARG, "%s: need nsrc > 0, sources and radii", fn);
+    if (refine < 1 || refine > 16) return fail(c, FMM3D_E_ARG, "%s: refine must be 1..16", fn);
+    offs.assign(nsrc + 1, 0);
+    node.clear();
+    tv.clear();
+    for (int q = 0; q < nsrc; ++q) {
+        if (radius[q] < 1 || radius[q] > 64) return fail(c, FMM3D_E_ARG, "%s: radius of source %d must be 1..64", fn, q);
+        int rc = point_fixed_set(c, src + 3 * q, refine, radius[q], order, node, tv);
+        if (rc) return rc;
+        offs[q + 1] = (int64_t)node.size();
+    }
+    return FMM3D_OK;
+}
+
 extern "C" int fmm3d_solve_point(fmm3d_ctx* c, int nsrc, const double* src, int refine, int radius,
                                  int order, int precision, int flags)
 {
@@ -2826,17 +2910,53 @@ extern "C" int fmm3d_solve_point(fmm3d_ctx* c, int nsrc, const double* src, int 
     if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: bad precision %d", precision);
     if (flags != 0) return fail(c, FMM3D_E_ARG, "fmm3d_solve_point: flags must be 0");
     if (!c->have_model) return fail(c, FMM3D_E_STATE, "fmm3d_solve_point: no velocity model (call fmm3d_set_model_vtx)");
-    std::vector<int64_t> offs(nsrc + 1, 0), node;
+    std::vector<int64_t> offs, node;
     std::vector<double> tv;
-    for (int q = 0; q < nsrc; ++q) {
-        int rc = point_fixed_set(c, src + 3 * q, refine, radius, order, node, tv);
-        if (rc) return rc;
-        offs[q + 1] = (int64_t)node.size();
-    }
-    int rc = fmm3d_solve_fixed(c, nsrc, offs.data(), node.data(), tv.data(), order, precision, FMM3D_WALK_ALTERNATE);
+    const std::vector<int> rad(nsrc, radius);
+    int rc = point_sets(c, "fmm3d_solve_point", nsrc, src, refine, rad.data(), order, offs, node, tv);
+    if (rc) return rc;
+    rc = fmm3d_solve_fixed(c, nsrc, offs.data(), node.data(), tv.data(), order, precision, FMM3D_WALK_ALTERNATE);
     if (rc) return rc;
     // rays of these sources end at them
     HIPCHK(c, hipMemcpy(c->sl.srcpos, src, (size_t)3 * nsrc * sizeof(double), hipMemcpyHostToDevice));
+    return FMM3D_OK;
+}
+
+extern "C" int fmm3d_solve_mixed(fmm3d_ctx* c, int ntele, const double* faces, const double* xface, const double* zface,
+                                 const int* side, int npoint, const double* src, int refine, const int* radius,
+                                 int order, int precision, int flags)
+{
+    const char* fn = "fmm3d_solve_mixed";
+    if (!c) return fail(c, FMM3D_E_ARG, "%s: null context", fn);
+    if (ntele < 0 || npoint < 0 || ntele + npoint == 0) return fail(c, FMM3D_E_ARG, "%s: need ntele, npoint >= 0 and a source", fn);
+    if (order != FMM3D_ORDER_FIRST && order != FMM3D_ORDER_MIXED) return fail(c, FMM3D_E_ARG, "%s: bad order %d", fn, order);
+    if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "%s: bad precision %d", fn, precision);
+    if (flags != 0) return fail(c, FMM3D_E_ARG, "%s: flags must be 0", fn);
+    if (!c->have_model) return fail(c, FMM3D_E_STATE, "%s: no velocity model (call fmm3d_set_model_vtx)", fn);
+    std::vector<double> ttoff, ptoff;
+    std::vector<int64_t> offs, node;
+    std::vector<double> tv;
+    std::vector<int> pflow;
+    int rc;
+    if (ntele > 0 && (rc = tele_offsets(c, fn, ntele, faces, xface, zface, side, ttoff))) return rc;
+    if (npoint > 0) {
+        if ((rc = point_sets(c, fn, npoint, src, refine, radius, order, offs, node, tv))) return rc;
+        if ((rc = fixed_prepare(c, fn, npoint, offs.data(), node.data(), tv.data(), ptoff, pflow))) return rc;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nsrc = ntele + npoint;
+    if ((rc = ensure_slots(c, nsrc, precision))) return rc;
+    c->sl.n = 0;
+    if (ntele > 0 && (rc = tele_upload(c, ntele, faces, xface, zface, side, precision, ttoff))) return rc;
+    if (npoint > 0 && (rc = fixed_upload(c, fn, ntele, npoint, offs.data(), node.data(), tv.data(), precision, ptoff, pflow)))
+        return rc;
+    c->walk_slots.assign(nsrc, 0);
+    for (int q = ntele; q < nsrc; ++q) c->walk_slots[q] = 1;
+    c->walk_alt = npoint == 0 ? 0 : ntele == 0 ? 1 : 2;
+    rc = solve_run(c, nsrc, order, precision);
+    if (rc) return rc;
+    if (npoint > 0)
+        HIPCHK(c, hipMemcpy(c->sl.srcpos + 3 * (size_t)ntele, src, (size_t)3 * npoint * sizeof(double), hipMemcpyHostToDevice));
     return FMM3D_OK;
 }
 
@@ -3318,6 +3438,10 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "ordered_sweeps")) {
         if (iv < 0 || iv > 1000000) return fail(c, FMM3D_E_ARG, "ordered_sweeps out of range");
         c->ordered_until = iv;
+    } else if (!strcmp(key, "check_every")) {
+        // at most 8: the per-launch timing events hold one chunk of 8 launches
+        if (iv < 1 || iv > 8) return fail(c, FMM3D_E_ARG, "check_every must be in 1..8");
+        c->check_every = iv;
     } else if (!strcmp(key, "spin_limit")) {
         if (iv < 0 || iv > 10000000) return fail(c, FMM3D_E_ARG, "spin_limit out of range");
         c->spin_limit = iv;

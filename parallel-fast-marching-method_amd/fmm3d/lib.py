@@ -24,7 +24,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 WALK_ALTERNATE = 1
 
 EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model_vtx",
-           "fmm3d_solve_fixed", "fmm3d_solve_point", "fmm3d_solve_tele_sides",
+           "fmm3d_solve_fixed", "fmm3d_solve_point", "fmm3d_solve_tele_sides", "fmm3d_solve_mixed",
            "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_result",
            "fmm3d_server_stats", "fmm3d_server_destroy",
            "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_frechet", "fmm3d_frechet_pinned", "fmm3d_get_field",
@@ -101,6 +101,8 @@ def load(path: Optional[str] = None):
     L.fmm3d_server_destroy.argtypes = [vp]
     L.fmm3d_solve_fixed.argtypes = [vp, C.c_int, lp, lp, dp, C.c_int, C.c_int, C.c_int]
     L.fmm3d_solve_point.argtypes = [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    # nullable arrays go as plain addresses
+    L.fmm3d_solve_mixed.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, C.c_int]
     L.fmm3d_receivers.argtypes = [vp, C.c_int, dp, fp]
     L.fmm3d_frechet.argtypes = [vp, C.c_int, dp, lp, ip, fp, C.c_int64]
     L.fmm3d_frechet_pinned.argtypes = [vp, C.c_int, dp, C.POINTER(C.POINTER(C.c_int64)),
@@ -205,6 +207,40 @@ class Context:
         src = np.ascontiguousarray(np.atleast_2d(src), dtype=np.float64)
         self._check(self.L.fmm3d_solve_point(self._h, src.shape[0], src, refine, radius, order, precision, 0))
         self.nsrc = src.shape[0]
+
+    def solve_mixed(self, faces=None, sides=None, src=None, radius=None, refine: int = 5,
+                    order: int = ORDER_MIXED, precision: int = PREC_FP64) -> None:
+        """One batch of teleseismic sources (bottom faces (nt, nnx, nnz) and,
+        optionally, sides = (xface, zface, side) as for solve_tele_sides) and
+        point sources (src (np, 3), radius (np,) nodes): fields 0 .. nt-1 are
+        the teleseismic ones, nt .. nt+np-1 the point ones."""
+        g = self.g
+        keep = []
+
+        def addr(a, dtype):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return a.ctypes.data_as(C.c_void_p)
+        nt = 0 if faces is None else int(np.asarray(faces).shape[0])
+        npt = 0 if src is None else int(np.atleast_2d(src).shape[0])
+        if faces is not None and np.asarray(faces).shape != (nt, g.nnx, g.nnz):
+            raise ValueError("solve_mixed: faces must be (nsrc, nnx, nnz)")
+        xf = zf = sd = None
+        if sides is not None:
+            xface, zface, side = sides
+            if (np.shape(xface) != (nt, g.nnr, g.nnz) or np.shape(zface) != (nt, g.nnr, g.nnx)
+                    or np.shape(side) != (nt, 2)):
+                raise ValueError("solve_mixed: side arrays of inconsistent shapes")
+            xf, zf, sd = addr(xface, np.float64), addr(zface, np.float64), addr(side, np.int32)
+        if npt and (radius is None or np.shape(radius) != (npt,)):
+            raise ValueError("solve_mixed: need one radius per point source")
+        self._check(self.L.fmm3d_solve_mixed(
+            self._h, nt, addr(faces, np.float64), xf, zf, sd, npt,
+            addr(None if src is None else np.atleast_2d(src), np.float64), refine,
+            addr(radius, np.int32), order, precision, 0))
+        self.nsrc = nt + npt
 
     def field(self, src: int) -> np.ndarray:
         out = np.zeros((self.g.nnr, self.g.nnx, self.g.nnz))

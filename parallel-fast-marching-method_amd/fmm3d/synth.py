@@ -157,6 +157,42 @@ def make_velv(g: GridDesc, amp: float = 0.05, seed: int = 1) -> np.ndarray:
     return np.ascontiguousarray(base * (1.0 + d))
 
 
+def make_velv_slab(g: GridDesc, seed: int = 4, slab_amp: float = 0.10, slab_km: float = 60.0,
+                   dip_deg: float = 45.0, nblobs: int = 10, blob_amp: float = 0.05,
+                   blob_km=(30.0, 60.0)) -> np.ndarray:
+    """Sharp-contrast vertex model (same layout as make_velv): the 1-D
+    gradient times (1 + slab + blobs).  The slab is a layer slab_km thick
+    through the grid centre at mid-depth, striking along phi and dipping
+    dip_deg along theta, +slab_amp inside with a 10 km cosine taper; the
+    blobs are Gaussians of radius U(blob_km) at uniform positions, each
+    +-blob_amp (random sign)."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(g.nvr + 2)
+    depth = -(g.gor - (k - 1) * g.dvr)
+    lat = g.goxd - (np.arange(g.nvx + 2) - 1) * g.dvxd
+    lon = g.gozd + (np.arange(g.nvz + 2) - 1) * g.dvzd
+    km = 111.19
+    lat0, lon0 = float(np.mean(lat)), float(np.mean(lon))
+    X = ((lat - lat0) * km)[None, :, None]
+    Y = ((lon - lon0) * km * np.cos(np.deg2rad(lat0)))[:, None, None]
+    D = depth[None, None, :]
+    dmid = 0.5 * (depth[1] + depth[-2])
+    dip = np.deg2rad(dip_deg)
+    dist = np.abs(np.sin(dip) * X + np.cos(dip) * (D - dmid))
+    half, taper = 0.5 * slab_km, 10.0
+    slab = np.where(dist <= half - taper, 1.0,
+                    np.where(dist >= half, 0.0, 0.5 * (1.0 + np.cos(np.pi * (dist - half + taper) / taper))))
+    d = slab_amp * np.broadcast_to(slab, (Y.shape[0], X.shape[1], D.shape[2]))
+    span = [(X.min(), X.max()), (Y.min(), Y.max()), (depth[1], depth[-2])]
+    for _ in range(nblobs):
+        c = [rng.uniform(lo, hi) for lo, hi in span]
+        r = rng.uniform(*blob_km)
+        a = blob_amp * rng.choice([-1.0, 1.0])
+        d = d + a * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (D - c[2]) ** 2) / r ** 2)
+    base = v1d(depth)[None, None, :]
+    return np.ascontiguousarray(base * (1.0 + d))
+
+
 def node_coords(g: GridDesc):
     """(h[ir], theta[ix], phi[iz]) of propagation nodes, reference arithmetic."""
     h = np.array([g.rgor - (i - 1) * g.dnr for i in range(1, g.nnr + 1)])
@@ -422,6 +458,16 @@ def tele_case(nn=(61, 61, 61), depth_km=400.0, span_deg=(10.0, 10.0), nv=(12, 12
     velv = make_velv(g, amp, seed)
     srcs = tele_sources(g, nsrc, seed)
     rcv = receivers(g, nrc, seed, max_per_quadrant=nrc // 2)
+    return g, velv, srcs, rcv
+
+
+def config_321(nsrc=64, nrc=500, seed=4):
+    """321^3 nodes, 0-600 km, 15x15 deg, 33^3 vertices, a +10 % dipping slab
+    (60 km thick, 45 deg) and ten +-5 % Gaussian blobs (radius 30-60 km)."""
+    g = make_grid((321, 321, 321), 600.0, (15.0, 15.0), nv=(33, 33, 33))
+    velv = make_velv_slab(g, seed)
+    srcs = tele_sources(g, nsrc, seed)
+    rcv = receivers(g, nrc, seed)
     return g, velv, srcs, rcv
 
 

@@ -134,3 +134,63 @@ def test_side_faces_as_face_arrays_equal_the_fixed_set():
         with pytest.raises(lib.FmmError) as e:
             c.solve_tele_sides(faces, xf, zf, bad)
         assert e.value.code == lib.E_ARG
+
+
+def test_mixed_batch_equals_single_kind_solves():
+    """fmm3d_solve_mixed: teleseismic sources (bottom and side faces) and
+    point sources of different refinement radii in one batch give, slot by
+    slot, the fields of the single-kind calls (float64 within 1e-9 s: the
+    same fixed points), the same receiver times and Frechet rows (rays of
+    the point slots end at their sources), and float32 within 1 ms."""
+    g, velv, tele, rcv = synth.tele_case((61, 61, 61), 400.0, (10.0, 10.0), (12, 12, 12), 0.05, 3, 20, 1)
+    rcv = synth.receivers_rad(rcv)
+    parts = [synth.tele_faces(g, s) for s in tele]
+    F, X, Z, S = (np.stack([p[k] for p in parts]) for k in range(4))
+    pts = np.array([[-20.0, g.rgox + 30.37 * g.dnx, g.rgoz + 30.61 * g.dnz],
+                    [-55.0, g.rgox + 14.2 * g.dnx, g.rgoz + 40.9 * g.dnz]])
+    rad = np.array([3, 7])
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        c.solve_tele_sides(F, X, Z, S, lib.ORDER_MIXED, lib.PREC_FP64)
+        ref = [c.field(k) for k in range(3)]
+        ref_t = list(c.receivers(rcv))
+        rp, cols, vals = c.frechet(rcv)
+        ref_rows = [(cols[rp[i]:rp[i + 1]], vals[rp[i]:rp[i + 1]]) for i in range(len(rp) - 1)]
+        for k in range(2):
+            c.solve_point(pts[k:k + 1], 5, int(rad[k]), lib.ORDER_MIXED, lib.PREC_FP64)
+            ref.append(c.field(0))
+            ref_t.append(c.receivers(rcv)[0])
+            rp, cols, vals = c.frechet(rcv)
+            ref_rows += [(cols[rp[i]:rp[i + 1]], vals[rp[i]:rp[i + 1]]) for i in range(len(rp) - 1)]
+        c.solve_mixed(F, (X, Z, S), pts, rad, 5, lib.ORDER_MIXED, lib.PREC_FP64)
+        assert c.nsrc == 5
+        for k in range(5):
+            d = float(np.abs(c.field(k) - ref[k]).max())
+            assert d <= 1e-9, (k, d)
+        t = c.receivers(rcv)
+        for k in range(5):
+            assert np.abs(t[k] - ref_t[k]).max() <= np.spacing(np.abs(ref_t[k]).max()), k
+        rp, cols, vals = c.frechet(rcv)
+        assert len(rp) - 1 == len(ref_rows)
+        for r, (rc_, rv) in enumerate(ref_rows):
+            a = dict(zip(cols[rp[r]:rp[r + 1]].tolist(), vals[rp[r]:rp[r + 1]].astype(np.float64).tolist()))
+            b = dict(zip(rc_.tolist(), rv.astype(np.float64).tolist()))
+            assert set(a) == set(b), r
+            scale = max([abs(v) for v in b.values()] + [1e-30])
+            assert max([abs(a[q] - b[q]) for q in b] + [0.0]) <= 1e-6 * scale, r
+        c.solve_mixed(F, (X, Z, S), pts, rad, 5, lib.ORDER_MIXED, lib.PREC_FP32)
+        t32 = c.receivers(rcv)
+        assert np.abs(t32.astype(np.float64) - t).max() <= 1e-3
+        # one kind alone through the mixed call
+        c.solve_tele(F[2:3], lib.ORDER_MIXED, lib.PREC_FP64)
+        bottom = c.field(0)
+        c.solve_mixed(faces=F, precision=lib.PREC_FP64)
+        assert c.nsrc == 3 and float(np.abs(c.field(2) - bottom).max()) <= 1e-9
+        c.solve_mixed(src=pts, radius=rad, precision=lib.PREC_FP64)
+        assert float(np.abs(c.field(1) - ref[4]).max()) <= 1e-9
+        with pytest.raises(lib.FmmError) as e:
+            c.solve_mixed(src=pts, radius=np.array([3, 0]))
+        assert e.value.code == lib.E_ARG
+        with pytest.raises(lib.FmmError) as e:
+            c.solve_mixed()
+        assert e.value.code == lib.E_ARG
