@@ -57,6 +57,11 @@ def parse():
     p.add_argument("--cpu-sources", type=int, default=1)
     p.add_argument("--serve-s", type=float, default=6.0,
                    help="seconds of Poisson arrivals for the serving measurement (0: skip)")
+    p.add_argument("--workload", choices=("tele201", "mixed256", "serving-mix"), default="tele201",
+                   help="tele201: the measured configuration (default); mixed256: 192 plane waves + 64 local "
+                        "sources in one batch at 201^3; serving-mix: Poisson requests over 65-257^3 grids "
+                        "with teleseismic and local sources")
+    p.add_argument("--serve-rate", type=float, default=1.0, help="serving-mix: requests per second offered")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="first run one device-to-device copy of this many MiB (the reference kernel "
                         "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -176,12 +181,154 @@ def cpu_baseline(g, velv, srcs, rcv, n):
                       f"{dt:.1f} s on one host core"}
 
 
+def mixed256(a):
+    """256 sources in one batch on the 201^3 model: 192 plane waves (bottom
+    and entry side faces, seed 3) and 64 local sources (depth 5-80 km,
+    refinement factor 5 over radius 5-20 nodes, seed 3), solved by one
+    fmm3d_solve_mixed call, then receiver times and Frechet rows.  A step
+    is that whole batch in float32; the float64 batch gives the receiver
+    error.  The same sources split by kind are timed beside it."""
+    g, velv, _, rcv_deg = synth.config_201(nsrc=1, nrc=a.nrc, seed=2)
+    rcv = synth.receivers_rad(rcv_deg)
+    tele = synth.tele_sources(g, 192, 3)
+    loc = synth.local_sources(g, 64, 3)
+    parts = [synth.tele_faces(g, s) for s in tele]
+    F, X, Z, S = (np.stack([p[k] for p in parts]) for k in range(4))
+    src = np.stack([synth.point_source_rad(g, s) for s in loc])
+    rad = np.array([s.radius_nodes for s in loc], np.int32)
+    with lib.Context(g) as c:
+        c.set_model(velv)
+
+        def step(prec, tele_on=True, point_on=True):
+            s0 = time.perf_counter()
+            c.solve_mixed(F if tele_on else None, (X, Z, S) if tele_on else None,
+                          src if point_on else None, rad if point_on else None, 5, lib.ORDER_MIXED, prec)
+            s1 = time.perf_counter()
+            st = c.stats()
+            t = c.receivers(rcv)
+            rp, _, _ = c.frechet(rcv, pinned=True)
+            print(f"mixed256: step ({'fp64' if prec == lib.PREC_FP64 else 'fp32'}) {time.perf_counter() - s0:.2f} s",
+                  file=sys.stderr, flush=True)
+            return {"t": t, "solve_wall_s": s1 - s0, "device_solve_ms": st["solve_ms"], "sweeps": st["sweeps"],
+                    "step_s": time.perf_counter() - s0, "entries": int(rp[-1])}
+        r64 = step(lib.PREC_FP64)
+        for _ in range(a.warmup):
+            step(lib.PREC_FP32)
+        runs = [step(lib.PREC_FP32) for _ in range(a.steps)]
+        tele_only = step(lib.PREC_FP32, point_on=False)
+        point_only = step(lib.PREC_FP32, tele_on=False)
+    el = sum(r["step_s"] for r in runs)
+    err = float(np.nanmax(np.abs(runs[-1]["t"].astype(np.float64) - r64["t"])))
+    med = lambda k: round(float(np.median([r[k] for r in runs])), 3)
+    return {"metric": "source-solves/s, 192 teleseismic + 64 local sources in one batch at 201^3",
+            "value": round(256 * a.steps / el, 2), "unit": "source-solves/s", "n_gpus": 1, "steps": a.steps,
+            "warmup": a.warmup, "ms_per_step": round(1e3 * el / a.steps, 1), "higher_is_better": True,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "201^3 (0-600 km, 15x15 deg), 25^3 vertices +-3 %; 192 plane waves on bottom + entry "
+                                   "side faces, 64 local sources (5-80 km deep, refinement 5 over 5-20 nodes); "
+                                   f"{a.nrc} receivers; receiver times + Frechet rows"},
+            "solve_wall_s": med("solve_wall_s"), "device_solve_ms": med("device_solve_ms"),
+            "sweeps": runs[-1]["sweeps"], "frechet_entries": runs[-1]["entries"],
+            "max_receiver_err_s_vs_fp64": err, "fp64_step_s": round(r64["step_s"], 3),
+            "tele_only": {k: round(tele_only[k], 3) for k in ("solve_wall_s", "device_solve_ms", "step_s")} | {"sweeps": tele_only["sweeps"]},
+            "point_only": {k: round(point_only[k], 3) for k in ("solve_wall_s", "device_solve_ms", "step_s")} | {"sweeps": point_only["sweeps"]}}
+
+
+def serving_mix(a):
+    """The serving loop under a Poisson stream of uneven requests: grid size
+    uniform over {65, 129, 201, 257}^3 (0-600 km, 15x15 deg, 25^3 vertices),
+    two +-5 % velocity models per size that requests reuse (as successive
+    inversion iterations would), 8-128 sources per request, three requests
+    in four teleseismic (bottom faces) and one local (5-80 km deep,
+    refinement 5 over 5 nodes), 100 receivers per grid, float32.  Reports
+    p50/p99 request latency and the sustained source-solves/s; a sample of
+    requests is checked against float64 solves of the same sources (1 ms)."""
+    rng = np.random.default_rng(5)
+    sizes = (65, 129, 201, 257)
+    cases = {}
+    for n in sizes:
+        g = synth.make_grid((n, n, n), 600.0, (15.0, 15.0), nv=(25, 25, 25))
+        tele = synth.tele_sources(g, 32, 5)
+        cases[n] = {"g": g, "velv": [synth.make_velv(g, 0.05, 5 + k) for k in range(2)],
+                    "faces": np.stack([synth.tele_bottom_face(g, s) for s in tele]),
+                    "src": np.stack([synth.point_source_rad(g, s) for s in synth.local_sources(g, 32, 5)]),
+                    "rcv": synth.receivers_rad(synth.receivers(g, 100, 5))}
+    n_req = max(8, int(a.serve_rate * a.serve_s))
+    gaps = rng.exponential(1.0 / a.serve_rate, n_req)
+    reqs = []
+    for i in range(n_req):
+        n = sizes[rng.integers(0, 4)]
+        m = int(rng.integers(0, 2))
+        k = int(rng.integers(8, 129))
+        pick = rng.integers(0, 32, k)
+        reqs.append((n, m, "tele" if rng.random() < 0.75 else "point", pick))
+    tickets = []
+    with lib.Server(hbm_budget=200e9, max_slots=128) as srv:
+        t0 = time.perf_counter()
+        nxt = t0
+        for i, (n, m, kind, pick) in enumerate(reqs):
+            nxt += gaps[i]
+            d = nxt - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            c = cases[n]
+            kw = {"faces": c["faces"][pick]} if kind == "tele" else {"src": c["src"][pick], "refine": 5, "radius": 5}
+            tickets.append(srv.submit(c["g"], c["velv"][m], 100 * n + m, rcv=c["rcv"], precision=lib.PREC_FP32, **kw))
+            if i % 10 == 9:
+                print(f"serving-mix: {i + 1} of {n_req} requests submitted", file=sys.stderr, flush=True)
+        lat, outs = [], []
+        for i, t in enumerate(tickets):
+            srv.wait(t, 900.0)
+            o, l = srv.result(t)
+            lat.append(l)
+            outs.append(o)
+            if i % 10 == 9:
+                print(f"serving-mix: {i + 1} of {n_req} requests done", file=sys.stderr, flush=True)
+        span = time.perf_counter() - t0
+        st = srv.stats()
+    # accuracy: the first request of each (size, kind) against float64 solves of its sources
+    err, seen = 0.0, set()
+    for (n, m, kind, pick), o in zip(reqs, outs):
+        if (n, kind) in seen:
+            continue
+        seen.add((n, kind))
+        c = cases[n]
+        with lib.Context(c["g"]) as ctx:
+            ctx.set_model(c["velv"][m])
+            if kind == "tele":
+                ctx.solve_tele(c["faces"][pick], lib.ORDER_MIXED, lib.PREC_FP64)
+            else:
+                ctx.solve_point(c["src"][pick], 5, 5, lib.ORDER_MIXED, lib.PREC_FP64)
+            ref = ctx.receivers(c["rcv"])
+        err = max(err, float(np.nanmax(np.abs(o.astype(np.float64) - ref))))
+    nsrc = int(sum(len(r[3]) for r in reqs))
+    return {"metric": "serving: source-solves/s and request latency, uneven grids and source kinds",
+            "value": round(nsrc / span, 2), "unit": "source-solves/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "Poisson requests over 65/129/201/257^3 grids (2 models each), 8-128 sources, "
+                                   "3/4 teleseismic (bottom faces) and 1/4 local (refinement 5 over 5 nodes), "
+                                   "100 receivers, float32",
+                       "requests": n_req, "offered_req_per_s": a.serve_rate, "sources": nsrc},
+            "p50_ms": round(float(np.percentile(lat, 50)), 1), "p99_ms": round(float(np.percentile(lat, 99)), 1),
+            "p50_ms_by_kind": {kd: round(float(np.percentile([l for l, r in zip(lat, reqs) if r[2] == kd], 50)), 1)
+                               for kd in ("tele", "point") if any(r[2] == kd for r in reqs)},
+            "span_s": round(span, 2), "models_resident_at_end": st["models_resident"],
+            "server_hbm_gb": round(st["hbm_bytes"] / 1e9, 2),
+            "max_receiver_err_s_vs_fp64": err}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (sources shard across GPUs "
               f"with no exchange, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "mixed256":
+        print(json.dumps(mixed256(a)), flush=True)
+        return
+    if a.workload == "serving-mix":
+        print(json.dumps(serving_mix(a)), flush=True)
+        return
     if a.copy_mib > 0:
         copy_calibration(a.copy_mib)
     g, velv, srcs, rcv_deg = synth.config_201(nsrc=a.batch, nrc=a.nrc, seed=2)

@@ -133,8 +133,9 @@ int fmm3d_solve_fixed(fmm3d_ctx *ctx, int nsrc, const int64_t *offs, const int64
  * `refine` per axis; the sub-grid gets B-spline velocities, straight-ray
  * times on the 8 corners of the fine cell holding the source and a float64
  * solve; the coarse nodes it covers then seed the coarse solve
- * (FMM3D_WALK_ALTERNATE).  Returns FMM3D_E_ARG for a source outside the
- * grid.  flags: 0. */
+ * (FMM3D_WALK_ALTERNATE).  Up to four refined blocks are solved at once, on
+ * worker threads with their own streams.  Returns FMM3D_E_ARG for a source
+ * outside the grid.  flags: 0. */
 int fmm3d_solve_point(fmm3d_ctx *ctx, int nsrc, const double *src, int refine, int radius,
                       int order, int precision, int flags);
 
@@ -194,7 +195,9 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *   "strategy"      0: brick sweeps in three phases; 1: column walk (default);
  *                   2: hyperplane walk (8 x 8 patches, lane (i, j) relaxes
  *                   layer s - i - j at step s: one Gauss-Seidel sweep in flow
- *                   order per launch, one update per node)
+ *                   order per launch, one update per node) for batches of
+ *                   teleseismic sources; batches holding point sources and
+ *                   the serving loop use the column walk
  *   "plane_chunk"   hyperplane walk: steps per staging chunk (2, 4 or 8; default 4)
  *   "plane_sweeps"  hyperplane walk: sweeps before the column walk takes over
  *                   (0, the default: hyperplane sweeps throughout)

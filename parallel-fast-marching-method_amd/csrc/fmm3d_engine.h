@@ -11,6 +11,7 @@
 #pragma once
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 #include "../../include/fmm3d.h"
@@ -30,9 +31,18 @@ FMM3D_HIDDEN int fmm3d_engine_collect(fmm3d_ctx* c, std::vector<int>& done, std:
 FMM3D_HIDDEN int fmm3d_engine_receivers(fmm3d_ctx* c, int slot, int nrc, const double* rcv, float* out);
 FMM3D_HIDDEN int fmm3d_engine_retire(fmm3d_ctx* c, int slot);
 FMM3D_HIDDEN int fmm3d_engine_active(const fmm3d_ctx* c);
-// fixed set of a point source (refined neighbourhood) / of a bottom face
-FMM3D_HIDDEN int fmm3d_engine_point_set(fmm3d_ctx* c, const double* src3, int refine, int radius, int order,
-                                        std::vector<int64_t>& node, std::vector<double>& tv);
+// A host-only view of a grid and vertex model (no device memory, no
+// stream): what the refined neighbourhood of a point source needs, for a
+// thread that prepares fixed sets beside the one that sweeps.
+FMM3D_HIDDEN fmm3d_ctx* fmm3d_engine_shadow(const fmm3d_grid_desc* g, const double* velv, int device);
+FMM3D_HIDDEN void fmm3d_engine_shadow_free(fmm3d_ctx* shadow);
+// fixed set of a point source (its refined neighbourhood, solved in
+// float64 on *fine, a refined-grid context created or resized here and
+// destroyed by the caller with fmm3d_destroy); reads only the grid, model
+// and device of c
+FMM3D_HIDDEN int fmm3d_engine_point_set(const fmm3d_ctx* c, fmm3d_ctx** fine, std::string& err, const double* src3,
+                                        int refine, int radius, int order, std::vector<int64_t>& node,
+                                        std::vector<double>& tv);
 // device bytes: the model and metric tables; one slot of a precision
 FMM3D_HIDDEN size_t fmm3d_engine_model_bytes(const fmm3d_grid_desc* g);
 FMM3D_HIDDEN size_t fmm3d_engine_slot_bytes(const fmm3d_grid_desc* g, int precision);

@@ -10,13 +10,18 @@
 // own stream, so contexts of different grids overlap on the device), then
 // collects the converged sources, reads their receiver times and frees
 // their slots.  Idle contexts are evicted, least recently used first, when
-// a new model needs the memory.
+// a new model needs the memory.  Point sources first need their refined
+// neighbourhood solved (float64, on a refined sub-grid): prep workers make
+// those fixed sets, each on its own refined-grid context and stream, while
+// the scheduler keeps sweeping, and point sources are admitted in order as
+// their sets are ready.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -35,6 +40,7 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr int SWEEPS_PER_CHUNK = 4;
+constexpr int PREP_WORKERS = 4;    // threads solving refined point-source neighbourhoods
 
 struct Request {
     int64_t ticket = 0;
@@ -46,7 +52,17 @@ struct Request {
     std::vector<float> times;
     int admitted = 0, done = 0;
     int status = FMM3D_OK;
-    bool finished = false;
+    std::atomic<bool> finished{false};     // set under the mutex, read by the scheduler without it
+    // point sources: fixed sets made by the prep workers (under the server
+    // mutex): next_prep is the next source to hand out, pready marks the
+    // finished ones and prepared counts the ready prefix the scheduler may
+    // admit; prep_busy counts workers holding the request
+    std::vector<std::vector<int64_t>> pnode;
+    std::vector<std::vector<double>> ptv;
+    std::vector<char> pready;
+    int next_prep = 0, prepared = 0, prep_busy = 0;
+    int retired = 0;                       // sources that left their slots (under the mutex)
+    bool released = false;
     Clock::time_point t_submit, t_done;
 };
 
@@ -77,8 +93,10 @@ struct fmm3d_server {
     double budget = 0.0;
     int max_slots = 64;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_prep;
     std::deque<Request*> queue;                          // not yet fully admitted
+    std::deque<Request*> prep_q;                         // point requests with sources to hand out
+    std::vector<std::thread> prep_th;                    // prep workers
     std::map<int64_t, std::unique_ptr<Request>> reqs;    // live tickets
     std::vector<std::unique_ptr<Model>> models;          // scheduler thread only
     int64_t next_ticket = 1;
@@ -94,6 +112,8 @@ struct fmm3d_server {
     int nmodels = 0;
 
     void run();
+    void prep();
+    void reap();
     Model* model_for(Request* r);
     void finish(Request* r, int status);
 };
@@ -156,28 +176,112 @@ Model* fmm3d_server::model_for(Request* r)
     return models.back().get();
 }
 
+// Drop released requests (their result was taken) that no thread still
+// holds: not queued for or in preparation, not in the admission queue, no
+// source left in a slot.  Called with the mutex held.
+void fmm3d_server::reap()
+{
+    for (auto it = reqs.begin(); it != reqs.end();) {
+        Request* r = it->second.get();
+        if (r->released && r->prep_busy == 0 && r->retired == r->admitted &&
+            std::find(queue.begin(), queue.end(), r) == queue.end() &&
+            std::find(prep_q.begin(), prep_q.end(), r) == prep_q.end())
+            it = reqs.erase(it);
+        else
+            ++it;
+    }
+}
+
+// A prep worker: fixed sets of point sources, one source at a time from the
+// oldest request with sources left, beside the scheduler's sweeps.  Each
+// worker has its own refined-grid context (and stream).
+void fmm3d_server::prep()
+{
+    // the scheduler may be capturing a sweep graph meanwhile
+    hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
+    (void)hipSetDevice(device);
+    fmm3d_ctx* fine = nullptr;
+    fmm3d_ctx* shadow = nullptr;
+    std::string shadow_key;
+    std::vector<int64_t> node;
+    std::vector<double> tv;
+    std::string err;
+    for (;;) {
+        Request* r = nullptr;
+        int q = -1;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_prep.wait(lk, [&] { return stop || !prep_q.empty(); });
+            if (stop) break;
+            r = prep_q.front();
+            if (r->finished || r->next_prep >= r->nsrc) {
+                prep_q.pop_front();
+                continue;
+            }
+            q = r->next_prep++;
+            if (r->next_prep >= r->nsrc) prep_q.pop_front();
+            r->prep_busy++;
+        }
+        // the grid and model view, kept while requests share them
+        const std::string key = model_key(*r);
+        if (!shadow || key != shadow_key) {
+            if (shadow) fmm3d_engine_shadow_free(shadow);
+            shadow = fmm3d_engine_shadow(&r->grid, r->velv.data(), device);
+            shadow_key = key;
+        }
+        node.clear();
+        tv.clear();
+        const int rc = fmm3d_engine_point_set(shadow, &fine, err, r->src.data() + 3 * q, r->refine, r->radius,
+                                              r->order, node, tv);
+        if (rc != FMM3D_OK) finish(r, rc);
+        std::lock_guard<std::mutex> lk(mu);
+        if (rc == FMM3D_OK) {
+            r->pnode[q].swap(node);
+            r->ptv[q].swap(tv);
+            r->pready[q] = 1;
+            while (r->prepared < r->nsrc && r->pready[r->prepared]) r->prepared++;
+            cv_work.notify_all();
+        }
+        r->prep_busy--;
+        reap();
+    }
+    if (shadow) fmm3d_engine_shadow_free(shadow);
+    if (fine) fmm3d_destroy(fine);
+}
+
 void fmm3d_server::run()
 {
+    hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
     (void)hipSetDevice(device);
     std::vector<int64_t> node;
     std::vector<double> tv;
     std::vector<int> done, failed;
     for (;;) {
         std::vector<Request*> waiting;
+        std::vector<int> ready;
         {
             std::unique_lock<std::mutex> lk(mu);
             if (stop) break;
-            if (queue.empty() && in_flight == 0)
-                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            // with nothing in flight, sleep until a source can be admitted
+            auto admissible = [&] {
+                for (Request* r : queue)
+                    if (!r->finished && (r->kind == FMM3D_SRC_TELE || r->prepared > r->admitted)) return true;
+                return false;
+            };
+            if (in_flight == 0) cv_work.wait(lk, [&] { return stop || admissible(); });
             if (stop) break;
             waiting.assign(queue.begin(), queue.end());
+            for (Request* r : waiting) ready.push_back(r->kind == FMM3D_SRC_TELE ? r->nsrc : r->prepared);
         }
         // admission: free slots of each request's model take its next sources
-        for (Request* r : waiting) {
-            if (r->finished) continue;
+        for (size_t wi = 0; wi < waiting.size(); ++wi) {
+            Request* r = waiting[wi];
+            if (r->finished || r->admitted >= ready[wi]) continue;
             Model* m = model_for(r);
             if (!m) continue;
-            for (int slot = 0; slot < m->cap && r->admitted < r->nsrc && !r->finished; ++slot) {
+            for (int slot = 0; slot < m->cap && r->admitted < ready[wi] && !r->finished; ++slot) {
                 if (m->owner[slot].first) continue;
                 const int q = r->admitted;
                 node.clear();
@@ -191,7 +295,9 @@ void fmm3d_server::run()
                         if (f[i] > 0.0) { node.push_back((int64_t)((r->grid.nnr - 1) * NL + i)); tv.push_back(f[i]); }
                 } else {
                     src3 = r->src.data() + 3 * q;
-                    rc = fmm3d_engine_point_set(m->ctx, src3, r->refine, r->radius, r->order, node, tv);
+                    std::lock_guard<std::mutex> lk(mu);
+                    node.swap(r->pnode[q]);
+                    tv.swap(r->ptv[q]);
                 }
                 if (rc == FMM3D_OK)
                     rc = fmm3d_engine_admit(m->ctx, slot, node.data(), tv.data(), (int64_t)node.size(),
@@ -199,8 +305,8 @@ void fmm3d_server::run()
                 if (rc != FMM3D_OK) { finish(r, rc); break; }
                 m->owner[slot] = { r, q };
                 m->in_flight++;
-                r->admitted++;
                 std::lock_guard<std::mutex> lk(mu);
+                r->admitted++;
                 in_flight++;
             }
             m->last_used = Clock::now();
@@ -210,6 +316,7 @@ void fmm3d_server::run()
             queue.erase(std::remove_if(queue.begin(), queue.end(),
                                        [](Request* r) { return r->finished || r->admitted >= r->nsrc; }),
                         queue.end());
+            reap();
             double used = 0.0;
             for (auto& m : models) used += m->bytes;
             hbm_used = used;
@@ -240,6 +347,7 @@ void fmm3d_server::run()
                         std::lock_guard<std::mutex> lk(mu);
                         in_flight--;
                         src_done++;
+                        r->retired++;
                         if (st != FMM3D_OK && r->status == FMM3D_OK) r->status = st;
                     }
                     if (++r->done == r->nsrc) finish(r, FMM3D_OK);
@@ -262,6 +370,7 @@ extern "C" int fmm3d_server_create(int device, double hbm_budget, int max_slots,
     s->budget = hbm_budget;
     s->max_slots = max_slots;
     s->th = std::thread([s] { s->run(); });
+    for (int w = 0; w < PREP_WORKERS; ++w) s->prep_th.emplace_back([s] { s->prep(); });
     *out = s;
     return FMM3D_OK;
 }
@@ -291,12 +400,21 @@ extern "C" int fmm3d_server_submit(fmm3d_server* s, const fmm3d_request* q, int6
     else r->src.assign(q->src, q->src + 3 * (size_t)q->nsrc);
     r->rcv.assign(q->rcv, q->rcv + 3 * (size_t)q->nrc);
     r->times.assign((size_t)q->nsrc * q->nrc, NAN);
+    if (q->kind == FMM3D_SRC_POINT) {
+        r->pnode.resize(q->nsrc);
+        r->ptv.resize(q->nsrc);
+        r->pready.assign(q->nsrc, 0);
+    }
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
     if (!s->any) { s->any = true; s->t_first = r->t_submit; }
     r->ticket = s->next_ticket++;
     *ticket = r->ticket;
     s->queue.push_back(r.get());
+    if (r->kind == FMM3D_SRC_POINT) {
+        s->prep_q.push_back(r.get());
+        s->cv_prep.notify_all();
+    }
     s->reqs[r->ticket] = std::move(r);
     s->cv_work.notify_all();
     return FMM3D_OK;
@@ -311,7 +429,7 @@ extern "C" int fmm3d_server_wait(fmm3d_server* s, int64_t ticket, double timeout
     Request* r = it->second.get();
     if (r->finished) return FMM3D_OK;
     if (timeout_s <= 0.0) return FMM3D_E_TIMEOUT;
-    const bool ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] { return r->finished; });
+    const bool ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] { return r->finished.load(); });
     return ok ? FMM3D_OK : FMM3D_E_TIMEOUT;
 }
 
@@ -323,10 +441,14 @@ extern "C" int fmm3d_server_result(fmm3d_server* s, int64_t ticket, float* times
     if (it == s->reqs.end()) return FMM3D_E_ARG;
     Request* r = it->second.get();
     if (!r->finished) return FMM3D_E_TIMEOUT;
+    if (r->released) return FMM3D_E_ARG;
     if (times) memcpy(times, r->times.data(), r->times.size() * sizeof(float));
     if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
     const int st = r->status;
-    s->reqs.erase(it);
+    // a failed request can still have sources in slots or in preparation:
+    // it goes once no thread holds it
+    r->released = true;
+    s->reap();
     return st;
 }
 
@@ -361,8 +483,10 @@ extern "C" int fmm3d_server_destroy(fmm3d_server* s)
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
         s->cv_work.notify_all();
+        s->cv_prep.notify_all();
     }
     if (s->th.joinable()) s->th.join();
+    for (auto& t : s->prep_th) t.join();
     delete s;
     return FMM3D_OK;
 }
