@@ -87,3 +87,29 @@ def test_bad_requests_are_rejected():
     with lib.Server(hbm_budget=1e9, max_slots=2) as S:
         with pytest.raises(lib.FmmError):
             S.submit(a.g, a.velv, 1, faces=np.zeros((0, a.g.nnx, a.g.nnz)), rcv=a.rcv)
+
+
+def test_point_preparation_does_not_hold_up_other_requests():
+    """A large point request's refined blocks are solved beside the sweeps:
+    a teleseismic request submitted right after it finishes first, and
+    both give their solo results."""
+    a = G.load("tele_31x33x35_mixed")
+    g, velv, _, rcv = synth.tele_case((61, 61, 61), 400.0, (10.0, 10.0), (12, 12, 12), 0.05, 1, 20, 1)
+    rcv = synth.receivers_rad(rcv)
+    rng = np.random.default_rng(3)
+    pts = np.stack([[-rng.uniform(5.0, 60.0), g.rgox + rng.uniform(10, 50) * g.dnx, g.rgoz + rng.uniform(10, 50) * g.dnz]
+                    for _ in range(24)])
+    with lib.Server(hbm_budget=16e9, max_slots=8) as S:
+        tp = S.submit(g, velv, 1, src=pts, refine=5, radius=8, rcv=rcv)
+        tt = S.submit(a.g, a.velv, 2, faces=a.faces[:2], rcv=a.rcv)
+        assert S.wait(tt, 300.0)
+        assert not S.wait(tp, 0.0), "the point request finished before the teleseismic one"
+        got_t, _ = S.result(tt)
+        assert S.wait(tp, 600.0)
+        got_p, _ = S.result(tp)
+    assert np.nanmax(np.abs(got_t.astype(np.float64) - _solo(a.g, a.velv, a.rcv, faces=a.faces[:2]))) <= 1e-3
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        c.solve_point(pts, 5, 8, lib.ORDER_MIXED, lib.PREC_FP32)
+        ref = c.receivers(rcv)
+    assert np.nanmax(np.abs(got_p.astype(np.float64) - ref)) <= 1e-3
