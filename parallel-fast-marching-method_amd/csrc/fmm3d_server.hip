@@ -217,6 +217,7 @@ void fmm3d_server::prep()
             r = prep_q.front();
             if (r->finished || r->next_prep >= r->nsrc) {
                 prep_q.pop_front();
+                reap();   // a failed request may have been released meanwhile
                 continue;
             }
             q = r->next_prep++;
