@@ -14,15 +14,20 @@ Prints one JSON line.  Extra fields: the plain float64 solver's throughput
 on the same sources (one source per call), the float32 error against float64
 at the receivers, per-step latency percentiles, the relaxation kernel's
 roofline figure, the same figures with bottom-face injection only (the
-reference's own boundary condition), the serving loop, and a CPU baseline
-(the oracle's heap march from the same fixed nodes, one core).
+reference's own boundary condition), the serving loop, and CPU baselines:
+the reference program compiled from its sources (oracle/_ref) under
+mpirun -np 9 on 6 sources (when built), and the oracle's heap march from
+the same fixed nodes on one core.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -34,6 +39,9 @@ sys.path.insert(0, REPO)
 from fmm3d import lib, synth  # noqa: E402
 
 METRIC = "source-solves/s @200³, batched FP32 vs plain GPU at ≤1 ms err; p50/p99 latency"
+# the reference program compiled from its own sources (make -C oracle ref)
+REF_EXE = os.path.join(REPO, "oracle", "_ref", "fm3dt_parallel")
+MPIRUN = "/opt/conda/bin/mpirun"
 HBM_PEAK_GBS = 8000.0
 # algorithmic HBM bytes per node visit of the column walk: read the node's
 # time and slowness once, write its time once (float32)
@@ -55,6 +63,9 @@ def parse():
     p.add_argument("--plain", type=int, default=3, help="float64 single-source solves timed")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-sources", type=int, default=1)
+    p.add_argument("--cpu-port-only", action="store_true",
+                   help="skip the reference program (oracle/_ref under mpirun -np 9 on 6 sources, about 30 s on "
+                        "16 host cores) and report the oracle port on one core as the CPU baseline")
     p.add_argument("--serve-s", type=float, default=6.0,
                    help="seconds of Poisson arrivals for the serving measurement (0: skip)")
     p.add_argument("--workload", choices=("tele201", "mixed256", "serving-mix"), default="tele201",
@@ -154,6 +165,34 @@ def serving(g, velv, faces, rcv, seconds, t_src):
     return {"requests": n, "sources_per_request": per, "offered_req_per_s": round(rate, 2),
             "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
             "source_solves_per_s": round(n * per / span, 2), "server_hbm_gb": round(st["hbm_bytes"] / 1e9, 2)}, tickets, errs
+
+
+def cpu_baseline_reference(g, velv, srcs, rcv_deg):
+    """The reference program itself under `mpirun -np 9` (its hard-wired
+    2x2x2 layout plus the writer rank) on 6 sources of the workload -- the
+    fewest one receiver array takes -- with bottom faces, its own boundary
+    condition, the same receivers and Frechet rows; the whole run is timed.
+    None when the build or mpirun is missing or the run fails."""
+    if not (os.path.exists(REF_EXE) and os.path.exists(MPIRUN)):
+        return None
+    d = tempfile.mkdtemp(prefix="fmm3d_ref_")
+    try:
+        synth.write_case(d, g, velv, srcs[:6], rcv_deg, fom=1, cfd=1)
+        t0 = time.perf_counter()
+        r = subprocess.run([MPIRUN, "-np", "9", REF_EXE], cwd=d, capture_output=True, text=True, timeout=300)
+        dt = time.perf_counter() - t0
+        ok = r.returncode == 0 and os.path.getsize(os.path.join(d, "rtravel.out")) > 0
+    except (OSError, subprocess.SubprocessError):
+        ok = False
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    if not ok:
+        return None
+    return {"value": 6 / dt, "unit": "source-solves/s", "cores": 9, "kind": "reference",
+            "sample": f"6 plane-wave sources of the 201^3 workload with bottom faces (the reference's boundary "
+                      f"condition), {len(rcv_deg)} receivers, rtravel.out + frechet.out: fm3dt_parallel compiled "
+                      f"from its sources, mpirun -np 9 (2x2x2 ranks + writer) on {os.cpu_count()} visible host "
+                      f"cores, whole run {dt:.1f} s"}
 
 
 def cpu_baseline(g, velv, srcs, rcv, n):
@@ -329,9 +368,12 @@ def main():
     if a.workload == "serving-mix":
         print(json.dumps(serving_mix(a)), flush=True)
         return
+    g, velv, srcs, rcv_deg = synth.config_201(nsrc=a.batch, nrc=a.nrc, seed=2)
+    # the reference on the host cores, first: a child process started
+    # before this process touches the GPU
+    ref_cpu = None if (a.no_cpu or a.cpu_port_only) else cpu_baseline_reference(g, velv, srcs, rcv_deg)
     if a.copy_mib > 0:
         copy_calibration(a.copy_mib)
-    g, velv, srcs, rcv_deg = synth.config_201(nsrc=a.batch, nrc=a.nrc, seed=2)
     rcv = synth.receivers_rad(rcv_deg)
     ctx = lib.Context(g)
     ctx.set_model(velv)
@@ -426,7 +468,12 @@ def main():
         sv["max_abs_diff_vs_batch_s"] = err
         out["serving"] = sv
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(g, velv, srcs, rcv, a.cpu_sources)
+        port = cpu_baseline(g, velv, srcs, rcv, a.cpu_sources)
+        if ref_cpu is not None:
+            out["cpu_baseline"] = ref_cpu
+            out["cpu_baseline_port"] = port
+        else:
+            out["cpu_baseline"] = port
     print(json.dumps(out), flush=True)
 
 
