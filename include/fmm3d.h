@@ -215,14 +215,23 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *                   default 4); at a check, every source whose last sweep
  *                   changed nothing leaves the remaining sweeps
  *   "graph"         1 (default): replay each chunk of check_every sweeps
- *                   from a captured hipGraph
+ *                   from a captured hipGraph (batch solves; the serving
+ *                   engine launches directly)
  *   "autotune"      1: the first solve of a kind (precision, order, walk,
  *                   batch size class) times patch shapes 8x8/8x16/16x8/16x16
  *                   with 8 or 16 passes on the real batch and keeps the
  *                   fastest (stats.tile_x/tile_z/tile_r report the choice)
  *   "max_sweeps"    column walk: stop after this many sweeps (0 = at
  *                   convergence); a diagnostic, the field is then approximate
- *   "timing"        1: time every relaxation launch with events (stats.kernel_ms)
+ *   "timing"        1: time every relaxation launch with events (stats.kernel_ms);
+ *                   the sweeps are then launched directly, not from a graph
+ *   "verify"        column walk: after this sweep (default 1; 0 never) one
+ *                   read-only pass evaluates every node's update on the field
+ *                   as it stands: a source with no node that would move is
+ *                   at its fixed point and leaves the sweeps
+ *   "verify_patches" 1: the check also narrows the next sweep to the patches
+ *                   with such a node and their neighbours; 0: not; -1
+ *                   (default): in float32 only
  * Returns FMM3D_E_ARG for an unknown key or a value out of range. */
 int fmm3d_set_option(fmm3d_ctx *ctx, const char *key, double value);
 

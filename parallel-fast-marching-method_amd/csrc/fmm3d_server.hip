@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -118,6 +120,13 @@ struct fmm3d_server {
     void finish(Request* r, int status);
 };
 
+// Why a request failed, to stderr when FMM3D_SERVER_TRACE is set.
+static void trace_failure(const Request* r, int status, const char* what)
+{
+    static const bool on = getenv("FMM3D_SERVER_TRACE") != nullptr;
+    if (on) fprintf(stderr, "fmm3d server: request %lld: status %d: %s\n", (long long)r->ticket, status, what);
+}
+
 void fmm3d_server::finish(Request* r, int status)
 {
     std::lock_guard<std::mutex> lk(mu);
@@ -140,6 +149,11 @@ Model* fmm3d_server::model_for(Request* r)
         if (m->key == key) return m.get();
     const double mb = (double)fmm3d_engine_model_bytes(&r->grid);
     const double sb = (double)fmm3d_engine_slot_bytes(&r->grid, r->precision);
+    if (mb + sb > budget) {
+        // not even alone with one slot: waiting would never end
+        finish(r, FMM3D_E_NOMEM);
+        return nullptr;
+    }
     for (;;) {
         double used = 0.0;
         for (auto& m : models) used += m->bytes;
@@ -235,7 +249,10 @@ void fmm3d_server::prep()
         tv.clear();
         const int rc = fmm3d_engine_point_set(shadow, &fine, err, r->src.data() + 3 * q, r->refine, r->radius,
                                               r->order, node, tv);
-        if (rc != FMM3D_OK) finish(r, rc);
+        if (rc != FMM3D_OK) {
+            trace_failure(r, rc, err.c_str());
+            finish(r, rc);
+        }
         std::lock_guard<std::mutex> lk(mu);
         if (rc == FMM3D_OK) {
             r->pnode[q].swap(node);
@@ -303,7 +320,11 @@ void fmm3d_server::run()
                 if (rc == FMM3D_OK)
                     rc = fmm3d_engine_admit(m->ctx, slot, node.data(), tv.data(), (int64_t)node.size(),
                                             r->kind == FMM3D_SRC_POINT, src3);
-                if (rc != FMM3D_OK) { finish(r, rc); break; }
+                if (rc != FMM3D_OK) {
+                    trace_failure(r, rc, fmm3d_last_error(m->ctx));
+                    finish(r, rc);
+                    break;
+                }
                 m->owner[slot] = { r, q };
                 m->in_flight++;
                 std::lock_guard<std::mutex> lk(mu);
@@ -328,6 +349,7 @@ void fmm3d_server::run()
             m->launched = false;
             if (m->in_flight == 0) continue;
             if (fmm3d_engine_launch(m->ctx, SWEEPS_PER_CHUNK) == FMM3D_OK) m->launched = true;
+            else if (getenv("FMM3D_SERVER_TRACE")) fprintf(stderr, "fmm3d server: launch: %s\n", fmm3d_last_error(m->ctx));
         }
         // ... then the converged sources leave their slots
         for (auto& m : models) {
@@ -341,6 +363,7 @@ void fmm3d_server::run()
                     if (!r) continue;
                     int st = pass == 0 ? fmm3d_engine_receivers(m->ctx, slot, r->nrc, r->rcv.data(), r->times.data() + (size_t)q * r->nrc)
                                        : FMM3D_E_NOCONV;
+                    if (st != FMM3D_OK) trace_failure(r, st, pass == 0 ? fmm3d_last_error(m->ctx) : "no convergence");
                     fmm3d_engine_retire(m->ctx, slot);
                     m->owner[slot] = { nullptr, -1 };
                     m->in_flight--;

@@ -393,6 +393,12 @@ template <typename R> struct ColArgs {
     int down;               // 1: walk top to bottom (when walkalt is null)
     const int* iterp;       // sweep number from device memory (graph replays), or null: iter
     const int* walkalt;     // per slot: 1 = alternate the walk (down on even sweeps), or null
+    int px, pz;             // patch shape (nodes)
+    int verify;             // check the fixed point after this sweep (0: never)
+    int verify_patches;     // the check also narrows the next sweep to the moving patches
+    int* pend;              // per slot per patch: a node would move (fixed-point check)
+    int* slotpend;          // per slot: some node would move
+    const int* born;        // per slot: its first sweep, checked after it (the serving engine), or null
 };
 
 // float32 helpers for the axis-sum octant form: fused multiply-subtract,
@@ -698,6 +704,82 @@ k_column(ColArgs<R> a)
         st[b] = iter;
         atomicMax(a.lastchg + slot, iter);
         atomicOr(a.valchg + slot, 1);
+    }
+}
+
+// Fixed-point check after sweep a.verify of the column walk.  One thread
+// per free node evaluates the node update on the field as it stands -- the
+// walk's own col_update on the same stencil, in the same arithmetic -- and
+// a node whose value would move marks its patch and its slot.  Nothing is
+// written to the field.  At a fixed point no node moves whatever the order,
+// so a slot with no marked node is where any further sweep would leave it.
+template <typename R, bool FAST, bool SHIFT>
+__global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
+{
+    const int iter = a.iterp ? __hip_atomic_load(a.iterp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : a.iter;
+    if (!a.born && iter != a.verify) return;
+    const size_t NL = (size_t)a.nnx * a.nnz, N = NL * a.nnr, NW = (N + 31) / 32;
+    const int np = a.npx * a.npz;
+    const R inf = Lim<R>::inf();
+    const R hr = a.hr, ihr = R(1) / a.hr;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)a.nslots * N;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int slot = (int)(i / N);
+        const size_t c = i - (size_t)slot * N;
+        if (!a.slot_on[slot] || (a.born && a.born[slot] != iter)) continue;
+        if ((a.fixbits[(size_t)slot * NW + (c >> 5)] >> (c & 31)) & 1u) continue;
+        const R* T = a.T + (size_t)slot * N;
+        const int ir = (int)(c / NL);
+        const size_t f = c - (size_t)ir * NL;
+        const int ix = (int)(f / a.nnz), iz = (int)(f - (size_t)ix * a.nnz);
+        const R tc = T[c];
+        R v1[6], v2[6];
+        bool in1[6], in2[6];
+        in1[0] = ir >= 1; in2[0] = ir >= 2;
+        in1[1] = ir + 1 < a.nnr; in2[1] = ir + 2 < a.nnr;
+        in1[2] = ix >= 1; in2[2] = ix >= 2;
+        in1[3] = ix + 1 < a.nnx; in2[3] = ix + 2 < a.nnx;
+        in1[4] = iz >= 1; in2[4] = iz >= 2;
+        in1[5] = iz + 1 < a.nnz; in2[5] = iz + 2 < a.nnz;
+        // side 0 the layer above (ir-1), side 1 the one below (ir+1); out of
+        // the grid reads as inf, as the walk's staging does
+        const ptrdiff_t st[3] = { (ptrdiff_t)NL, (ptrdiff_t)a.nnz, 1 };
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const ptrdiff_t d = (q & 1) ? st[q / 2] : -st[q / 2];
+            v1[q] = in1[q] ? T[(ptrdiff_t)c + d] : inf;
+            v2[q] = in2[q] ? T[(ptrdiff_t)c + 2 * d] : inf;
+        }
+        const R hxl = a.hx[ir], hzl = a.hz[(size_t)ir * a.nnx + ix];
+        const R h[3] = { hr, hxl, hzl };
+        const R ih[3] = { ihr, R(1) / hxl, R(1) / hzl };
+        const R nv = col_update<R, FAST, SHIFT>(a.fom, tc, v1, v2, in1, in2, h, ih, a.slo[c]);
+        if (nv != tc) {
+            a.pend[(size_t)slot * np + (ix / a.px) * a.npz + iz / a.pz] = 1;
+            a.slotpend[slot] = 1;
+        }
+    }
+}
+
+// After the check: a slot with no marked node leaves the sweeps; in the
+// others a patch with no marked node takes the stamp of the sweep before,
+// so the next sweep relaxes the marked patches and their neighbours only
+// (a patch runs when it or a neighbour changed in the sweep before).
+template <typename R>
+__global__ void __launch_bounds__(256) k_verify_apply(ColArgs<R> a, int* on)
+{
+    const int iter = a.iterp ? __hip_atomic_load(a.iterp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : a.iter;
+    if (!a.born && iter != a.verify) return;
+    const int np = a.npx * a.npz;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)a.nslots * np;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int slot = (int)(i / np);
+        if (!a.slot_on[slot] || (a.born && a.born[slot] != iter)) continue;
+        if (!a.slotpend[slot]) {
+            if (i == (size_t)slot * np) on[slot] = 0;
+            continue;
+        }
+        if (a.verify_patches && !a.pend[i]) a.stamp[i] = min(a.stamp[i], iter - 1);
     }
 }
 
@@ -1442,6 +1524,9 @@ struct Slots {
     int* flow = nullptr;      // column walk: per slot lateral flow signs
     double* srcpos = nullptr; // per slot point-source position (height, colat, lon), NaN if none
     int* walkalt = nullptr;   // per slot: alternate the column walk direction
+    int* pend = nullptr;      // column walk: per slot per patch, a node would move (fixed-point check)
+    int* slotpend = nullptr;  // column walk: per slot, some node would move
+    int* born = nullptr;      // serving engine: per slot, the sweep it joined at
 };
 
 }  // namespace
@@ -1474,6 +1559,11 @@ struct fmm3d_ctx {
     int use_graph = 1;             // replay sweep chunks from a captured hipGraph
     int ordered_until = 0;         // column walk: sweeps up to this one wait on upstream patches (0: all)
     int check_every = 4;           // column walk: sweeps per convergence check (one graph replay)
+    int verify = 1;                // column walk: fixed-point check after this sweep (0: none)
+    // the check narrows the next sweep to the moving patches (-1: float32
+    // only; in float64 the narrower sweep slowed propagation, 6 -> 17 sweeps
+    // on 50 plane waves at 201^3)
+    int verify_patches = -1;
     int autotune = 0;              // time candidate patch shapes / pass caps at first use
     std::vector<std::array<int, 8>> tuned;   // key (prec, order, walk, batch class) -> (px, pz, passes)
     int* d_iter = nullptr;         // device sweep counter (graph replays)
@@ -1713,6 +1803,8 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     if (at) c->autotune = atoi(at) != 0;
     const char* gv = getenv("FMM3D_GRAPH");
     if (gv) c->use_graph = atoi(gv) != 0;
+    const char* vf = getenv("FMM3D_VERIFY");
+    if (vf) c->verify = std::max(0, atoi(vf));
     const char* tr = getenv("FMM3D_TRACE");
     if (tr) c->trace = atoi(tr);
     const char* sg = getenv("FMM3D_STRATEGY");   // "brick" | "column[:PXxPZ[:passes]]" | "plane[:chunk]"
@@ -1741,6 +1833,7 @@ static void free_slots(Slots& s)
     (void)hipFree(s.T); (void)hipFree(s.P); (void)hipFree(s.fix); (void)hipFree(s.stamp);
     (void)hipFree(s.lastchg); (void)hipFree(s.on); (void)hipFree(s.toff);
     (void)hipFree(s.valchg); (void)hipFree(s.prog); (void)hipFree(s.flow); (void)hipFree(s.srcpos); (void)hipFree(s.walkalt);
+    (void)hipFree(s.pend); (void)hipFree(s.slotpend); (void)hipFree(s.born);
     s = Slots();
 }
 
@@ -1849,6 +1942,9 @@ static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
     HIPCHK(c, dalloc(&s.flow, (size_t)nsrc * 2 * sizeof(int)));
     HIPCHK(c, dalloc(&s.srcpos, (size_t)nsrc * 3 * sizeof(double)));
     HIPCHK(c, dalloc(&s.walkalt, (size_t)nsrc * sizeof(int)));
+    HIPCHK(c, dalloc(&s.pend, (size_t)nsrc * nbmax * sizeof(int)));
+    HIPCHK(c, dalloc(&s.slotpend, (size_t)nsrc * sizeof(int)));
+    HIPCHK(c, dalloc(&s.born, (size_t)nsrc * sizeof(int)));
     s.cap = nsrc;
     s.prec = prec;
     return FMM3D_OK;
@@ -2227,6 +2323,13 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     a.spin_limit = c->spin_limit;
     a.down = 0;
     a.iter = 1;
+    a.px = patch_x(c);
+    a.pz = patch_z(c);
+    a.verify = c->verify;
+    a.verify_patches = c->verify_patches < 0 ? !std::is_same<R, double>::value : c->verify_patches;
+    a.pend = s.pend;
+    a.slotpend = s.slotpend;
+    a.born = nullptr;
     blocks = 8 * ((nslots * npx + 7) / 8) * npz;   // rows of patches round-robin over 8 XCDs
     return FMM3D_OK;
 }
@@ -2245,21 +2348,44 @@ static bool column_supported(int px, int pz)
 // followed by k_next_iter, the sweep number living in c->d_iter; kernel
 // nodes only); other counts, and tracing, launch directly.  With timing on,
 // each launch (or the whole replay, shared out evenly) is timed with events.
+// The fixed-point check kernels after a column-walk launch (they act after
+// sweep a.verify only, read from the same sweep counter).
+template <typename R, bool FAST, bool SHIFT>
+static hipError_t launch_verify(fmm3d_ctx* c, const ColArgs<R>& a)
+{
+    const size_t n = (size_t)a.nslots * c->N;
+    k_verify_mark<R, FAST, SHIFT><<<(int)std::min<size_t>(8192, (n + 255) / 256), 256, 0, c->stream>>>(a);
+    const size_t m = (size_t)a.nslots * a.npx * a.npz;
+    k_verify_apply<R><<<(int)std::min<size_t>(1024, (m + 255) / 256), 256, 0, c->stream>>>(a, c->sl.on);
+    return hipGetLastError();
+}
+
 template <typename R, bool FAST, bool SHIFT>
 static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k, bool engine = false)
 {
     if (c->strategy != 2 && !column_supported(c->col_px, c->col_pz))
         return fail(c, FMM3D_E_ARG, "solver: unsupported column patch %dx%d", c->col_px, c->col_pz);
     const int kind = sweep_kernel(c, k, engine);
-    if (!c->use_graph || n != (engine ? FMM3D_CHECK : c->check_every) || c->trace ||
+    // timing: direct launches, each relaxation kernel between its own events
+    // the serving engine launches directly: its server runs refined
+    // point-source solves on other threads, and HIP refuses their
+    // synchronizing calls while any stream of the process captures (relaxed
+    // mode or not), which also voids the capture and leaves this stream
+    // stuck in it
+    if (!c->use_graph || c->timing || engine || n != c->check_every || c->trace ||
         kind != sweep_kernel(c, k + n - 1, engine)) {
         a.iterp = nullptr;
         for (int j = 0; j < n; ++j) {
             a.iter = k + j;
             if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j], c->stream));
-            hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks, sweep_kernel(c, k + j, engine));
+            const int kd = sweep_kernel(c, k + j, engine);
+            hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks, kd);
             HIPCHK(c, le);
             if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j + 1], c->stream));
+            if (kd == 1 && a.verify > 0) {
+                le = launch_verify<R, FAST, SHIFT>(c, a);
+                HIPCHK(c, le);
+            }
             c->stats.kernel_launches++;
             if (c->trace) {
                 unsigned long long v[2];
@@ -2288,13 +2414,14 @@ static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k, bo
     if (!c->gexec || sig != c->g_sig) {
         if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
         if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
-        // relaxed: threads driving other contexts (refined point-source
-        // solves, the serving loop's workers) keep synchronizing and
-        // allocating on their own streams while this one captures
+        // relaxed mode; still, no other thread of this library runs device
+        // work meanwhile (refined point-source workers have joined, the
+        // serving engine does not capture)
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
         hipError_t e = hipSuccess;
         for (int j = 0; j < n && e == hipSuccess; ++j) {
             e = launch_column<R, FAST, SHIFT>(c, a, blocks, kind);
+            if (e == hipSuccess && kind == 1 && a.verify > 0) e = launch_verify<R, FAST, SHIFT>(c, a);
             if (e == hipSuccess) {
                 k_next_iter<<<1, 64, 0, c->stream>>>(c->d_iter);
                 e = hipGetLastError();
@@ -2314,15 +2441,7 @@ static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k, bo
     }
     k_fill_int<<<1, 1, 0, c->stream>>>(c->d_iter, 1, k);
     HIPCHK(c, hipGetLastError());
-    if (c->timing) HIPCHK(c, hipEventRecord(c->kev[0], c->stream));
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
-    if (c->timing) {
-        HIPCHK(c, hipEventRecord(c->kev[1], c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        float ms = 0.0f;
-        HIPCHK(c, hipEventElapsedTime(&ms, c->kev[0], c->kev[1]));
-        c->stats.kernel_ms += ms;
-    }
     c->stats.kernel_launches += n;
     return FMM3D_OK;
 }
@@ -2345,6 +2464,8 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     const int maxit = 4 * (g.nnr + g.nnx + g.nnz) + 200;
     std::vector<int> lc(nsrc);
     k_stamp_init<<<std::min(4096, (blocks + 255) / 256), 256, 0, c->stream>>>(nsrc, np, 1, 1, 1, s.stamp);
+    k_fill_int<<<(nsrc * np + 255) / 256, 256, 0, c->stream>>>(s.pend, nsrc * np, 0);
+    k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.slotpend, nsrc, 0);
     k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.lastchg, nsrc, 0);
     k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.on, nsrc, 1);
     k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.valchg, nsrc, 0);
@@ -3400,6 +3521,9 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
         k_fill_int<<<1, 1, 0, c->stream>>>(s.lastchg + slot, 1, c->eng_k);
         k_fill_int<<<1, 1, 0, c->stream>>>(s.walkalt + slot, 1, walkalt ? 1 : 0);
         k_fill_int<<<1, 1, 0, c->stream>>>(s.on + slot, 1, 1);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.born + slot, 1, c->eng_k);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.slotpend + slot, 1, 0);
+        k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.pend + (size_t)slot * np, np, 0);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -3416,6 +3540,7 @@ static int engine_sweeps(fmm3d_ctx* c, int nsweeps)
     int blocks = 0;
     int rc = column_args<R>(c, c->sl.cap, c->eng_order, a, blocks);
     if (rc) return rc;
+    a.born = c->sl.born;   // each slot's fixed-point check follows its own first sweep
     rc = launch_chunk<R, FAST, SHIFT>(c, a, blocks, nsweeps, c->eng_k, true);
     if (rc) return rc;
     c->eng_k += nsweeps;
@@ -3566,6 +3691,12 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "ordered_sweeps")) {
         if (iv < 0 || iv > 1000000) return fail(c, FMM3D_E_ARG, "ordered_sweeps out of range");
         c->ordered_until = iv;
+    } else if (!strcmp(key, "verify_patches")) {
+        if (iv < -1 || iv > 1) return fail(c, FMM3D_E_ARG, "verify_patches must be -1, 0 or 1");
+        c->verify_patches = iv;
+    } else if (!strcmp(key, "verify")) {
+        if (iv < 0 || iv > 1000) return fail(c, FMM3D_E_ARG, "verify must be in 0..1000");
+        c->verify = iv;
     } else if (!strcmp(key, "check_every")) {
         // at most 8: the per-launch timing events hold one chunk of 8 launches
         if (iv < 1 || iv > 8) return fail(c, FMM3D_E_ARG, "check_every must be in 1..8");

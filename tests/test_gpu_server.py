@@ -87,6 +87,13 @@ def test_bad_requests_are_rejected():
     with lib.Server(hbm_budget=1e9, max_slots=2) as S:
         with pytest.raises(lib.FmmError):
             S.submit(a.g, a.velv, 1, faces=np.zeros((0, a.g.nnx, a.g.nnz)), rcv=a.rcv)
+    # a model that cannot fit the budget even alone fails instead of waiting
+    with lib.Server(hbm_budget=1e6, max_slots=2) as S:
+        t = S.submit(a.g, a.velv, 1, faces=a.faces[:1], rcv=a.rcv)
+        assert S.wait(t, 60.0)
+        with pytest.raises(lib.FmmError) as e:
+            S.result(t)
+        assert e.value.code == lib.E_NOMEM
 
 
 def test_point_preparation_does_not_hold_up_other_requests():
