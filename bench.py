@@ -50,7 +50,7 @@ BYTES_PER_NODE_VISIT = 12
 # rocprofv3 FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000), each scaled by a
 # 1 GiB copy, over the launches of one batch of 50
 # (profiles/k_column_tele201_batch50_traffic.md)
-MEASURED_BYTES_PER_NODE_VISIT = 34.91 + 4.39
+MEASURED_BYTES_PER_NODE_VISIT = 41.42 + 7.62
 
 
 def parse():

@@ -718,43 +718,58 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
 {
     const int iter = a.iterp ? __hip_atomic_load(a.iterp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : a.iter;
     if (!a.born && iter != a.verify) return;
+    // one thread per (slot, theta, phi) column walks the layers top down with
+    // the radial window (two above, the node, two below) in registers, so
+    // each value comes from HBM about once; lateral neighbours are loads of
+    // the same layer that adjacent threads share
     const size_t NL = (size_t)a.nnx * a.nnz, N = NL * a.nnr, NW = (N + 31) / 32;
     const int np = a.npx * a.npz;
     const R inf = Lim<R>::inf();
     const R hr = a.hr, ihr = R(1) / a.hr;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)a.nslots * N;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)a.nslots * NL;
          i += (size_t)gridDim.x * blockDim.x) {
-        const int slot = (int)(i / N);
-        const size_t c = i - (size_t)slot * N;
+        const int slot = (int)(i / NL);
+        const size_t f = i - (size_t)slot * NL;
         if (!a.slot_on[slot] || (a.born && a.born[slot] != iter)) continue;
-        if ((a.fixbits[(size_t)slot * NW + (c >> 5)] >> (c & 31)) & 1u) continue;
-        const R* T = a.T + (size_t)slot * N;
-        const int ir = (int)(c / NL);
-        const size_t f = c - (size_t)ir * NL;
         const int ix = (int)(f / a.nnz), iz = (int)(f - (size_t)ix * a.nnz);
-        const R tc = T[c];
-        R v1[6], v2[6];
-        bool in1[6], in2[6];
-        in1[0] = ir >= 1; in2[0] = ir >= 2;
-        in1[1] = ir + 1 < a.nnr; in2[1] = ir + 2 < a.nnr;
-        in1[2] = ix >= 1; in2[2] = ix >= 2;
-        in1[3] = ix + 1 < a.nnx; in2[3] = ix + 2 < a.nnx;
-        in1[4] = iz >= 1; in2[4] = iz >= 2;
-        in1[5] = iz + 1 < a.nnz; in2[5] = iz + 2 < a.nnz;
-        // side 0 the layer above (ir-1), side 1 the one below (ir+1); out of
-        // the grid reads as inf, as the walk's staging does
-        const ptrdiff_t st[3] = { (ptrdiff_t)NL, (ptrdiff_t)a.nnz, 1 };
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-            const ptrdiff_t d = (q & 1) ? st[q / 2] : -st[q / 2];
-            v1[q] = in1[q] ? T[(ptrdiff_t)c + d] : inf;
-            v2[q] = in2[q] ? T[(ptrdiff_t)c + 2 * d] : inf;
+        const R* T = a.T + (size_t)slot * N;
+        const uint32_t* fb = a.fixbits + (size_t)slot * NW;
+        bool moved = false;
+        // window w[0..4] = layers ir-2 .. ir+2 of this column
+        R w[5];
+        w[0] = inf; w[1] = inf;
+        w[2] = T[f];
+        w[3] = a.nnr > 1 ? T[NL + f] : inf;
+        w[4] = a.nnr > 2 ? T[2 * NL + f] : inf;
+        for (int ir = 0; ir < a.nnr; ++ir) {
+            const size_t c = (size_t)ir * NL + f;
+            if (!((fb[c >> 5] >> (c & 31)) & 1u)) {
+                R v1[6], v2[6];
+                bool in1[6], in2[6];
+                in1[0] = ir >= 1; in2[0] = ir >= 2;
+                in1[1] = ir + 1 < a.nnr; in2[1] = ir + 2 < a.nnr;
+                in1[2] = ix >= 1; in2[2] = ix >= 2;
+                in1[3] = ix + 1 < a.nnx; in2[3] = ix + 2 < a.nnx;
+                in1[4] = iz >= 1; in2[4] = iz >= 2;
+                in1[5] = iz + 1 < a.nnz; in2[5] = iz + 2 < a.nnz;
+                // side 0 the layer above (ir-1), side 1 the one below (ir+1);
+                // out of the grid reads as inf, as the walk's staging does
+                v1[0] = w[1]; v2[0] = w[0];
+                v1[1] = w[3]; v2[1] = w[4];
+                v1[2] = in1[2] ? T[c - a.nnz] : inf; v2[2] = in2[2] ? T[c - 2 * (size_t)a.nnz] : inf;
+                v1[3] = in1[3] ? T[c + a.nnz] : inf; v2[3] = in2[3] ? T[c + 2 * (size_t)a.nnz] : inf;
+                v1[4] = in1[4] ? T[c - 1] : inf; v2[4] = in2[4] ? T[c - 2] : inf;
+                v1[5] = in1[5] ? T[c + 1] : inf; v2[5] = in2[5] ? T[c + 2] : inf;
+                const R hxl = a.hx[ir], hzl = a.hz[(size_t)ir * a.nnx + ix];
+                const R h[3] = { hr, hxl, hzl };
+                const R ih[3] = { ihr, R(1) / hxl, R(1) / hzl };
+                const R nv = col_update<R, FAST, SHIFT>(a.fom, w[2], v1, v2, in1, in2, h, ih, a.slo[c]);
+                if (nv != w[2]) moved = true;
+            }
+            w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
+            w[4] = ir + 3 < a.nnr ? T[c + 3 * NL] : inf;
         }
-        const R hxl = a.hx[ir], hzl = a.hz[(size_t)ir * a.nnx + ix];
-        const R h[3] = { hr, hxl, hzl };
-        const R ih[3] = { ihr, R(1) / hxl, R(1) / hzl };
-        const R nv = col_update<R, FAST, SHIFT>(a.fom, tc, v1, v2, in1, in2, h, ih, a.slo[c]);
-        if (nv != tc) {
+        if (moved) {
             a.pend[(size_t)slot * np + (ix / a.px) * a.npz + iz / a.pz] = 1;
             a.slotpend[slot] = 1;
         }
@@ -2353,8 +2368,8 @@ static bool column_supported(int px, int pz)
 template <typename R, bool FAST, bool SHIFT>
 static hipError_t launch_verify(fmm3d_ctx* c, const ColArgs<R>& a)
 {
-    const size_t n = (size_t)a.nslots * c->N;
-    k_verify_mark<R, FAST, SHIFT><<<(int)std::min<size_t>(8192, (n + 255) / 256), 256, 0, c->stream>>>(a);
+    const size_t n = (size_t)a.nslots * a.nnx * a.nnz;   // one thread per column
+    k_verify_mark<R, FAST, SHIFT><<<(int)std::min<size_t>(65536, (n + 255) / 256), 256, 0, c->stream>>>(a);
     const size_t m = (size_t)a.nslots * a.npx * a.npz;
     k_verify_apply<R><<<(int)std::min<size_t>(1024, (m + 255) / 256), 256, 0, c->stream>>>(a, c->sl.on);
     return hipGetLastError();
