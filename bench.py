@@ -451,6 +451,8 @@ def main():
                      "avg_launch_ms": round(avg_launch_ms, 4),
                      "node_visits_per_launch": int(nvis / max(launches, 1)),
                      "algorithmic_bytes_per_node_visit": BYTES_PER_NODE_VISIT,
+                     # one source-solve's algorithmic bytes (every node once) at the measured rate
+                     "useful_work_gbs": round(g.nnr * g.nnx * g.nnz * BYTES_PER_NODE_VISIT * value / 1e9, 1),
                      "node_updates_per_s": round(upd / (kms * 1e-3), 1) if kms > 0 else None},
         "bottom_faces_only": {"value": round(bottom["value"], 3),
                               "ms_per_step": round(1e3 * bottom["elapsed"] / a.steps, 2),
