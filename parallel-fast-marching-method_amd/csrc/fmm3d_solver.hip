@@ -1610,6 +1610,7 @@ struct fmm3d_ctx {
     size_t st_offs_n = 0, st_n = 0;
     double* d_velv = nullptr;      // the vertex model on the device (gridder input)
     size_t d_velv_n = 0;
+    int* d_bad = nullptr;          // first source with a NaN boundary time (or the source count)
     double* d_sides = nullptr;     // side faces of the last teleseismic solve: theta faces, then phi faces
     int* d_side_of = nullptr;      // per source: theta side, phi side (-1 none, 0 low index, 1 high)
     size_t d_sides_n = 0, d_side_of_n = 0;
@@ -1869,7 +1870,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     if (c->graph) (void)hipGraphDestroy(c->graph);
     (void)hipFree(c->d_iter);
     (void)hipFree(c->d_faces);
-    (void)hipFree(c->st_offs); (void)hipFree(c->st_node); (void)hipFree(c->st_tv); (void)hipFree(c->d_velv);
+    (void)hipFree(c->st_offs); (void)hipFree(c->st_node); (void)hipFree(c->st_tv); (void)hipFree(c->d_velv); (void)hipFree(c->d_bad);
     (void)hipFree(c->d_sides); (void)hipFree(c->d_side_of);
     (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv);
     free_slots(c->sl);
@@ -1983,6 +1984,47 @@ __global__ void k_init_tele(int nnr, int nnx, int nnz, int nslots, const double*
             if (fv > 0.0) v = std::is_same<R, double>::value ? (R)fv : (R)(fv - toff[s]);
         }
         T[i] = v;
+    }
+}
+
+// Per source (one block): the smallest positive boundary time over its
+// bottom face and its selected side faces -- the float32 offset, 0 when
+// none is positive -- and the lowest source index holding a NaN.
+__global__ void __launch_bounds__(256) k_tele_offsets(size_t F, size_t XF, size_t ZF, const double* __restrict__ faces,
+                                                      const double* __restrict__ xf, const double* __restrict__ zf,
+                                                      const int* __restrict__ side, double* __restrict__ toff,
+                                                      int* __restrict__ bad)
+{
+    __shared__ double sm[256];
+    __shared__ int sn[256];
+    const int s = blockIdx.x, t = threadIdx.x;
+    double m = __builtin_huge_val();
+    int nan = 0;
+    auto scan = [&](const double* p, size_t n) {
+        for (size_t i = t; i < n; i += blockDim.x) {
+            const double v = p[i];
+            nan |= (v != v);
+            m = (v > 0.0 && v < m) ? v : m;
+        }
+    };
+    scan(faces + (size_t)s * F, F);
+    if (side) {
+        if (side[2 * s] >= 0) scan(xf + (size_t)s * XF, XF);
+        if (side[2 * s + 1] >= 0) scan(zf + (size_t)s * ZF, ZF);
+    }
+    sm[t] = m;
+    sn[t] = nan;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) {
+            sm[t] = sm[t + o] < sm[t] ? sm[t + o] : sm[t];
+            sn[t] |= sn[t + o];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        toff[s] = sm[0] < __builtin_huge_val() ? sm[0] : 0.0;
+        if (sn[0]) atomicMin(bad, s);
     }
 }
 
@@ -2662,51 +2704,24 @@ static void lateral_flow(const Grid& g, const double* face, int out[2])
     flow_code(gx, gz, out);
 }
 
-// Smallest positive value of n doubles and whether any is NaN (a
-// branch-free, vectorisable scan).
-static double positive_min(const double* f, size_t n, int* nan)
-{
-    double m = __builtin_huge_val();
-    int bad = 0;
-    for (size_t i = 0; i < n; ++i) {
-        const double v = f[i];
-        bad |= (v != v);
-        m = (v > 0.0 && v < m) ? v : m;
-    }
-    *nan |= bad;
-    return m;
-}
-
-// Argument checks of teleseismic face arrays and the float32 offset of
-// each source (its boundary minimum).  side[2s], side[2s+1]: -1 none, 0
-// the low-index face, 1 the high-index one.
-static int tele_offsets(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, const double* xf,
-                        const double* zf, const int* side, std::vector<double>& toff)
+// Argument checks of teleseismic face arrays.  side[2s], side[2s+1]: -1
+// none, 0 the low-index face, 1 the high-index one.
+static int tele_check(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, const double* xf,
+                      const double* zf, const int* side)
 {
     if (nsrc <= 0 || !faces) return fail(c, FMM3D_E_ARG, "%s: need nsrc > 0 and faces", fn);
     if (xf && (!zf || !side)) return fail(c, FMM3D_E_ARG, "%s: need both side faces and the side indices", fn);
-    const Grid& g = c->g;
-    const size_t F = (size_t)g.nnx * g.nnz, XF = (size_t)g.nnr * g.nnz, ZF = (size_t)g.nnr * g.nnx;
-    toff.assign(nsrc, 0.0);
-    for (int s = 0; s < nsrc; ++s) {
-        int nan = 0;
-        double m = positive_min(faces + (size_t)s * F, F, &nan);
-        if (xf) {
+    if (xf)
+        for (int s = 0; s < nsrc; ++s)
             if (side[2 * s] < -1 || side[2 * s] > 1 || side[2 * s + 1] < -1 || side[2 * s + 1] > 1)
                 return fail(c, FMM3D_E_ARG, "%s: side indices of source %d must be -1, 0 or 1", fn, s);
-            if (side[2 * s] >= 0) m = std::min(m, positive_min(xf + (size_t)s * XF, XF, &nan));
-            if (side[2 * s + 1] >= 0) m = std::min(m, positive_min(zf + (size_t)s * ZF, ZF, &nan));
-        }
-        if (nan) return fail(c, FMM3D_E_ARG, "%s: NaN boundary time (source %d)", fn, s);
-        toff[s] = isfinite(m) ? m : 0.0;
-    }
     return FMM3D_OK;
 }
 
 // Initial fields, fixed bits, offsets and lateral flows of slots
 // 0 .. nsrc-1 from teleseismic faces (slots must be allocated).
-static int tele_upload(fmm3d_ctx* c, int nsrc, const double* faces, const double* xf, const double* zf,
-                       const int* side, int precision, const std::vector<double>& toff)
+static int tele_upload(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, const double* xf,
+                       const double* zf, const int* side, int precision)
 {
     const Grid& g = c->g;
     Slots& s = c->sl;
@@ -2720,12 +2735,8 @@ static int tele_upload(fmm3d_ctx* c, int nsrc, const double* faces, const double
     }
     double* dfaces = c->d_faces;
     HIPCHK(c, hipMemcpyAsync(dfaces, faces, (size_t)nsrc * F * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(s.toff, toff.data(), nsrc * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    if (precision == FMM3D_PREC_FP64)
-        k_init_tele<double><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (double*)s.T, s.toff);
-    else
-        k_init_tele<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff);
-    k_fixbits_tele<<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, s.fix);
+    double* dx = nullptr;
+    double* dz = nullptr;
     if (xf) {
         const size_t nside = (size_t)nsrc * (XF + ZF);
         if (c->d_sides_n < nside) {
@@ -2742,11 +2753,26 @@ static int tele_upload(fmm3d_ctx* c, int nsrc, const double* faces, const double
             HIPCHK(c, dalloc(&c->d_side_of, (size_t)nsrc * 2 * sizeof(int)));
             c->d_side_of_n = (size_t)nsrc * 2;
         }
-        double* dx = c->d_sides;
-        double* dz = c->d_sides + (size_t)nsrc * XF;
+        dx = c->d_sides;
+        dz = c->d_sides + (size_t)nsrc * XF;
         HIPCHK(c, hipMemcpyAsync(dx, xf, (size_t)nsrc * XF * sizeof(double), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(dz, zf, (size_t)nsrc * ZF * sizeof(double), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->d_side_of, side, (size_t)nsrc * 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
+    // the float32 offsets (boundary minima) and the NaN check, on the device
+    if (!c->d_bad) HIPCHK(c, dalloc(&c->d_bad, sizeof(int)));
+    k_fill_int<<<1, 1, 0, c->stream>>>(c->d_bad, 1, nsrc);
+    k_tele_offsets<<<nsrc, 256, 0, c->stream>>>(F, XF, ZF, dfaces, dx, dz, xf ? c->d_side_of : nullptr, s.toff, c->d_bad);
+    int bad = nsrc;
+    HIPCHK(c, hipMemcpyAsync(&bad, c->d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (bad < nsrc) return fail(c, FMM3D_E_ARG, "%s: NaN boundary time (source %d)", fn, bad);
+    if (precision == FMM3D_PREC_FP64)
+        k_init_tele<double><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (double*)s.T, s.toff);
+    else
+        k_init_tele<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff);
+    k_fixbits_tele<<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, s.fix);
+    if (xf) {
         if (precision == FMM3D_PREC_FP64)
             k_init_sides<double><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dx, dz, c->d_side_of, s.toff, (double*)s.T, s.fix);
         else
@@ -2772,15 +2798,14 @@ static int solve_tele_impl(fmm3d_ctx* c, const char* fn, int nsrc, const double*
     if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "%s: bad precision %d", fn, precision);
     if (flags != 0) return fail(c, FMM3D_E_ARG, "%s: flags must be 0", fn);
     if (!c->have_model) return fail(c, FMM3D_E_STATE, "%s: no velocity model (call fmm3d_set_model_vtx)", fn);
-    std::vector<double> toff;
-    int rc = tele_offsets(c, fn, nsrc, faces, xf, zf, side, toff);
+    int rc = tele_check(c, fn, nsrc, faces, xf, zf, side);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     rc = ensure_slots(c, nsrc, precision);
     if (rc) return rc;
     c->sl.n = 0;
     c->walk_alt = 0;
-    rc = tele_upload(c, nsrc, faces, xf, zf, side, precision, toff);
+    rc = tele_upload(c, fn, nsrc, faces, xf, zf, side, precision);
     if (rc) return rc;
     return solve_run(c, nsrc, order, precision);
 }
@@ -3181,12 +3206,12 @@ extern "C" int fmm3d_solve_mixed(fmm3d_ctx* c, int ntele, const double* faces, c
     if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "%s: bad precision %d", fn, precision);
     if (flags != 0) return fail(c, FMM3D_E_ARG, "%s: flags must be 0", fn);
     if (!c->have_model) return fail(c, FMM3D_E_STATE, "%s: no velocity model (call fmm3d_set_model_vtx)", fn);
-    std::vector<double> ttoff, ptoff;
+    std::vector<double> ptoff;
     std::vector<int64_t> offs, node;
     std::vector<double> tv;
     std::vector<int> pflow;
     int rc;
-    if (ntele > 0 && (rc = tele_offsets(c, fn, ntele, faces, xface, zface, side, ttoff))) return rc;
+    if (ntele > 0 && (rc = tele_check(c, fn, ntele, faces, xface, zface, side))) return rc;
     if (npoint > 0) {
         if ((rc = point_sets(c, fn, npoint, src, refine, radius, order, offs, node, tv))) return rc;
         if ((rc = fixed_prepare(c, fn, npoint, offs.data(), node.data(), tv.data(), ptoff, pflow))) return rc;
@@ -3195,7 +3220,7 @@ extern "C" int fmm3d_solve_mixed(fmm3d_ctx* c, int ntele, const double* faces, c
     const int nsrc = ntele + npoint;
     if ((rc = ensure_slots(c, nsrc, precision))) return rc;
     c->sl.n = 0;
-    if (ntele > 0 && (rc = tele_upload(c, ntele, faces, xface, zface, side, precision, ttoff))) return rc;
+    if (ntele > 0 && (rc = tele_upload(c, fn, ntele, faces, xface, zface, side, precision))) return rc;
     if (npoint > 0 && (rc = fixed_upload(c, fn, ntele, npoint, offs.data(), node.data(), tv.data(), precision, ptoff, pflow)))
         return rc;
     c->walk_slots.assign(nsrc, 0);
