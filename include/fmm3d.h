@@ -230,8 +230,9 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *                   as it stands: a source with no node that would move is
  *                   at its fixed point and leaves the sweeps
  *   "verify_patches" 1: the check also narrows the next sweep to the patches
- *                   with such a node and their neighbours; 0: not; -1
- *                   (default): in float32 only
+ *                   with such a node and their neighbours (teleseismic,
+ *                   flow-ordered sources); 0: not; -1 (default): in float32
+ *                   only
  * Returns FMM3D_E_ARG for an unknown key or a value out of range. */
 int fmm3d_set_option(fmm3d_ctx *ctx, const char *key, double value);
 

@@ -794,7 +794,9 @@ __global__ void __launch_bounds__(256) k_verify_apply(ColArgs<R> a, int* on)
             if (i == (size_t)slot * np) on[slot] = 0;
             continue;
         }
-        if (a.verify_patches && !a.pend[i]) a.stamp[i] = min(a.stamp[i], iter - 1);
+        // narrowing only for flow-ordered slots: alternating walks (point
+        // sources) need their changes carried across the whole slot
+        if (a.verify_patches && !(a.walkalt && a.walkalt[slot]) && !a.pend[i]) a.stamp[i] = min(a.stamp[i], iter - 1);
     }
 }
 
