@@ -1976,16 +1976,20 @@ template <typename R>
 __global__ void k_init_tele(int nnr, int nnx, int nnz, int nslots, const double* __restrict__ faces, R* __restrict__ T,
                             const double* __restrict__ toff)
 {
-    const size_t NL = (size_t)nnx * nnz, N = NL * nnr;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N * nslots; i += (size_t)gridDim.x * blockDim.x) {
-        size_t s = i / N, c = i % N;
-        size_t ir = c / NL, f = c % NL;
-        R v = Lim<R>::inf();
-        if (ir == (size_t)nnr - 1) {
-            double fv = faces[s * NL + f];
-            if (fv > 0.0) v = std::is_same<R, double>::value ? (R)fv : (R)(fv - toff[s]);
+    // one slot per grid row; the bottom layer is the index range from base
+    const size_t NL = (size_t)nnx * nnz, N = NL * nnr, base = N - NL;
+    for (size_t s = blockIdx.y; s < (size_t)nslots; s += gridDim.y) {
+        R* Ts = T + s * N;
+        const double* fs = faces + s * NL;
+        const double off = toff[s];
+        for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
+            R v = Lim<R>::inf();
+            if (c >= base) {
+                const double fv = fs[c - base];
+                if (fv > 0.0) v = std::is_same<R, double>::value ? (R)fv : (R)(fv - off);
+            }
+            Ts[c] = v;
         }
-        T[i] = v;
     }
 }
 
@@ -2032,18 +2036,23 @@ __global__ void __launch_bounds__(256) k_tele_offsets(size_t F, size_t XF, size_
 
 __global__ void k_fixbits_tele(int nnr, int nnx, int nnz, int nslots, const double* __restrict__ faces, uint32_t* __restrict__ fix)
 {
-    const size_t NL = (size_t)nnx * nnz, N = NL * nnr;
+    // one slot per grid row; only words reaching the bottom layer (nodes from
+    // base on) hold bits
+    const size_t NL = (size_t)nnx * nnz, N = NL * nnr, base = N - NL;
     const size_t NW = (N + 31) / 32;
-    for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < NW * nslots; w += (size_t)gridDim.x * blockDim.x) {
-        size_t s = w / NW, w0 = w % NW;
-        uint32_t bits = 0;
-        for (int b = 0; b < 32; ++b) {
-            size_t c = w0 * 32 + b;
-            if (c >= N) break;
-            if (c / NL != (size_t)nnr - 1) continue;
-            if (faces[s * NL + c % NL] > 0.0) bits |= 1u << b;
+    for (size_t s = blockIdx.y; s < (size_t)nslots; s += gridDim.y) {
+        const double* fs = faces + s * NL;
+        for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < NW; w += (size_t)gridDim.x * blockDim.x) {
+            uint32_t bits = 0;
+            if (w * 32 + 31 >= base) {
+                for (int b = 0; b < 32; ++b) {
+                    const size_t c = w * 32 + b;
+                    if (c >= N) break;
+                    if (c >= base && fs[c - base] > 0.0) bits |= 1u << b;
+                }
+            }
+            fix[s * NW + w] = bits;
         }
-        fix[w] = bits;
     }
 }
 
@@ -2770,10 +2779,10 @@ static int tele_upload(fmm3d_ctx* c, const char* fn, int nsrc, const double* fac
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (bad < nsrc) return fail(c, FMM3D_E_ARG, "%s: NaN boundary time (source %d)", fn, bad);
     if (precision == FMM3D_PREC_FP64)
-        k_init_tele<double><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (double*)s.T, s.toff);
+        k_init_tele<double><<<dim3(256, std::min(nsrc, 65535)), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (double*)s.T, s.toff);
     else
-        k_init_tele<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff);
-    k_fixbits_tele<<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, s.fix);
+        k_init_tele<float><<<dim3(256, std::min(nsrc, 65535)), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff);
+    k_fixbits_tele<<<dim3(64, std::min(nsrc, 65535)), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, s.fix);
     if (xf) {
         if (precision == FMM3D_PREC_FP64)
             k_init_sides<double><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dx, dz, c->d_side_of, s.toff, (double*)s.T, s.fix);
