@@ -23,6 +23,10 @@ FMM3D_HIDDEN int fmm3d_engine_open(fmm3d_ctx* c, int cap, int precision, int ord
 // its column walk alternates, and a point-source position (or null)
 FMM3D_HIDDEN int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double* tv, int64_t n,
                                     int walkalt, const double* src3);
+// a teleseismic source into a free slot from its bottom face (nnx*nnz
+// float64, phi fastest, values > 0 fixed); queued behind the previous
+// launch, no wait; `face` must stay valid until the next launch completes
+FMM3D_HIDDEN int fmm3d_engine_admit_face(fmm3d_ctx* c, int slot, const double* face);
 // queue `nsweeps` sweeps over the admitted slots on the context's stream
 FMM3D_HIDDEN int fmm3d_engine_launch(fmm3d_ctx* c, int nsweeps);
 // wait for the queued sweeps; slots that converged go to `done`, slots that

@@ -41,7 +41,7 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr int SWEEPS_PER_CHUNK = 4;
+constexpr int SWEEPS_PER_CHUNK = 4;   // default sweeps between collections (FMM3D_SERVER_CHUNK)
 constexpr int PREP_WORKERS = 4;    // threads solving refined point-source neighbourhoods
 
 struct Request {
@@ -102,6 +102,7 @@ struct fmm3d_server {
     std::map<int64_t, std::unique_ptr<Request>> reqs;    // live tickets
     std::vector<std::unique_ptr<Model>> models;          // scheduler thread only
     int64_t next_ticket = 1;
+    int chunk = SWEEPS_PER_CHUNK;                        // sweeps queued per context per loop iteration
     bool stop = false;
     std::thread th;
     // statistics (under mu)
@@ -307,19 +308,18 @@ void fmm3d_server::run()
                 int rc = FMM3D_OK;
                 const double* src3 = nullptr;
                 if (r->kind == FMM3D_SRC_TELE) {
+                    // straight from the face, queued behind the running sweeps
                     const size_t NL = (size_t)r->grid.nnx * r->grid.nnz;
-                    const double* f = r->faces.data() + (size_t)q * NL;
-                    for (size_t i = 0; i < NL; ++i)
-                        if (f[i] > 0.0) { node.push_back((int64_t)((r->grid.nnr - 1) * NL + i)); tv.push_back(f[i]); }
+                    rc = fmm3d_engine_admit_face(m->ctx, slot, r->faces.data() + (size_t)q * NL);
                 } else {
                     src3 = r->src.data() + 3 * q;
-                    std::lock_guard<std::mutex> lk(mu);
-                    node.swap(r->pnode[q]);
-                    tv.swap(r->ptv[q]);
+                    {
+                        std::lock_guard<std::mutex> lk(mu);
+                        node.swap(r->pnode[q]);
+                        tv.swap(r->ptv[q]);
+                    }
+                    rc = fmm3d_engine_admit(m->ctx, slot, node.data(), tv.data(), (int64_t)node.size(), 1, src3);
                 }
-                if (rc == FMM3D_OK)
-                    rc = fmm3d_engine_admit(m->ctx, slot, node.data(), tv.data(), (int64_t)node.size(),
-                                            r->kind == FMM3D_SRC_POINT, src3);
                 if (rc != FMM3D_OK) {
                     trace_failure(r, rc, fmm3d_last_error(m->ctx));
                     finish(r, rc);
@@ -348,7 +348,7 @@ void fmm3d_server::run()
         for (auto& m : models) {
             m->launched = false;
             if (m->in_flight == 0) continue;
-            if (fmm3d_engine_launch(m->ctx, SWEEPS_PER_CHUNK) == FMM3D_OK) m->launched = true;
+            if (fmm3d_engine_launch(m->ctx, chunk) == FMM3D_OK) m->launched = true;
             else if (getenv("FMM3D_SERVER_TRACE")) fprintf(stderr, "fmm3d server: launch: %s\n", fmm3d_last_error(m->ctx));
         }
         // ... then the converged sources leave their slots
@@ -393,6 +393,7 @@ extern "C" int fmm3d_server_create(int device, double hbm_budget, int max_slots,
     s->device = device;
     s->budget = hbm_budget;
     s->max_slots = max_slots;
+    if (const char* ch = getenv("FMM3D_SERVER_CHUNK")) s->chunk = std::min(8, std::max(1, atoi(ch)));
     s->th = std::thread([s] { s->run(); });
     for (int w = 0; w < PREP_WORKERS; ++w) s->prep_th.emplace_back([s] { s->prep(); });
     *out = s;
@@ -420,7 +421,11 @@ extern "C" int fmm3d_server_submit(fmm3d_server* s, const fmm3d_request* q, int6
     r->precision = q->precision;
     const size_t nv = (size_t)(g.nvr + 2) * (g.nvx + 2) * (g.nvz + 2);
     r->velv.assign(q->velv, q->velv + nv);
-    if (q->kind == FMM3D_SRC_TELE) r->faces.assign(q->faces, q->faces + (size_t)q->nsrc * g.nnx * g.nnz);
+    if (q->kind == FMM3D_SRC_TELE) {
+        r->faces.assign(q->faces, q->faces + (size_t)q->nsrc * g.nnx * g.nnz);
+        for (double v : r->faces)
+            if (v != v) return FMM3D_E_ARG;   // NaN boundary time
+    }
     else r->src.assign(q->src, q->src + 3 * (size_t)q->nsrc);
     r->rcv.assign(q->rcv, q->rcv + 3 * (size_t)q->nrc);
     r->times.assign((size_t)q->nsrc * q->nrc, NAN);

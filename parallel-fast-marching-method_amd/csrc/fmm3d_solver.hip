@@ -1619,6 +1619,11 @@ struct fmm3d_ctx {
     int64_t* eng_node = nullptr;   // device staging of an admission's fixed set
     double* eng_tv = nullptr;
     size_t eng_stage = 0;
+    double* eng_faces = nullptr;   // per slot bottom face of a teleseismic admission
+    std::vector<int> eng_flow;     // per slot lateral flow (host copy the uploads read from)
+    double* eng_rcv = nullptr;     // receiver staging (grows; no free on the serving path)
+    float* eng_out = nullptr;
+    size_t eng_rcv_n = 0;
     int ordered = 1;               // column walk: patches wait for their upstream neighbours
     int spin_limit = 20000;        // polls (~64 cycles each) before such a wait is given up
     int max_sweeps = 0;            // column walk: stop after this many sweeps (0: at convergence)
@@ -1874,7 +1879,8 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->d_faces);
     (void)hipFree(c->st_offs); (void)hipFree(c->st_node); (void)hipFree(c->st_tv); (void)hipFree(c->d_velv); (void)hipFree(c->d_bad);
     (void)hipFree(c->d_sides); (void)hipFree(c->d_side_of);
-    (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv);
+    (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv); (void)hipFree(c->eng_faces);
+    (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out);
     free_slots(c->sl);
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
     (void)hipFree(c->hx64); (void)hipFree(c->hz64); (void)hipFree(c->hx32); (void)hipFree(c->hz32);
@@ -3501,6 +3507,10 @@ int fmm3d_engine_open(fmm3d_ctx* c, int cap, int precision, int order)
     c->eng_pending = 0;
     c->eng_active.assign(cap, 0);
     c->eng_born.assign(cap, 0);
+    c->eng_flow.assign((size_t)cap * 2, 1);
+    (void)hipFree(c->eng_faces);
+    c->eng_faces = nullptr;
+    HIPCHK(c, dalloc(&c->eng_faces, (size_t)cap * c->g.nnx * c->g.nnz * sizeof(double)));
     s.n = cap;
     c->stats = fmm3d_stats{};
     return FMM3D_OK;
@@ -3584,6 +3594,52 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
     return FMM3D_OK;
 }
 
+// A teleseismic source into a free slot straight from its bottom face (the
+// batch solve's initial state for one slot), with no host node list and no
+// wait: the uploads and kernels queue behind the previous launch on the
+// context's stream, and the host arrays they read (the request's face, the
+// per-slot flow vector) outlive them.
+int fmm3d_engine_admit_face(fmm3d_ctx* c, int slot, const double* face)
+{
+    if (!c || !c->eng_open || slot < 0 || slot >= c->sl.cap || c->eng_active[slot] || !face)
+        return fail(c, FMM3D_E_ARG, "engine: bad admission");
+    Slots& s = c->sl;
+    const Grid& g = c->g;
+    const size_t NL = (size_t)g.nnx * g.nnz, N = c->N, NW = (N + 31) / 32;
+    const int npx = (g.nnx + patch_x(c) - 1) / patch_x(c), npz = (g.nnz + patch_z(c) - 1) / patch_z(c);
+    const int np = npx * npz;
+    lateral_flow(g, face, &c->eng_flow[2 * (size_t)slot]);
+    double* df = c->eng_faces + (size_t)slot * NL;
+    if (!c->d_bad) HIPCHK(c, dalloc(&c->d_bad, sizeof(int)));
+    hipError_t e = hipMemcpyAsync(df, face, NL * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s.flow + 2 * slot, &c->eng_flow[2 * (size_t)slot], 2 * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        // NaN faces are refused at submission; the flag is not read here
+        k_tele_offsets<<<1, 256, 0, c->stream>>>(NL, 0, 0, df, nullptr, nullptr, nullptr, s.toff + slot, c->d_bad);
+        if (s.prec == FMM3D_PREC_FP64)
+            k_init_tele<double><<<dim3(256, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, (double*)s.T + (size_t)slot * N, s.toff + slot);
+        else
+            k_init_tele<float><<<dim3(256, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, (float*)s.T + (size_t)slot * N, s.toff + slot);
+        k_fixbits_tele<<<dim3(64, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, s.fix + (size_t)slot * NW);
+        k_fill_real<double><<<1, 64, 0, c->stream>>>(s.srcpos + 3 * (size_t)slot, 3, __builtin_nan(""));
+        // every patch column of the new slot runs in the next sweep
+        k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.stamp + (size_t)slot * np, np, c->eng_k - 1);
+        k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.prog + (size_t)slot * np, np, 0);
+        k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.pend + (size_t)slot * np, np, 0);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.lastchg + slot, 1, c->eng_k);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.walkalt + slot, 1, 0);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.born + slot, 1, c->eng_k);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.slotpend + slot, 1, 0);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.on + slot, 1, 1);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "engine admit: %s", hipGetErrorString(e));
+    c->eng_active[slot] = 1;
+    c->eng_born[slot] = c->eng_k;
+    return FMM3D_OK;
+}
+
 template <typename R, bool FAST, bool SHIFT>
 static int engine_sweeps(fmm3d_ctx* c, int nsweeps)
 {
@@ -3634,10 +3690,16 @@ int fmm3d_engine_receivers(fmm3d_ctx* c, int slot, int nrc, const double* rcv, f
 {
     if (!c || !c->eng_open || slot < 0 || slot >= c->sl.cap || nrc < 0) return fail(c, FMM3D_E_ARG, "engine: bad receivers");
     if (nrc == 0) return FMM3D_OK;
-    double* drcv = nullptr;
-    float* dout = nullptr;
-    HIPCHK(c, dalloc(&drcv, (size_t)nrc * 3 * sizeof(double)));
-    HIPCHK(c, dalloc(&dout, (size_t)nrc * sizeof(float)));
+    if (c->eng_rcv_n < (size_t)nrc) {
+        (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out);
+        c->eng_rcv = nullptr; c->eng_out = nullptr; c->eng_rcv_n = 0;
+        const size_t want = std::max<size_t>(nrc, 4096);
+        HIPCHK(c, dalloc(&c->eng_rcv, want * 3 * sizeof(double)));
+        HIPCHK(c, dalloc(&c->eng_out, want * sizeof(float)));
+        c->eng_rcv_n = want;
+    }
+    double* drcv = c->eng_rcv;
+    float* dout = c->eng_out;
     hipError_t e = hipMemcpyAsync(drcv, rcv, (size_t)nrc * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
         if (c->sl.prec == FMM3D_PREC_FP64)
@@ -3650,7 +3712,6 @@ int fmm3d_engine_receivers(fmm3d_ctx* c, int slot, int nrc, const double* rcv, f
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)nrc * sizeof(float), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(drcv); (void)hipFree(dout);
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "engine receivers: %s", hipGetErrorString(e));
     return FMM3D_OK;
 }
