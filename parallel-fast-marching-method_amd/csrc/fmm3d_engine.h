@@ -33,6 +33,10 @@ FMM3D_HIDDEN int fmm3d_engine_launch(fmm3d_ctx* c, int nsweeps);
 // hit the sweep cap go to `failed`
 FMM3D_HIDDEN int fmm3d_engine_collect(fmm3d_ctx* c, std::vector<int>& done, std::vector<int>& failed);
 FMM3D_HIDDEN int fmm3d_engine_receivers(fmm3d_ctx* c, int slot, int nrc, const double* rcv, float* out);
+// receiver times of n slots in one launch and one wait: slot slots[k] at
+// its nrc[k] receivers rcv[k] into out[k]
+FMM3D_HIDDEN int fmm3d_engine_receivers_many(fmm3d_ctx* c, int n, const int* slots, const int* nrc,
+                                             const double* const* rcv, float* const* out);
 FMM3D_HIDDEN int fmm3d_engine_retire(fmm3d_ctx* c, int slot);
 FMM3D_HIDDEN int fmm3d_engine_active(const fmm3d_ctx* c);
 // A host-only view of a grid and vertex model (no device memory, no

@@ -277,6 +277,9 @@ void fmm3d_server::run()
     std::vector<int64_t> node;
     std::vector<double> tv;
     std::vector<int> done, failed;
+    std::vector<int> rslots, rnrc;
+    std::vector<const double*> rrcv;
+    std::vector<float*> rout;
     for (;;) {
         std::vector<Request*> waiting;
         std::vector<int> ready;
@@ -356,13 +359,23 @@ void fmm3d_server::run()
             if (!m->launched) continue;
             int rc = fmm3d_engine_collect(m->ctx, done, failed);
             if (rc != FMM3D_OK) { failed.clear(); done.clear(); }
+            // the converged slots' receiver times in one launch and one wait
+            rslots.clear(); rnrc.clear(); rrcv.clear(); rout.clear();
+            for (int slot : done) {
+                Request* r = m->owner[slot].first;
+                if (!r) continue;
+                rslots.push_back(slot);
+                rnrc.push_back(r->nrc);
+                rrcv.push_back(r->rcv.data());
+                rout.push_back(r->times.data() + (size_t)m->owner[slot].second * r->nrc);
+            }
+            const int rst = fmm3d_engine_receivers_many(m->ctx, (int)rslots.size(), rslots.data(), rnrc.data(),
+                                                        rrcv.data(), rout.data());
             for (int pass = 0; pass < 2; ++pass)
                 for (int slot : (pass == 0 ? done : failed)) {
                     Request* r = m->owner[slot].first;
-                    const int q = m->owner[slot].second;
                     if (!r) continue;
-                    int st = pass == 0 ? fmm3d_engine_receivers(m->ctx, slot, r->nrc, r->rcv.data(), r->times.data() + (size_t)q * r->nrc)
-                                       : FMM3D_E_NOCONV;
+                    int st = pass == 0 ? rst : FMM3D_E_NOCONV;
                     if (st != FMM3D_OK) trace_failure(r, st, pass == 0 ? fmm3d_last_error(m->ctx) : "no convergence");
                     fmm3d_engine_retire(m->ctx, slot);
                     m->owner[slot] = { nullptr, -1 };

@@ -1095,27 +1095,18 @@ __device__ __forceinline__ double field_at(const R* T, double toff, size_t i) { 
 template <>
 __device__ __forceinline__ double field_at<double>(const double* T, double, size_t i) { return T[i]; }
 
-// trilinear receiver time with a float32 accumulator (fm3dt_parallel.f90:7063-7106)
+// trilinear receiver time with a float32 accumulator (fm3dt_parallel.f90:7063-7106);
+// NaN outside the grid
 template <typename R>
-__global__ void k_receivers(Grid g, const R* __restrict__ T, const double* __restrict__ toff, int nslots,
-                            int nrc, const double* __restrict__ rcv, float* __restrict__ out)
+__device__ __forceinline__ float receiver_time(const Grid& g, const R* __restrict__ F, double to, double rr, double rx,
+                                               double rz)
 {
-    int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= nslots * nrc) return;
-    int s = gid / nrc, i = gid % nrc;
-    const size_t N = (size_t)g.nnr * g.nnx * g.nnz;
-    const R* F = T + (size_t)s * N;
-    double to = toff[s];
-    double rr = rcv[3 * i], rx = rcv[3 * i + 1], rz = rcv[3 * i + 2];
     int irr = (int)floor((g.rgor - rr) / g.dnr) + 1;
     int irx = (int)floor((rx - g.rgox) / g.dnx) + 1;
     int irz = (int)floor((rz - g.rgoz) / g.dnz) + 1;
     if (irx == g.nnx) irx--;
     if (irz == g.nnz) irz--;
-    if (irr < 1 || irr >= g.nnr || irx < 1 || irx >= g.nnx || irz < 1 || irz >= g.nnz) {
-        out[gid] = __builtin_nanf("");
-        return;
-    }
+    if (irr < 1 || irr >= g.nnr || irx < 1 || irx >= g.nnx || irz < 1 || irz >= g.nnz) return __builtin_nanf("");
     double drx = (rx - g.rgox) - (irx - 1) * g.dnx;
     double drz = (rz - g.rgoz) - (irz - 1) * g.dnz;
     double drr = (g.rgor - rr) - (irr - 1) * g.dnr;
@@ -1128,7 +1119,31 @@ __global__ void k_receivers(Grid g, const R* __restrict__ T, const double* __res
                 size_t idx = ((size_t)(irr - 1 + j) * g.nnx + (irx - 1 + k)) * g.nnz + (irz - 1 + l);
                 acc = (float)((double)acc + field_at<R>(F, to, idx) * w);
             }
-    out[gid] = acc;
+    return acc;
+}
+
+template <typename R>
+__global__ void k_receivers(Grid g, const R* __restrict__ T, const double* __restrict__ toff, int nslots,
+                            int nrc, const double* __restrict__ rcv, float* __restrict__ out)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= nslots * nrc) return;
+    int s = gid / nrc, i = gid % nrc;
+    const size_t N = (size_t)g.nnr * g.nnx * g.nnz;
+    out[gid] = receiver_time<R>(g, T + (size_t)s * N, toff[s], rcv[3 * i], rcv[3 * i + 1], rcv[3 * i + 2]);
+}
+
+// Receiver times of many (slot, receiver) items at once: item i reads slot
+// islot[i] at receiver rcv[3i..3i+2].
+template <typename R>
+__global__ void k_receivers_items(Grid g, const R* __restrict__ T, const double* __restrict__ toff, int n,
+                                  const int* __restrict__ islot, const double* __restrict__ rcv, float* __restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = islot[i];
+    const size_t N = (size_t)g.nnr * g.nnx * g.nnz;
+    out[i] = receiver_time<R>(g, T + (size_t)s * N, toff[s], rcv[3 * i], rcv[3 * i + 1], rcv[3 * i + 2]);
 }
 
 // ------------------------------------------------------------ rays + Frechet
@@ -1623,7 +1638,11 @@ struct fmm3d_ctx {
     std::vector<int> eng_flow;     // per slot lateral flow (host copy the uploads read from)
     double* eng_rcv = nullptr;     // receiver staging (grows; no free on the serving path)
     float* eng_out = nullptr;
+    int* eng_islot = nullptr;
     size_t eng_rcv_n = 0;
+    std::vector<double> eng_hrcv;  // host staging of a batched receiver read
+    std::vector<int> eng_hslot;
+    std::vector<float> eng_hout;
     int ordered = 1;               // column walk: patches wait for their upstream neighbours
     int spin_limit = 20000;        // polls (~64 cycles each) before such a wait is given up
     int max_sweeps = 0;            // column walk: stop after this many sweeps (0: at convergence)
@@ -1880,7 +1899,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->st_offs); (void)hipFree(c->st_node); (void)hipFree(c->st_tv); (void)hipFree(c->d_velv); (void)hipFree(c->d_bad);
     (void)hipFree(c->d_sides); (void)hipFree(c->d_side_of);
     (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv); (void)hipFree(c->eng_faces);
-    (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out);
+    (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out); (void)hipFree(c->eng_islot);
     free_slots(c->sl);
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
     (void)hipFree(c->hx64); (void)hipFree(c->hz64); (void)hipFree(c->hx32); (void)hipFree(c->hz32);
@@ -3686,18 +3705,69 @@ int fmm3d_engine_collect(fmm3d_ctx* c, std::vector<int>& done, std::vector<int>&
     return FMM3D_OK;
 }
 
+// Receiver staging of at least n items (grows; kept across calls).
+static hipError_t engine_rcv_stage(fmm3d_ctx* c, size_t n)
+{
+    if (c->eng_rcv_n >= n) return hipSuccess;
+    (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out); (void)hipFree(c->eng_islot);
+    c->eng_rcv = nullptr; c->eng_out = nullptr; c->eng_islot = nullptr; c->eng_rcv_n = 0;
+    const size_t want = std::max<size_t>(n, 4096);
+    hipError_t e = dalloc(&c->eng_rcv, want * 3 * sizeof(double));
+    if (e == hipSuccess) e = dalloc(&c->eng_out, want * sizeof(float));
+    if (e == hipSuccess) e = dalloc(&c->eng_islot, want * sizeof(int));
+    if (e == hipSuccess) c->eng_rcv_n = want;
+    return e;
+}
+
+int fmm3d_engine_receivers_many(fmm3d_ctx* c, int n, const int* slots, const int* nrc, const double* const* rcv,
+                                float* const* out)
+{
+    if (!c || !c->eng_open || n < 0) return fail(c, FMM3D_E_ARG, "engine: bad receivers");
+    size_t total = 0;
+    for (int k = 0; k < n; ++k) {
+        if (slots[k] < 0 || slots[k] >= c->sl.cap || nrc[k] < 0) return fail(c, FMM3D_E_ARG, "engine: bad receivers");
+        total += (size_t)nrc[k];
+    }
+    if (total == 0) return FMM3D_OK;
+    if (total > (size_t)INT32_MAX) return fail(c, FMM3D_E_ARG, "engine: too many receivers");
+    HIPCHK(c, engine_rcv_stage(c, total));
+    c->eng_hrcv.resize(total * 3);
+    c->eng_hslot.resize(total);
+    c->eng_hout.resize(total);
+    size_t o = 0;
+    for (int k = 0; k < n; ++k) {
+        memcpy(c->eng_hrcv.data() + 3 * o, rcv[k], (size_t)nrc[k] * 3 * sizeof(double));
+        std::fill(c->eng_hslot.begin() + o, c->eng_hslot.begin() + o + nrc[k], slots[k]);
+        o += nrc[k];
+    }
+    hipError_t e = hipMemcpyAsync(c->eng_rcv, c->eng_hrcv.data(), total * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->eng_islot, c->eng_hslot.data(), total * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        const int nb = (int)((total + 255) / 256);
+        if (c->sl.prec == FMM3D_PREC_FP64)
+            k_receivers_items<double><<<nb, 256, 0, c->stream>>>(c->g, (const double*)c->sl.T, c->sl.toff, (int)total,
+                                                                 c->eng_islot, c->eng_rcv, c->eng_out);
+        else
+            k_receivers_items<float><<<nb, 256, 0, c->stream>>>(c->g, (const float*)c->sl.T, c->sl.toff, (int)total,
+                                                                c->eng_islot, c->eng_rcv, c->eng_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(c->eng_hout.data(), c->eng_out, total * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "engine receivers: %s", hipGetErrorString(e));
+    o = 0;
+    for (int k = 0; k < n; ++k) {
+        memcpy(out[k], c->eng_hout.data() + o, (size_t)nrc[k] * sizeof(float));
+        o += nrc[k];
+    }
+    return FMM3D_OK;
+}
+
 int fmm3d_engine_receivers(fmm3d_ctx* c, int slot, int nrc, const double* rcv, float* out)
 {
     if (!c || !c->eng_open || slot < 0 || slot >= c->sl.cap || nrc < 0) return fail(c, FMM3D_E_ARG, "engine: bad receivers");
     if (nrc == 0) return FMM3D_OK;
-    if (c->eng_rcv_n < (size_t)nrc) {
-        (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out);
-        c->eng_rcv = nullptr; c->eng_out = nullptr; c->eng_rcv_n = 0;
-        const size_t want = std::max<size_t>(nrc, 4096);
-        HIPCHK(c, dalloc(&c->eng_rcv, want * 3 * sizeof(double)));
-        HIPCHK(c, dalloc(&c->eng_out, want * sizeof(float)));
-        c->eng_rcv_n = want;
-    }
+    HIPCHK(c, engine_rcv_stage(c, (size_t)nrc));
     double* drcv = c->eng_rcv;
     float* dout = c->eng_out;
     hipError_t e = hipMemcpyAsync(drcv, rcv, (size_t)nrc * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
