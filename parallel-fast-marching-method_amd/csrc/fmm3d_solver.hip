@@ -1177,10 +1177,12 @@ struct RayOut {
 // its B-spline weights and velocity open the step's first sub-segment) or a
 // sub-segment's end (kind 1: the taps of the sub-segment from the previous
 // sample to this one, over length dinc).  The weights are taken in vertex
-// cell (ivr, ivx, ivz).
+// cell (ivr, ivx, ivz), at the sample's fractions (tr, tx, tz) of that
+// cell; ivel2 is 1 / velocity^2.  Both are formed once per sample here, not
+// on each of the 64 lanes that spread the sample's taps.
 struct RaySample {
-    double r, x, z;
-    double vel;
+    double tr, tx, tz;
+    double ivel2;
     double dinc;
     int ivr, ivx, ivz;
     int kind;
@@ -1223,7 +1225,11 @@ k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, co
     auto put = [&](double r, double x, double z, double vel, double dinc, int a, int b, int c, int kind) {
         if (ns >= P.cap) { over = true; return; }
         RaySample S;
-        S.r = r; S.x = x; S.z = z; S.vel = vel; S.dinc = dinc; S.ivr = a; S.ivx = b; S.ivz = c; S.kind = kind;
+        S.tr = ((g.gor - r) - (a - 1) * g.dvr) / g.dvr;
+        S.tx = ((x - g.gox) - (b - 1) * g.dvx) / g.dvx;
+        S.tz = ((z - g.goz) - (c - 1) * g.dvz) / g.dvz;
+        S.ivel2 = 1.0 / (vel * vel);
+        S.dinc = dinc; S.ivr = a; S.ivx = b; S.ivz = c; S.kind = kind;
         out[ns++] = S;
     };
     double r0 = rcv[3 * i], x0 = rcv[3 * i + 1], z0 = rcv[3 * i + 2];
@@ -1445,30 +1451,28 @@ k_ray_frechet(Grid g, int nrays, RayPaths P, float* __restrict__ scratch, RayOut
         const RaySample* S = P.rec + (int64_t)q * P.cap;
         int64_t acc_at = -1;   // scratch index of the vertex held in acc, -1 none
         float acc = 0.0f;
-        double vel = 1.0;
+        double ivel2 = 1.0;
         double pt[3] = { 0.0, 0.0, 0.0 };   // previous sample's B-spline fractions (r, theta, phi)
         double pw[3] = { 0.0, 0.0, 0.0 };   // its weights at offsets pn[], when pn[] >= 0
         int pn[3] = { -1, -1, -1 };
         RaySample mine;   // this lane's sample of the current batch of 64
-        mine.r = mine.x = mine.z = mine.vel = mine.dinc = 0.0;
+        mine.tr = mine.tx = mine.tz = mine.ivel2 = mine.dinc = 0.0;
         mine.ivr = mine.ivx = mine.ivz = mine.kind = 0;
         for (int j = 0; j < ns; ++j) {
             const int jb = j & 63;
             if (jb == 0) {
                 const int k = j + lane;
                 if (k < ns) mine = S[k];
-                else { mine.r = mine.x = mine.z = mine.vel = mine.dinc = 0.0; mine.ivr = mine.ivx = mine.ivz = mine.kind = 0; }
+                else { mine.tr = mine.tx = mine.tz = mine.ivel2 = mine.dinc = 0.0; mine.ivr = mine.ivx = mine.ivz = mine.kind = 0; }
             }
             RaySample sm;
-            sm.r = lane_double(mine.r, jb); sm.x = lane_double(mine.x, jb); sm.z = lane_double(mine.z, jb);
-            sm.vel = lane_double(mine.vel, jb); sm.dinc = lane_double(mine.dinc, jb);
+            sm.tr = lane_double(mine.tr, jb); sm.tx = lane_double(mine.tx, jb); sm.tz = lane_double(mine.tz, jb);
+            sm.ivel2 = lane_double(mine.ivel2, jb); sm.dinc = lane_double(mine.dinc, jb);
             sm.ivr = __builtin_amdgcn_readlane(mine.ivr, jb); sm.ivx = __builtin_amdgcn_readlane(mine.ivx, jb);
             sm.ivz = __builtin_amdgcn_readlane(mine.ivz, jb); sm.kind = __builtin_amdgcn_readlane(mine.kind, jb);
-            const double velo = vel;
-            vel = sm.vel;
-            const double tc[3] = { ((g.gor - sm.r) - (sm.ivr - 1) * g.dvr) / g.dvr,
-                                   ((sm.x - g.gox) - (sm.ivx - 1) * g.dvx) / g.dvx,
-                                   ((sm.z - g.goz) - (sm.ivz - 1) * g.dvz) / g.dvz };
+            const double ivel2o = ivel2;
+            ivel2 = sm.ivel2;
+            const double tc[3] = { sm.tr, sm.tx, sm.tz };
             if (sm.kind != 1) {
                 for (int a = 0; a < 3; ++a) { pt[a] = tc[a]; pn[a] = -1; }
                 continue;
@@ -1483,8 +1487,8 @@ k_ray_frechet(Grid g, int nrays, RayPaths P, float* __restrict__ scratch, RayOut
                 wc[a] = bspline_w(tc[a], off[a]);
                 wp[a] = pn[a] == off[a] ? pw[a] : bspline_w(pt[a], off[a]);
             }
-            const double e1 = wc[0] * wc[1] * wc[2] / (vel * vel);
-            const double e0 = wp[0] * wp[1] * wp[2] / (velo * velo);
+            const double e1 = wc[0] * wc[1] * wc[2] * ivel2;
+            const double e0 = wp[0] * wp[1] * wp[2] * ivel2o;
             for (int a = 0; a < 3; ++a) { pt[a] = tc[a]; pw[a] = wc[a]; pn[a] = off[a]; }
             const double inc = -(e1 + e0) * sm.dinc / 2.0;
             if (zz >= 0 && zz < NVZ && xx >= 0 && xx < NVX && rr >= 0 && rr < NVR) {
