@@ -2025,13 +2025,13 @@ __global__ void k_init_tele(int nnr, int nnx, int nnz, int nslots, const double*
 // Per source (one block): the smallest positive boundary time over its
 // bottom face and its selected side faces -- the float32 offset, 0 when
 // none is positive -- and the lowest source index holding a NaN.
-__global__ void __launch_bounds__(256) k_tele_offsets(size_t F, size_t XF, size_t ZF, const double* __restrict__ faces,
+__global__ void __launch_bounds__(1024) k_tele_offsets(size_t F, size_t XF, size_t ZF, const double* __restrict__ faces,
                                                       const double* __restrict__ xf, const double* __restrict__ zf,
                                                       const int* __restrict__ side, double* __restrict__ toff,
                                                       int* __restrict__ bad)
 {
-    __shared__ double sm[256];
-    __shared__ int sn[256];
+    __shared__ double sm[1024];
+    __shared__ int sn[1024];
     const int s = blockIdx.x, t = threadIdx.x;
     double m = __builtin_huge_val();
     int nan = 0;
@@ -2050,7 +2050,7 @@ __global__ void __launch_bounds__(256) k_tele_offsets(size_t F, size_t XF, size_
     sm[t] = m;
     sn[t] = nan;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = (int)blockDim.x / 2; o > 0; o >>= 1) {
         if (t < o) {
             sm[t] = sm[t + o] < sm[t] ? sm[t + o] : sm[t];
             sn[t] |= sn[t + o];
@@ -2479,7 +2479,8 @@ static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k, bo
             hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks, kd);
             HIPCHK(c, le);
             if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j + 1], c->stream));
-            if (kd == 1 && a.verify > 0) {
+            // engine slots join at a chunk's first sweep: their checks follow it
+            if (kd == 1 && a.verify > 0 && (!engine || j == 0)) {
                 le = launch_verify<R, FAST, SHIFT>(c, a);
                 HIPCHK(c, le);
             }
@@ -2802,7 +2803,7 @@ static int tele_upload(fmm3d_ctx* c, const char* fn, int nsrc, const double* fac
     // the float32 offsets (boundary minima) and the NaN check, on the device
     if (!c->d_bad) HIPCHK(c, dalloc(&c->d_bad, sizeof(int)));
     k_fill_int<<<1, 1, 0, c->stream>>>(c->d_bad, 1, nsrc);
-    k_tele_offsets<<<nsrc, 256, 0, c->stream>>>(F, XF, ZF, dfaces, dx, dz, xf ? c->d_side_of : nullptr, s.toff, c->d_bad);
+    k_tele_offsets<<<nsrc, 1024, 0, c->stream>>>(F, XF, ZF, dfaces, dx, dz, xf ? c->d_side_of : nullptr, s.toff, c->d_bad);
     int bad = nsrc;
     HIPCHK(c, hipMemcpyAsync(&bad, c->d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -3617,6 +3618,31 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
     return FMM3D_OK;
 }
 
+// A newly admitted slot's state: stamps (every patch column runs in sweep
+// k), progress and check marks cleared, last change k, walk mode, first
+// sweep k, switched on, no point-source position.
+__global__ void k_slot_setup(int slot, int np, int k, int walkalt, int* __restrict__ stamp, int* __restrict__ prog,
+                             int* __restrict__ pend, int* __restrict__ lastchg, int* __restrict__ walk,
+                             int* __restrict__ born, int* __restrict__ slotpend, int* __restrict__ on,
+                             double* __restrict__ srcpos)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) {
+        const size_t p = (size_t)slot * np + i;
+        stamp[p] = k - 1;
+        prog[p] = 0;
+        pend[p] = 0;
+    }
+    if (i == 0) {
+        lastchg[slot] = k;
+        walk[slot] = walkalt;
+        born[slot] = k;
+        slotpend[slot] = 0;
+        on[slot] = 1;
+    }
+    if (i < 3) srcpos[3 * (size_t)slot + i] = __builtin_nan("");
+}
+
 // A teleseismic source into a free slot straight from its bottom face (the
 // batch solve's initial state for one slot), with no host node list and no
 // wait: the uploads and kernels queue behind the previous launch on the
@@ -3639,22 +3665,16 @@ int fmm3d_engine_admit_face(fmm3d_ctx* c, int slot, const double* face)
         e = hipMemcpyAsync(s.flow + 2 * slot, &c->eng_flow[2 * (size_t)slot], 2 * sizeof(int), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
         // NaN faces are refused at submission; the flag is not read here
-        k_tele_offsets<<<1, 256, 0, c->stream>>>(NL, 0, 0, df, nullptr, nullptr, nullptr, s.toff + slot, c->d_bad);
+        k_tele_offsets<<<1, 1024, 0, c->stream>>>(NL, 0, 0, df, nullptr, nullptr, nullptr, s.toff + slot, c->d_bad);
         if (s.prec == FMM3D_PREC_FP64)
             k_init_tele<double><<<dim3(256, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, (double*)s.T + (size_t)slot * N, s.toff + slot);
         else
             k_init_tele<float><<<dim3(256, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, (float*)s.T + (size_t)slot * N, s.toff + slot);
         k_fixbits_tele<<<dim3(64, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, s.fix + (size_t)slot * NW);
-        k_fill_real<double><<<1, 64, 0, c->stream>>>(s.srcpos + 3 * (size_t)slot, 3, __builtin_nan(""));
-        // every patch column of the new slot runs in the next sweep
-        k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.stamp + (size_t)slot * np, np, c->eng_k - 1);
-        k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.prog + (size_t)slot * np, np, 0);
-        k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.pend + (size_t)slot * np, np, 0);
-        k_fill_int<<<1, 1, 0, c->stream>>>(s.lastchg + slot, 1, c->eng_k);
-        k_fill_int<<<1, 1, 0, c->stream>>>(s.walkalt + slot, 1, 0);
-        k_fill_int<<<1, 1, 0, c->stream>>>(s.born + slot, 1, c->eng_k);
-        k_fill_int<<<1, 1, 0, c->stream>>>(s.slotpend + slot, 1, 0);
-        k_fill_int<<<1, 1, 0, c->stream>>>(s.on + slot, 1, 1);
+        // the slot's per-patch and per-slot state in one launch; every patch
+        // column of the new slot runs in the next sweep
+        k_slot_setup<<<(np + 255) / 256, 256, 0, c->stream>>>(slot, np, c->eng_k, 0, s.stamp, s.prog, s.pend, s.lastchg,
+                                                             s.walkalt, s.born, s.slotpend, s.on, s.srcpos);
         e = hipGetLastError();
     }
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "engine admit: %s", hipGetErrorString(e));
