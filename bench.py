@@ -51,6 +51,12 @@ BYTES_PER_NODE_VISIT = 12
 # 1 GiB copy, over the launches of one batch of 50
 # (profiles/k_column_tele201_batch50_traffic.md)
 MEASURED_BYTES_PER_NODE_VISIT = 41.42 + 7.62
+# VALU instructions per float32 node update in the column walk's pass loop
+# (counted in the kernel's gfx950 assembly), and the non-packed wave64 VALU
+# issue rate of the device: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
+# instruction per 4 cycles of a 16-lane SIMD
+VALU_PER_UPDATE = 240
+VALU_WAVE_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 def parse():
@@ -453,7 +459,10 @@ def main():
                      "algorithmic_bytes_per_node_visit": BYTES_PER_NODE_VISIT,
                      # one source-solve's algorithmic bytes (every node once) at the measured rate
                      "useful_work_gbs": round(g.nnr * g.nnx * g.nnz * BYTES_PER_NODE_VISIT * value / 1e9, 1),
-                     "node_updates_per_s": round(upd / (kms * 1e-3), 1) if kms > 0 else None},
+                     "node_updates_per_s": round(upd / (kms * 1e-3), 1) if kms > 0 else None,
+                     # the kernel's actual bound: VALU issue (an estimate, not the contract's roofline)
+                     "valu_issue_frac_estimate": round(upd / (kms * 1e-3) * VALU_PER_UPDATE / 64
+                                                       / VALU_WAVE_ISSUE_PER_S, 3) if kms > 0 else None},
         "bottom_faces_only": {"value": round(bottom["value"], 3),
                               "ms_per_step": round(1e3 * bottom["elapsed"] / a.steps, 2),
                               "plain_fp64_source_solves_per_s": round(bottom["plain"], 3) if bottom["plain"] else None,
