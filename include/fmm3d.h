@@ -210,6 +210,12 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *                   flow) finished it; 0 no ordering
  *   "ordered_sweeps" column walk: only sweeps up to this one wait on their
  *                   upstream patches (default 0: every sweep)
+ *   "point_order"   sources whose walk alternates (point sources, and
+ *                   fmm3d_solve_fixed with FMM3D_WALK_ALTERNATE): 1 (default)
+ *                   their patches start outward from the one holding the
+ *                   earliest fixed node and wait on their neighbour towards
+ *                   it, as ordered sources wait along their flow; 0 no
+ *                   ordering.  Refined-block solves follow the setting
  *   "spin_limit"    polls before such a wait is given up (default 20000)
  *   "check_every"   column walk: sweeps between convergence checks (1..8,
  *                   default 4); at a check, every source whose last sweep

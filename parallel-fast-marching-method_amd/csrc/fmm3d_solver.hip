@@ -364,6 +364,12 @@ k_brick(SweepArgs<R> a)
 // (neighbouring patches walk the same layers concurrently), so a patch
 // column runs again in the next sweep while it or a lateral neighbour
 // changed.
+// Flow entries at or above FLOW_CENTER (alternating walks: point sources)
+// hold the (theta, phi) node index of the slot's earliest fixed node plus
+// FLOW_CENTER; the column walk then orders that slot's patches outward from
+// the patch holding it.  Smaller entries are lateral flow codes (+-1, +-2).
+constexpr int FLOW_CENTER = 1 << 24;
+
 template <typename R> struct ColArgs {
     R* T;
     const R* slo;
@@ -487,6 +493,21 @@ __device__ __forceinline__ R col_update(int fom, R tc, const R v1[6], const R v2
     return best;
 }
 
+// Patch index of launch rank r on an axis of n patches ordered outward from
+// patch s: s, s+1, s-1, s+2, s-2, ..., then the rest of the longer side.
+// A patch's neighbour towards s always has the lower rank.
+__device__ __forceinline__ int centre_rank(int r, int s, int n)
+{
+    const int lo = s, hi = n - 1 - s, m = min(lo, hi);
+    if (r == 0) return s;
+    if (r <= 2 * m) {
+        const int d = (r + 1) >> 1;
+        return (r & 1) ? s + d : s - d;
+    }
+    const int d = r - m;
+    return hi > lo ? s + d : s - d;
+}
+
 template <typename R, int PX, int PZ, bool FAST, bool SHIFT>
 __global__ void __launch_bounds__(64)
 k_column(ColArgs<R> a)
@@ -512,15 +533,31 @@ k_column(ColArgs<R> a)
     if (!a.slot_on[slot]) return;
     // sweep number: an argument, or a device counter (graph replays)
     const int iter = a.iterp ? __hip_atomic_load(a.iterp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : a.iter;
-    const int fx = a.flow && a.flow[2 * slot] < 0 ? -1 : 1, fz = a.flow && a.flow[2 * slot + 1] < 0 ? -1 : 1;
+    const int f0 = a.flow ? a.flow[2 * slot] : 1, f1 = a.flow ? a.flow[2 * slot + 1] : 1;
     const int rx = row - slot * a.npx, rz = k % a.npz;
-    const int bx = fx > 0 ? rx : a.npx - 1 - rx;
-    const int bz = fz > 0 ? rz : a.npz - 1 - rz;
+    // fx, fz: the direction a patch's wave arrives from along each axis
+    // (0: none, the patch is upstream of everything on that axis)
+    int bx, bz, fx, fz;
+    const bool centred = f0 >= FLOW_CENTER;
+    if (centred) {
+        // outward from the patch holding the slot's earliest fixed node
+        const int sx = min((f0 - FLOW_CENTER) / a.px, a.npx - 1), sz = min((f1 - FLOW_CENTER) / a.pz, a.npz - 1);
+        bx = centre_rank(rx, sx, a.npx);
+        bz = centre_rank(rz, sz, a.npz);
+        fx = bx > sx ? 1 : (bx < sx ? -1 : 0);
+        fz = bz > sz ? 1 : (bz < sz ? -1 : 0);
+    } else {
+        fx = f0 < 0 ? -1 : 1;
+        fz = f1 < 0 ? -1 : 1;
+        bx = fx > 0 ? rx : a.npx - 1 - rx;
+        bz = fz > 0 ? rz : a.npz - 1 - rz;
+    }
     const int b = bx * a.npz + bz;
     int* st = a.stamp + (size_t)slot * np;
     const bool alt = a.walkalt && a.walkalt[slot];
     const bool down = a.walkalt ? (alt && !(iter & 1)) : (a.down != 0);
-    int* pg = (a.prog && !alt && (a.ordered_until == 0 || iter <= a.ordered_until)) ? a.prog + (size_t)slot * np : nullptr;
+    // alternating walks wait on their upstream patches only when ordered outward from their source
+    int* pg = (a.prog && (!alt || centred) && (a.ordered_until == 0 || iter <= a.ordered_until)) ? a.prog + (size_t)slot * np : nullptr;
     const int pbase = iter * a.pstride;
     const int t = threadIdx.x;
     if (t == 0) {
@@ -537,8 +574,8 @@ k_column(ColArgs<R> a)
     if (!s_act) return;
     if (t == 0) atomicAdd(a.visits, 1ull);
     // upstream neighbours along the flow, -1 where none
-    const int ux = (bx - fx >= 0 && bx - fx < a.npx) ? b - fx * a.npz : -1;
-    const int uz = (bz - fz >= 0 && bz - fz < a.npz) ? b - fz : -1;
+    const int ux = (fx != 0 && bx - fx >= 0 && bx - fx < a.npx) ? b - fx * a.npz : -1;
+    const int uz = (fz != 0 && bz - fz >= 0 && bz - fz < a.npz) ? b - fz : -1;
     bool wx = pg && ux >= 0, wz = pg && uz >= 0;   // still waiting on them (given up after a timeout)
     int seen_x = 0, seen_z = 0;                     // their progress as last read (lane 0)
     const size_t NL = (size_t)a.nnx * a.nnz;
@@ -1650,6 +1687,7 @@ struct fmm3d_ctx {
     int ordered = 1;               // column walk: patches wait for their upstream neighbours
     int spin_limit = 20000;        // polls (~64 cycles each) before such a wait is given up
     int max_sweeps = 0;            // column walk: stop after this many sweeps (0: at convergence)
+    int point_order = 1;           // alternating walks: patches outward from the earliest fixed node, waiting on their inner neighbours
     int trace = 0;                 // per-sweep counters to stderr (FMM3D_TRACE)
     int timing = 0;                // time every relaxation launch
     hipEvent_t kev[2 * 8] = {};    // per-launch events (timing)
@@ -2907,10 +2945,25 @@ static void fixed_flow(const Grid& g, const int64_t* node, const double* tv, int
     flow_code(gx, gz, out);
 }
 
+// Flow entries of a slot whose walk alternates: the (theta, phi) index of
+// its earliest fixed node (a point source's own cell) plus FLOW_CENTER.
+static void centre_flow(const Grid& g, const int64_t* node, const double* tv, int64_t n, int out[2])
+{
+    int64_t best = -1;
+    double m = __builtin_huge_val();
+    for (int64_t i = 0; i < n; ++i)
+        if (tv[i] < m) { m = tv[i]; best = node[i]; }
+    if (best < 0) { out[0] = out[1] = 1; return; }
+    const int64_t f = best % ((int64_t)g.nnx * g.nnz);
+    out[0] = FLOW_CENTER + (int)(f / g.nnz);
+    out[1] = FLOW_CENTER + (int)(f % g.nnz);
+}
+
 // Argument checks of fixed node sets, and per source the float32 offset
-// (the smallest fixed time) and the lateral flow code.
+// (the smallest fixed time) and the lateral flow code (centred: the
+// earliest fixed node, for alternating walks).
 static int fixed_prepare(fmm3d_ctx* c, const char* fn, int nsrc, const int64_t* offs, const int64_t* node,
-                         const double* tv, std::vector<double>& toff, std::vector<int>& flow)
+                         const double* tv, std::vector<double>& toff, std::vector<int>& flow, bool centred)
 {
     if (nsrc <= 0 || !offs) return fail(c, FMM3D_E_ARG, "%s: need nsrc > 0 and offsets", fn);
     if (offs[0] != 0) return fail(c, FMM3D_E_ARG, "%s: offs[0] must be 0", fn);
@@ -2929,7 +2982,8 @@ static int fixed_prepare(fmm3d_ctx* c, const char* fn, int nsrc, const int64_t* 
             m = std::min(m, tv[i]);
         }
         toff[q] = isfinite(m) ? m : 0.0;
-        fixed_flow(g, node + offs[q], tv + offs[q], offs[q + 1] - offs[q], &flow[2 * q]);
+        if (centred) centre_flow(g, node + offs[q], tv + offs[q], offs[q + 1] - offs[q], &flow[2 * q]);
+        else fixed_flow(g, node + offs[q], tv + offs[q], offs[q + 1] - offs[q], &flow[2 * q]);
     }
     return FMM3D_OK;
 }
@@ -3001,7 +3055,7 @@ extern "C" int fmm3d_solve_fixed(fmm3d_ctx* c, int nsrc, const int64_t* offs, co
     if (!c->have_model) return fail(c, FMM3D_E_STATE, "%s: no velocity model (call fmm3d_set_model_vtx)", fn);
     std::vector<double> toff;
     std::vector<int> flow;
-    int rc = fixed_prepare(c, fn, nsrc, offs, node, tv, toff, flow);
+    int rc = fixed_prepare(c, fn, nsrc, offs, node, tv, toff, flow, (flags & FMM3D_WALK_ALTERNATE) && c->point_order);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
     rc = ensure_slots(c, nsrc, precision);
@@ -3106,6 +3160,7 @@ static int point_fixed_set(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& e
         int rc = upload_metrics(fc);
         if (rc) return fail_to(err, rc, "refined grid: %s", fc->err.c_str());
     }
+    fc->point_order = c->point_order;
     int rc = fmm3d_set_model_vtx(fc, velv);
     if (rc) return fail_to(err, rc, "refined gridder: %s", fc->err.c_str());
     // straight-ray times on the corners of the fine cell holding the source
@@ -3255,7 +3310,8 @@ extern "C" int fmm3d_solve_mixed(fmm3d_ctx* c, int ntele, const double* faces, c
     if (ntele > 0 && (rc = tele_check(c, fn, ntele, faces, xface, zface, side))) return rc;
     if (npoint > 0) {
         if ((rc = point_sets(c, fn, npoint, src, refine, radius, order, offs, node, tv))) return rc;
-        if ((rc = fixed_prepare(c, fn, npoint, offs.data(), node.data(), tv.data(), ptoff, pflow))) return rc;
+        if ((rc = fixed_prepare(c, fn, npoint, offs.data(), node.data(), tv.data(), ptoff, pflow, c->point_order != 0)))
+            return rc;
     }
     HIPCHK(c, hipSetDevice(c->device));
     const int nsrc = ntele + npoint;
@@ -3561,7 +3617,8 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
     }
     const double toff = isfinite(m) ? m : 0.0;
     int flow[2] = { 1, 1 };
-    fixed_flow(c->g, node, tv, n, flow);
+    if (walkalt && c->point_order) centre_flow(c->g, node, tv, n, flow);
+    else fixed_flow(c->g, node, tv, n, flow);
     if (c->eng_stage < (size_t)n + 2) {
         (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv);
         c->eng_node = nullptr; c->eng_tv = nullptr; c->eng_stage = 0;
@@ -3917,6 +3974,9 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "graph")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "graph must be 0 or 1");
         c->use_graph = iv;
+    } else if (!strcmp(key, "point_order")) {
+        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "point_order must be 0 or 1");
+        c->point_order = iv;
     } else if (!strcmp(key, "max_sweeps")) {
         if (iv < 0) return fail(c, FMM3D_E_ARG, "max_sweeps must be >= 0");
         c->max_sweeps = iv;

@@ -54,6 +54,28 @@ def test_point_source_rays_match_oracle(cfg0):
             assert np.abs(va - vb).max() <= 1e-6 * np.abs(vb).max(), i
 
 
+def test_point_order_keeps_the_field(cfg0):
+    """Patches ordered outward from the source (point_order 1, the default)
+    and unordered (0) reach the same fixed point, for the refined blocks and
+    the coarse solve; both match the oracle."""
+    g, velv, veln, srcs, rcv, ref = cfg0
+    fields, sweeps = {}, {}
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        for po in (0, 1):
+            c.set_option("point_order", po)
+            c.solve_point(srcs, 5, 10, lib.ORDER_MIXED, lib.PREC_FP64)
+            fields[po] = [c.field(s) for s in range(len(srcs))]
+            sweeps[po] = c.stats()["sweeps"]
+        with pytest.raises(lib.FmmError) as e:
+            c.set_option("point_order", 2)
+        assert e.value.code == lib.E_ARG
+    print(f"coarse sweeps: unordered {sweeps[0]}, outward {sweeps[1]}")
+    for s in range(len(srcs)):
+        assert float(np.abs(fields[1][s] - fields[0][s]).max()) <= 1e-9, s
+        assert float(np.abs(fields[1][s] - ref[s]).max()) <= 1e-6, s
+
+
 def test_point_sources_fp32_within_one_millisecond(cfg0):
     g, velv, veln, srcs, rcv, ref = cfg0
     with lib.Context(g) as c:
