@@ -133,9 +133,10 @@ int fmm3d_solve_fixed(fmm3d_ctx *ctx, int nsrc, const int64_t *offs, const int64
  * `refine` per axis; the sub-grid gets B-spline velocities, straight-ray
  * times on the 8 corners of the fine cell holding the source and a float64
  * solve; the coarse nodes it covers then seed the coarse solve
- * (FMM3D_WALK_ALTERNATE).  Up to four refined blocks are solved at once, on
- * worker threads with their own streams.  Returns FMM3D_E_ARG for a source
- * outside the grid.  flags: 0. */
+ * (FMM3D_WALK_ALTERNATE).  Up to "point_workers" (default 4) refined
+ * blocks are solved at once, largest first, on worker threads with their
+ * own streams; their contexts stay with ctx, pooled by block shape.
+ * Returns FMM3D_E_ARG for a source outside the grid.  flags: 0. */
 int fmm3d_solve_point(fmm3d_ctx *ctx, int nsrc, const double *src, int refine, int radius,
                       int order, int precision, int flags);
 
@@ -216,6 +217,7 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *                   earliest fixed node and wait on their neighbour towards
  *                   it, as ordered sources wait along their flow; 0 no
  *                   ordering.  Refined-block solves follow the setting
+ *   "point_workers" threads solving refined blocks at once (1..16, default 4)
  *   "spin_limit"    polls before such a wait is given up (default 20000)
  *   "check_every"   column walk: sweeps between convergence checks (1..8,
  *                   default 4); at a check, every source whose last sweep

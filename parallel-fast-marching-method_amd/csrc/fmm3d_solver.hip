@@ -34,6 +34,8 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <thread>
@@ -1647,7 +1649,8 @@ struct fmm3d_ctx {
     std::vector<int> walk_slots;   // per slot walk flags when walk_alt == 2 (mixed batches)
     std::vector<double> velv_host; // the vertex model (for refined sub-grids)
     fmm3d_ctx* fine = nullptr;     // refined source neighbourhood (point sources)
-    std::vector<fmm3d_ctx*> fines; // the same, one per worker of a batch of point sources
+    std::vector<fmm3d_ctx*> fines; // the same for batches of point sources: a pool kept by block shape
+    int point_workers = 4;         // threads running a batch's refined-block solves
     // continuous-batching engine
     bool eng_open = false;
     int eng_k = 1, eng_order = FMM3D_ORDER_MIXED, eng_pending = 0;
@@ -3218,42 +3221,72 @@ static int point_sets(fmm3d_ctx* c, const char* fn, int nsrc, const double* src,
     if (refine < 1 || refine > 16) return fail(c, FMM3D_E_ARG, "%s: refine must be 1..16", fn);
     for (int q = 0; q < nsrc; ++q)
         if (radius[q] < 1 || radius[q] > 64) return fail(c, FMM3D_E_ARG, "%s: radius of source %d must be 1..64", fn, q);
-    const int nw = std::min(nsrc, 4);
-    if ((int)c->fines.size() < nw) c->fines.resize(nw, nullptr);
+    const int nw = std::min(nsrc, c->point_workers);
     std::vector<std::vector<int64_t>> pn(nsrc);
     std::vector<std::vector<double>> pt(nsrc);
     std::vector<int> prc(nsrc, FMM3D_OK);
     std::vector<std::string> perr(nsrc);
-    // sources by refined-block size, each worker a contiguous run: a worker
-    // re-creates its refined context only when the size changes
+    // largest refined block first, each worker taking the next source when
+    // it is free, so the big blocks do not pile up on one worker
     std::vector<std::pair<int64_t, int>> order_by(nsrc);
+    std::vector<std::array<int, 3>> shape(nsrc, std::array<int, 3>{ 0, 0, 0 });
     for (int q = 0; q < nsrc; ++q) {
         int lo[3], hi[3];
         int64_t key = -1;
-        if (point_block(c->g, src + 3 * q, radius[q], lo, hi))
-            key = ((int64_t)(hi[0] - lo[0]) * 4096 + (hi[1] - lo[1])) * 4096 + (hi[2] - lo[2]);
-        order_by[q] = { key, q };
+        if (point_block(c->g, src + 3 * q, radius[q], lo, hi)) {
+            for (int a = 0; a < 3; ++a) shape[q][a] = (hi[a] - lo[a]) * refine + 1;
+            key = (int64_t)shape[q][0] * shape[q][1] * shape[q][2];
+        }
+        order_by[q] = { -key, q };
     }
     std::sort(order_by.begin(), order_by.end());
-    auto work = [&](int w) {
+    // refined contexts stay in a pool between calls, matched by block
+    // shape: a worker reuses an idle one of its block's shape and creates
+    // one otherwise (freeing device memory would wait for the other
+    // workers' streams); past 32 contexts an idle one is rebuilt instead
+    std::mutex pool_m;
+    std::vector<char> busy(c->fines.size(), 0);
+    auto acquire = [&](const std::array<int, 3>& sh) {
+        std::lock_guard<std::mutex> lk(pool_m);
+        int empty = -1, spare = -1;
+        for (size_t i = 0; i < c->fines.size(); ++i) {
+            if (busy[i]) continue;
+            const fmm3d_ctx* f = c->fines[i];
+            if (f && f->g.nnr == sh[0] && f->g.nnx == sh[1] && f->g.nnz == sh[2]) { busy[i] = 1; return (int)i; }
+            if (!f && empty < 0) empty = (int)i;
+            if (f && spare < 0) spare = (int)i;
+        }
+        if (empty >= 0) { busy[empty] = 1; return empty; }
+        if (c->fines.size() >= 32 && spare >= 0) { busy[spare] = 1; return spare; }
+        c->fines.push_back(nullptr);
+        busy.push_back(1);
+        return (int)c->fines.size() - 1;
+    };
+    std::atomic<int> next{ 0 };
+    auto work = [&]() {
         hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
         (void)hipThreadExchangeStreamCaptureMode(&mode);
-        const int b = (int)((int64_t)nsrc * w / nw), e = (int)((int64_t)nsrc * (w + 1) / nw);
-        if (hipSetDevice(c->device) != hipSuccess) {
-            for (int k = b; k < e; ++k) { prc[order_by[k].second] = FMM3D_E_DEVICE; perr[order_by[k].second] = "hipSetDevice failed"; }
-            return;
-        }
-        for (int k = b; k < e; ++k) {
+        const bool dev_ok = hipSetDevice(c->device) == hipSuccess;
+        for (int k; (k = next.fetch_add(1)) < nsrc;) {
             const int q = order_by[k].second;
-            prc[q] = point_fixed_set(c, &c->fines[w], perr[q], src + 3 * q, refine, radius[q], order, pn[q], pt[q]);
-            if (prc[q]) return;
+            if (!dev_ok) { prc[q] = FMM3D_E_DEVICE; perr[q] = "hipSetDevice failed"; continue; }
+            const int i = acquire(shape[q]);
+            fmm3d_ctx* f;
+            {
+                std::lock_guard<std::mutex> lk(pool_m);
+                f = c->fines[i];
+            }
+            prc[q] = point_fixed_set(c, &f, perr[q], src + 3 * q, refine, radius[q], order, pn[q], pt[q]);
+            std::lock_guard<std::mutex> lk(pool_m);
+            c->fines[i] = f;
+            busy[i] = 0;
         }
     };
     if (nw == 1) {
-        work(0);
+        work();
     } else {
         std::vector<std::thread> th;
-        for (int w = 0; w < nw; ++w) th.emplace_back(work, w);
+        for (int w = 0; w < nw; ++w) th.emplace_back(work);
         for (auto& t : th) t.join();
     }
     offs.assign(nsrc + 1, 0);
@@ -3974,6 +4007,9 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "graph")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "graph must be 0 or 1");
         c->use_graph = iv;
+    } else if (!strcmp(key, "point_workers")) {
+        if (iv < 1 || iv > 16) return fail(c, FMM3D_E_ARG, "point_workers must be in 1..16");
+        c->point_workers = iv;
     } else if (!strcmp(key, "point_order")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "point_order must be 0 or 1");
         c->point_order = iv;

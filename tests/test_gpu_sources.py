@@ -76,6 +76,26 @@ def test_point_order_keeps_the_field(cfg0):
         assert float(np.abs(fields[1][s] - ref[s]).max()) <= 1e-6, s
 
 
+def test_point_workers_and_context_pool(cfg0):
+    """One worker or several, first call or with the refined contexts pooled
+    from earlier calls: the same fields."""
+    g, velv, veln, srcs, rcv, ref = cfg0
+    fields = []
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        for nw in (1, 3, 3, 16):
+            c.set_option("point_workers", nw)
+            c.solve_point(srcs, 5, 10, lib.ORDER_MIXED, lib.PREC_FP64)
+            fields.append([c.field(s) for s in range(len(srcs))])
+        for bad in (0, 17):
+            with pytest.raises(lib.FmmError) as e:
+                c.set_option("point_workers", bad)
+            assert e.value.code == lib.E_ARG
+    for run in fields[1:]:
+        for s in range(len(srcs)):
+            assert float(np.abs(run[s] - fields[0][s]).max()) <= 1e-9, s
+
+
 def test_point_sources_fp32_within_one_millisecond(cfg0):
     g, velv, veln, srcs, rcv, ref = cfg0
     with lib.Context(g) as c:
