@@ -407,6 +407,7 @@ template <typename R> struct ColArgs {
     int* pend;              // per slot per patch: a node would move (fixed-point check)
     int* slotpend;          // per slot: some node would move
     const int* born;        // per slot: its first sweep, checked after it (the serving engine), or null
+    size_t geo_n, geo_hx, geo_hz;   // per-slot strides of slo, hx, hz (slots on grids of their own), or 0 (shared)
 };
 
 // float32 helpers for the axis-sum octant form: fused multiply-subtract,
@@ -556,6 +557,9 @@ k_column(ColArgs<R> a)
     }
     const int b = bx * a.npz + bz;
     int* st = a.stamp + (size_t)slot * np;
+    const R* slo = a.slo + slot * a.geo_n;
+    const R* hxs = a.hx + slot * a.geo_hx;
+    const R* hzs = a.hz + slot * a.geo_hz;
     const bool alt = a.walkalt && a.walkalt[slot];
     const bool down = a.walkalt ? (alt && !(iter & 1)) : (a.down != 0);
     // alternating walks wait on their upstream patches only when ordered outward from their source
@@ -637,7 +641,7 @@ k_column(ColArgs<R> a)
             u1[e] = (in && n1 >= 0 && n1 < a.nnr) ? T[c + (ptrdiff_t)dn * (ptrdiff_t)NL] : inf;
             u2[e] = (in && n2 >= 0 && n2 < a.nnr) ? T[c + (ptrdiff_t)(2 * dn) * (ptrdiff_t)NL] : inf;
             const bool fixed = !in || ((fb[c >> 5] >> (c & 31)) & 1u);
-            sl[e] = fixed ? R(-1) : a.slo[c];
+            sl[e] = fixed ? R(-1) : slo[c];
             nfree += fixed ? 0u : 1u;
             if (q == 0) { b1[e] = inf; b2[e] = inf; }
         }
@@ -652,11 +656,11 @@ k_column(ColArgs<R> a)
         }
         for (int e = t; e < PX; e += 64) {
             const int gx = min(x0 + e, a.nnx - 1);
-            const R h = a.hz[(size_t)ir * a.nnx + gx];
+            const R h = hzs[(size_t)ir * a.nnx + gx];
             shz[e] = h;
             sihz[e] = R(1) / h;
         }
-        const R hxl = a.hx[ir], ihx = R(1) / hxl;
+        const R hxl = hxs[ir], ihx = R(1) / hxl;
         __syncthreads();
         // the passes stop after one that changes nothing: the layer is then
         // at a fixpoint of the update and later passes would not move it
@@ -799,10 +803,10 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
                 v1[3] = in1[3] ? T[c + a.nnz] : inf; v2[3] = in2[3] ? T[c + 2 * (size_t)a.nnz] : inf;
                 v1[4] = in1[4] ? T[c - 1] : inf; v2[4] = in2[4] ? T[c - 2] : inf;
                 v1[5] = in1[5] ? T[c + 1] : inf; v2[5] = in2[5] ? T[c + 2] : inf;
-                const R hxl = a.hx[ir], hzl = a.hz[(size_t)ir * a.nnx + ix];
+                const R hxl = a.hx[slot * a.geo_hx + ir], hzl = a.hz[slot * a.geo_hz + (size_t)ir * a.nnx + ix];
                 const R h[3] = { hr, hxl, hzl };
                 const R ih[3] = { ihr, R(1) / hxl, R(1) / hzl };
-                const R nv = col_update<R, FAST, SHIFT>(a.fom, w[2], v1, v2, in1, in2, h, ih, a.slo[c]);
+                const R nv = col_update<R, FAST, SHIFT>(a.fom, w[2], v1, v2, in1, in2, h, ih, a.slo[slot * a.geo_n + c]);
                 if (nv != w[2]) moved = true;
             }
             w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
@@ -1620,6 +1624,15 @@ struct fmm3d_ctx {
     double* hz64 = nullptr;
     float* hx32 = nullptr;
     float* hz32 = nullptr;
+    // refined-block contexts: each slot on a block of its own (same shape,
+    // its own origin), with slowness and spacings per slot (geo_cap slots)
+    int geo_cap = 0;
+    double* gslo64 = nullptr;
+    float* gslo32 = nullptr;
+    double* ghx64 = nullptr;
+    double* ghz64 = nullptr;
+    float* ghx32 = nullptr;
+    float* ghz32 = nullptr;
     bool have_model = false;
     Slots sl;
     int tile_x = 8, tile_z = 32;   // lateral patch of a brick (theta x phi)
@@ -1947,6 +1960,8 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out); (void)hipFree(c->eng_islot);
     free_slots(c->sl);
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
+    (void)hipFree(c->gslo64); (void)hipFree(c->gslo32);
+    (void)hipFree(c->ghx64); (void)hipFree(c->ghz64); (void)hipFree(c->ghx32); (void)hipFree(c->ghz32);
     (void)hipFree(c->hx64); (void)hipFree(c->hz64); (void)hipFree(c->hx32); (void)hipFree(c->hz32);
     (void)hipFree(c->d_updates); (void)hipFree(c->d_visits); (void)hipFree(c->ray_scratch);
     (void)hipFree(c->fr_path); (void)hipFree(c->fr_pn); (void)hipFree(c->fr_pbox);
@@ -2380,7 +2395,7 @@ static int sweep_kernel(const fmm3d_ctx* c, int k, bool engine)
     // sources only: with alternating-walk (point) slots it crept down by
     // float32 ulps for thousands of sweeps on a 16-source batch, and the
     // serving engine may admit such slots at any time
-    if (c->strategy != 2 || engine || c->walk_alt) return 1;
+    if (c->strategy != 2 || engine || c->walk_alt || c->geo_cap) return 1;
     if (c->plane_sweeps == 0) return 2;
     return (!engine && k <= c->plane_sweeps) ? 2 : 1;
 }
@@ -2468,6 +2483,16 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     a.pend = s.pend;
     a.slotpend = s.slotpend;
     a.born = nullptr;
+    a.geo_n = a.geo_hx = a.geo_hz = 0;
+    if (c->geo_cap > 0) {
+        if (nslots > c->geo_cap) return fail(c, FMM3D_E_STATE, "solver: %d slots on a %d-block context", nslots, c->geo_cap);
+        a.slo = std::is_same<R, double>::value ? (const R*)c->gslo64 : (const R*)c->gslo32;
+        a.hx = std::is_same<R, double>::value ? (const R*)c->ghx64 : (const R*)c->ghx32;
+        a.hz = std::is_same<R, double>::value ? (const R*)c->ghz64 : (const R*)c->ghz32;
+        a.geo_n = c->N;
+        a.geo_hx = (size_t)g.nnr;
+        a.geo_hz = (size_t)g.nnr * g.nnx;
+    }
     blocks = 8 * ((nslots * npx + 7) / 8) * npz;   // rows of patches round-robin over 8 XCDs
     return FMM3D_OK;
 }
@@ -3106,8 +3131,6 @@ static void cart3(const Grid& g, double h, double th, double ph, double x[3])
     x[2] = r * cos(th);
 }
 
-// The refined neighbourhood of one point source: its fixed set for the
-// coarse grid (appended to node/tv).  Uses the context's refined sub-grid.
 // Coarse nodes lo..hi (per axis r, theta, phi) of a point source's refined
 // block: the cell holding it widened by `radius` nodes, clipped to the grid.
 // False when the source lies outside the grid.
@@ -3124,96 +3147,189 @@ static bool point_block(const Grid& g, const double* src3, int radius, int lo[3]
     return true;
 }
 
-static int point_fixed_set(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, const double* src3, int refine,
-                           int radius, int order, std::vector<int64_t>& node, std::vector<double>& tv)
+// Spacings of grid f (the reference's metric arrays): per layer r dtheta,
+// per layer and theta row r sin(theta) dphi.
+static void grid_metrics(const Grid& g, std::vector<double>& hx, std::vector<double>& hz)
+{
+    hx.resize(g.nnr);
+    hz.resize((size_t)g.nnx * g.nnr);
+    for (int ir = 0; ir < g.nnr; ++ir) {
+        const double ri = g.rgor - (ir) * g.dnr + g.earth;
+        hx[ir] = ri * g.dnx;
+        for (int ix = 0; ix < g.nnx; ++ix) hz[(size_t)ir * g.nnx + ix] = ri * sin(g.rgox + (ix) * g.dnx) * g.dnz;
+    }
+}
+
+// Slot q of a refined-block context on block grid f: its spacings, and the
+// B-spline slowness of f's nodes from the context's vertex model.
+static int geo_slot(fmm3d_ctx* fc, int q, const Grid& f)
+{
+    std::vector<double> hx, hz;
+    grid_metrics(f, hx, hz);
+    std::vector<float> hx32(hx.begin(), hx.end()), hz32(hz.begin(), hz.end());
+    const size_t N = fc->N;
+    HIPCHK(fc, hipMemcpyAsync(fc->ghx64 + (size_t)q * f.nnr, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, fc->stream));
+    HIPCHK(fc, hipMemcpyAsync(fc->ghz64 + (size_t)q * hz.size(), hz.data(), hz.size() * sizeof(double), hipMemcpyHostToDevice, fc->stream));
+    HIPCHK(fc, hipMemcpyAsync(fc->ghx32 + (size_t)q * f.nnr, hx32.data(), hx32.size() * sizeof(float), hipMemcpyHostToDevice, fc->stream));
+    HIPCHK(fc, hipMemcpyAsync(fc->ghz32 + (size_t)q * hz32.size(), hz32.data(), hz32.size() * sizeof(float), hipMemcpyHostToDevice, fc->stream));
+    const long total = (long)(f.nnr + 3) * (f.nnx + 3) * (f.nnz + 3);
+    const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
+    k_gridder<<<blocks, 256, 0, fc->stream>>>(f, fc->d_velv, fc->veln, fc->gslo64 + (size_t)q * N, fc->gslo32 + (size_t)q * N);
+    HIPCHK(fc, hipGetLastError());
+    // the metric vectors are pageable host memory: the copies finish before they go
+    HIPCHK(fc, hipStreamSynchronize(fc->stream));
+    return FMM3D_OK;
+}
+
+// Per-slot geometry arrays for k blocks (grown, never shrunk).
+static int geo_reserve(fmm3d_ctx* fc, int k)
+{
+    if (fc->geo_cap >= k) return FMM3D_OK;
+    HIPCHK(fc, hipStreamSynchronize(fc->stream));
+    (void)hipFree(fc->gslo64); (void)hipFree(fc->gslo32);
+    (void)hipFree(fc->ghx64); (void)hipFree(fc->ghz64); (void)hipFree(fc->ghx32); (void)hipFree(fc->ghz32);
+    fc->gslo64 = nullptr; fc->gslo32 = nullptr;
+    fc->ghx64 = fc->ghz64 = nullptr; fc->ghx32 = fc->ghz32 = nullptr;
+    fc->geo_cap = 0;
+    const Grid& f = fc->g;
+    const size_t nhz = (size_t)f.nnr * f.nnx;
+    HIPCHK(fc, dalloc(&fc->gslo64, (size_t)k * fc->N * sizeof(double)));
+    HIPCHK(fc, dalloc(&fc->gslo32, (size_t)k * fc->N * sizeof(float)));
+    HIPCHK(fc, dalloc(&fc->ghx64, (size_t)k * f.nnr * sizeof(double)));
+    HIPCHK(fc, dalloc(&fc->ghz64, (size_t)k * nhz * sizeof(double)));
+    HIPCHK(fc, dalloc(&fc->ghx32, (size_t)k * f.nnr * sizeof(float)));
+    HIPCHK(fc, dalloc(&fc->ghz32, (size_t)k * nhz * sizeof(float)));
+    fc->geo_cap = k;
+    return FMM3D_OK;
+}
+
+// The refined neighbourhoods of point sources members[0..k-1] (src and
+// radius index them), whose blocks have one shape, solved together: one
+// slot per source on a refined-block context, each slot with its block's
+// own origin, spacings and B-spline slowness.  Appends each source's fixed
+// set for the coarse grid to node[q] / tv[q].
+static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, const std::vector<int>& members,
+                       const double* src, const int* radius, int refine, int order,
+                       std::vector<int64_t>* node, std::vector<double>* tv)
 {
     const Grid& g = c->g;
     const double* velv = c->velv_host.data();
-    const double hs = src3[0], ths = src3[1], phs = src3[2];
-    int lo[3], hi[3];
-    if (!point_block(g, src3, radius, lo, hi))
-        return fail_to(err, FMM3D_E_ARG, "point source (%g km, %g, %g rad) lies outside the grid", hs, ths, phs);
-    // the refined sub-grid over coarse nodes lo..hi
-    Grid f = g;
-    f.nnr = (hi[0] - lo[0]) * refine + 1;
-    f.nnx = (hi[1] - lo[1]) * refine + 1;
-    f.nnz = (hi[2] - lo[2]) * refine + 1;
-    f.rgor = g.rgor - lo[0] * g.dnr;
-    f.rgox = g.rgox + lo[1] * g.dnx;
-    f.rgoz = g.rgoz + lo[2] * g.dnz;
-    f.dnr = g.dnr / refine; f.dnx = g.dnx / refine; f.dnz = g.dnz / refine;
-    if ((double)f.nnr * f.nnx * f.nnz > 64.0e6)
-        return fail_to(err, FMM3D_E_ARG, "refined block of %dx%dx%d nodes is too large", f.nnr, f.nnx, f.nnz);
+    const int k = (int)members.size();
+    std::vector<Grid> fg(k);
+    std::vector<std::array<int, 3>> blo(k);
+    for (int m = 0; m < k; ++m) {
+        const double* s3 = src + 3 * members[m];
+        int lo[3], hi[3];
+        if (!point_block(g, s3, radius[members[m]], lo, hi))
+            return fail_to(err, FMM3D_E_ARG, "point source (%g km, %g, %g rad) lies outside the grid", s3[0], s3[1], s3[2]);
+        Grid& f = fg[m];
+        f = g;
+        f.nnr = (hi[0] - lo[0]) * refine + 1;
+        f.nnx = (hi[1] - lo[1]) * refine + 1;
+        f.nnz = (hi[2] - lo[2]) * refine + 1;
+        f.rgor = g.rgor - lo[0] * g.dnr;
+        f.rgox = g.rgox + lo[1] * g.dnx;
+        f.rgoz = g.rgoz + lo[2] * g.dnz;
+        f.dnr = g.dnr / refine; f.dnx = g.dnx / refine; f.dnz = g.dnz / refine;
+        blo[m] = { lo[0], lo[1], lo[2] };
+        if (f.nnr != fg[0].nnr || f.nnx != fg[0].nnx || f.nnz != fg[0].nnz)
+            return fail_to(err, FMM3D_E_STATE, "refined blocks of one group differ in shape");
+    }
+    const Grid& f0 = fg[0];
+    if ((double)f0.nnr * f0.nnx * f0.nnz > 64.0e6)
+        return fail_to(err, FMM3D_E_ARG, "refined block of %dx%dx%d nodes is too large", f0.nnr, f0.nnx, f0.nnz);
     fmm3d_ctx* fc = *finep;
-    if (fc && (fc->g.nnr != f.nnr || fc->g.nnx != f.nnx || fc->g.nnz != f.nnz)) {
+    if (fc && (fc->g.nnr != f0.nnr || fc->g.nnx != f0.nnx || fc->g.nnz != f0.nnz)) {
         fmm3d_destroy(fc);
         *finep = fc = nullptr;
     }
     if (!fc) {
         fmm3d_ctx* n = new fmm3d_ctx();
-        n->g = f;
+        n->g = f0;
         int rc = ctx_setup(n, c->device, finep);
         if (rc) return fail_to(err, rc, "refined grid: %s", fmm3d_last_error(nullptr));
         fc = *finep;
-        // one source per solve: direct launches (no graph capture while other
-        // refined solves run on other threads)
+        // direct launches (no graph capture while other refined solves run
+        // on other threads)
         fc->use_graph = 0;
-    } else {
-        fc->g = f;
-        int rc = upload_metrics(fc);
-        if (rc) return fail_to(err, rc, "refined grid: %s", fc->err.c_str());
     }
+    fc->g = f0;
     fc->point_order = c->point_order;
+    // the vertex model (its gridding onto the shared arrays goes unused:
+    // every slot has its own block)
     int rc = fmm3d_set_model_vtx(fc, velv);
     if (rc) return fail_to(err, rc, "refined gridder: %s", fc->err.c_str());
-    // straight-ray times on the corners of the fine cell holding the source
-    int fcell[3];
-    const double fu[3] = { (f.rgor - hs) / f.dnr, (ths - f.rgox) / f.dnx, (phs - f.rgoz) / f.dnz };
-    const int fn[3] = { f.nnr, f.nnx, f.nnz };
-    for (int a = 0; a < 3; ++a) fcell[a] = std::min(std::max((int)floor(fu[a]), 0), fn[a] - 2);
-    double xs[3];
-    cart3(g, hs, ths, phs, xs);
-    const double ss = 1.0 / host_velocity(g, velv, hs, g.gor - hs, ths, phs);
-    int64_t fnode[8];
-    double ftime[8];
-    int m = 0;
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
-            for (int k = 0; k < 2; ++k, ++m) {
-                const int r = fcell[0] + i, x = fcell[1] + j, z = fcell[2] + k;
-                const double hn = f.rgor - r * f.dnr, thn = f.rgox + x * f.dnx, phn = f.rgoz + z * f.dnz;
-                double xn[3];
-                cart3(g, hn, thn, phn, xn);
-                const double d = sqrt((xn[0] - xs[0]) * (xn[0] - xs[0]) + (xn[1] - xs[1]) * (xn[1] - xs[1]) +
-                                      (xn[2] - xs[2]) * (xn[2] - xs[2]));
-                const double sn = 1.0 / host_velocity(f, velv, hn, f.gor - f.rgor + r * f.dnr, thn, phn);
-                fnode[m] = ((int64_t)r * f.nnx + x) * f.nnz + z;
-                ftime[m] = d * 0.5 * (ss + sn);
-            }
-    const int64_t foffs[2] = { 0, 8 };
+    if ((rc = geo_reserve(fc, k))) return fail_to(err, rc, "refined blocks: %s", fc->err.c_str());
+    for (int m = 0; m < k; ++m)
+        if ((rc = geo_slot(fc, m, fg[m]))) return fail_to(err, rc, "refined blocks: %s", fc->err.c_str());
+    // straight-ray times on the corners of the fine cell holding each source
+    std::vector<int64_t> offs(k + 1, 0), fnode(8 * (size_t)k);
+    std::vector<double> ftime(8 * (size_t)k);
+    for (int m = 0; m < k; ++m) {
+        const Grid& f = fg[m];
+        const double* s3 = src + 3 * members[m];
+        const double hs = s3[0], ths = s3[1], phs = s3[2];
+        int fcell[3];
+        const double fu[3] = { (f.rgor - hs) / f.dnr, (ths - f.rgox) / f.dnx, (phs - f.rgoz) / f.dnz };
+        const int fn[3] = { f.nnr, f.nnx, f.nnz };
+        for (int a = 0; a < 3; ++a) fcell[a] = std::min(std::max((int)floor(fu[a]), 0), fn[a] - 2);
+        double xs[3];
+        cart3(g, hs, ths, phs, xs);
+        const double ss = 1.0 / host_velocity(g, velv, hs, g.gor - hs, ths, phs);
+        int e = 8 * m;
+        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+                for (int l = 0; l < 2; ++l, ++e) {
+                    const int r = fcell[0] + i, x = fcell[1] + j, z = fcell[2] + l;
+                    const double hn = f.rgor - r * f.dnr, thn = f.rgox + x * f.dnx, phn = f.rgoz + z * f.dnz;
+                    double xn[3];
+                    cart3(g, hn, thn, phn, xn);
+                    const double d = sqrt((xn[0] - xs[0]) * (xn[0] - xs[0]) + (xn[1] - xs[1]) * (xn[1] - xs[1]) +
+                                          (xn[2] - xs[2]) * (xn[2] - xs[2]));
+                    const double sn = 1.0 / host_velocity(f, velv, hn, f.gor - f.rgor + r * f.dnr, thn, phn);
+                    fnode[e] = ((int64_t)r * f.nnx + x) * f.nnz + z;
+                    ftime[e] = d * 0.5 * (ss + sn);
+                }
+        offs[m + 1] = 8 * (int64_t)(m + 1);
+    }
     // float64: in float32 the refined block moved the coarse field by up to
     // 1.8 ms (61^3, refinement 5 over 10 nodes)
-    rc = fmm3d_solve_fixed(fc, 1, foffs, fnode, ftime, order, FMM3D_PREC_FP64, FMM3D_WALK_ALTERNATE);
+    rc = fmm3d_solve_fixed(fc, k, offs.data(), fnode.data(), ftime.data(), order, FMM3D_PREC_FP64, FMM3D_WALK_ALTERNATE);
     if (rc) return fail_to(err, rc, "refined solve: %s", fc->err.c_str());
-    std::vector<double> fine((size_t)f.nnr * f.nnx * f.nnz);
-    rc = fmm3d_get_field(fc, 0, fine.data());
-    if (rc) return fail_to(err, rc, "refined field: %s", fc->err.c_str());
-    // the coarse nodes the sub-grid covers seed the coarse solve
-    for (int r = lo[0]; r <= hi[0]; ++r)
-        for (int x = lo[1]; x <= hi[1]; ++x)
-            for (int z = lo[2]; z <= hi[2]; ++z) {
-                const size_t fq = ((size_t)(r - lo[0]) * refine * f.nnx + (size_t)(x - lo[1]) * refine) * f.nnz +
-                                  (size_t)(z - lo[2]) * refine;
-                node.push_back(((int64_t)r * g.nnx + x) * g.nnz + z);
-                tv.push_back(fine[fq]);
-            }
+    std::vector<double> fine((size_t)f0.nnr * f0.nnx * f0.nnz);
+    for (int m = 0; m < k; ++m) {
+        rc = fmm3d_get_field(fc, m, fine.data());
+        if (rc) return fail_to(err, rc, "refined field: %s", fc->err.c_str());
+        // the coarse nodes the block covers seed the coarse solve
+        const int q = members[m];
+        const Grid& f = fg[m];
+        const int* lo = blo[m].data();
+        const int hi[3] = { lo[0] + (f.nnr - 1) / refine, lo[1] + (f.nnx - 1) / refine, lo[2] + (f.nnz - 1) / refine };
+        for (int r = lo[0]; r <= hi[0]; ++r)
+            for (int x = lo[1]; x <= hi[1]; ++x)
+                for (int z = lo[2]; z <= hi[2]; ++z) {
+                    const size_t fq = ((size_t)(r - lo[0]) * refine * f.nnx + (size_t)(x - lo[1]) * refine) * f.nnz +
+                                      (size_t)(z - lo[2]) * refine;
+                    node[q].push_back(((int64_t)r * g.nnx + x) * g.nnz + z);
+                    tv[q].push_back(fine[fq]);
+                }
+    }
     return FMM3D_OK;
 }
 
-// Fixed sets of point sources, source q refined over radius[q] nodes.  The
-// refined solves are independent: up to four worker threads run
-// them at once, each on its own refined-grid context and stream, and the
-// sets are joined in source order.
+// One point source's refined neighbourhood (a group of one).
+static int point_fixed_set(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, const double* src3, int refine,
+                           int radius, int order, std::vector<int64_t>& node, std::vector<double>& tv)
+{
+    const std::vector<int> one{ 0 };
+    return point_group(c, finep, err, one, src3, &radius, refine, order, &node, &tv);
+}
+
+// Fixed sets of point sources, source q refined over radius[q] nodes.
+// Sources whose refined blocks have one shape are solved together, one
+// slot each on a refined-block context; up to point_workers threads run
+// such groups at once, each on its own context and stream, and the sets
+// are joined in source order.
 static int point_sets(fmm3d_ctx* c, const char* fn, int nsrc, const double* src, int refine, const int* radius,
                       int order, std::vector<int64_t>& offs, std::vector<int64_t>& node, std::vector<double>& tv)
 {
@@ -3221,29 +3337,42 @@ static int point_sets(fmm3d_ctx* c, const char* fn, int nsrc, const double* src,
     if (refine < 1 || refine > 16) return fail(c, FMM3D_E_ARG, "%s: refine must be 1..16", fn);
     for (int q = 0; q < nsrc; ++q)
         if (radius[q] < 1 || radius[q] > 64) return fail(c, FMM3D_E_ARG, "%s: radius of source %d must be 1..64", fn, q);
-    const int nw = std::min(nsrc, c->point_workers);
     std::vector<std::vector<int64_t>> pn(nsrc);
     std::vector<std::vector<double>> pt(nsrc);
     std::vector<int> prc(nsrc, FMM3D_OK);
     std::vector<std::string> perr(nsrc);
-    // largest refined block first, each worker taking the next source when
-    // it is free, so the big blocks do not pile up on one worker
-    std::vector<std::pair<int64_t, int>> order_by(nsrc);
-    std::vector<std::array<int, 3>> shape(nsrc, std::array<int, 3>{ 0, 0, 0 });
+    // groups of sources whose refined blocks have one shape, solved as one
+    // batch on one refined context (at most 16 sources and 64 M nodes a
+    // group); largest blocks first, each worker taking the next group when
+    // it is free
+    std::vector<std::array<int64_t, 5>> order_by(nsrc);
     for (int q = 0; q < nsrc; ++q) {
         int lo[3], hi[3];
-        int64_t key = -1;
-        if (point_block(c->g, src + 3 * q, radius[q], lo, hi)) {
-            for (int a = 0; a < 3; ++a) shape[q][a] = (hi[a] - lo[a]) * refine + 1;
-            key = (int64_t)shape[q][0] * shape[q][1] * shape[q][2];
-        }
-        order_by[q] = { -key, q };
+        std::array<int64_t, 3> sh{ 0, 0, 0 };
+        if (point_block(c->g, src + 3 * q, radius[q], lo, hi))
+            for (int a = 0; a < 3; ++a) sh[a] = (int64_t)(hi[a] - lo[a]) * refine + 1;
+        order_by[q] = { -sh[0] * sh[1] * sh[2], sh[0], sh[1], sh[2], q };
     }
     std::sort(order_by.begin(), order_by.end());
+    std::vector<std::vector<int>> groups;
+    std::vector<std::array<int, 3>> gshape;
+    for (int k = 0; k < nsrc; ++k) {
+        const auto& o = order_by[k];
+        const std::array<int, 3> sh{ (int)o[1], (int)o[2], (int)o[3] };
+        const int64_t nodes = std::max<int64_t>(1, -o[0]);
+        const size_t cap = (size_t)std::max<int64_t>(1, std::min<int64_t>(16, (int64_t)64000000 / nodes));
+        if (groups.empty() || gshape.back() != sh || groups.back().size() >= cap) {
+            groups.emplace_back();
+            gshape.push_back(sh);
+        }
+        groups.back().push_back((int)o[4]);
+    }
+    const int ng = (int)groups.size();
+    const int nw = std::min(ng, c->point_workers);
     // refined contexts stay in a pool between calls, matched by block
-    // shape: a worker reuses an idle one of its block's shape and creates
+    // shape: a worker reuses an idle one of its group's shape and creates
     // one otherwise (freeing device memory would wait for the other
-    // workers' streams); past 32 contexts an idle one is rebuilt instead
+    // workers' streams); past 16 contexts an idle one is rebuilt instead
     std::mutex pool_m;
     std::vector<char> busy(c->fines.size(), 0);
     auto acquire = [&](const std::array<int, 3>& sh) {
@@ -3257,7 +3386,7 @@ static int point_sets(fmm3d_ctx* c, const char* fn, int nsrc, const double* src,
             if (f && spare < 0) spare = (int)i;
         }
         if (empty >= 0) { busy[empty] = 1; return empty; }
-        if (c->fines.size() >= 32 && spare >= 0) { busy[spare] = 1; return spare; }
+        if (c->fines.size() >= 16 && spare >= 0) { busy[spare] = 1; return spare; }
         c->fines.push_back(nullptr);
         busy.push_back(1);
         return (int)c->fines.size() - 1;
@@ -3267,19 +3396,25 @@ static int point_sets(fmm3d_ctx* c, const char* fn, int nsrc, const double* src,
         hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
         (void)hipThreadExchangeStreamCaptureMode(&mode);
         const bool dev_ok = hipSetDevice(c->device) == hipSuccess;
-        for (int k; (k = next.fetch_add(1)) < nsrc;) {
-            const int q = order_by[k].second;
-            if (!dev_ok) { prc[q] = FMM3D_E_DEVICE; perr[q] = "hipSetDevice failed"; continue; }
-            const int i = acquire(shape[q]);
-            fmm3d_ctx* f;
-            {
+        for (int t; (t = next.fetch_add(1)) < ng;) {
+            std::string e;
+            int rc = FMM3D_E_DEVICE;
+            if (!dev_ok) {
+                e = "hipSetDevice failed";
+            } else {
+                const int i = acquire(gshape[t]);
+                fmm3d_ctx* f;
+                {
+                    std::lock_guard<std::mutex> lk(pool_m);
+                    f = c->fines[i];
+                }
+                rc = point_group(c, &f, e, groups[t], src, radius, refine, order, pn.data(), pt.data());
                 std::lock_guard<std::mutex> lk(pool_m);
-                f = c->fines[i];
+                c->fines[i] = f;
+                busy[i] = 0;
             }
-            prc[q] = point_fixed_set(c, &f, perr[q], src + 3 * q, refine, radius[q], order, pn[q], pt[q]);
-            std::lock_guard<std::mutex> lk(pool_m);
-            c->fines[i] = f;
-            busy[i] = 0;
+            if (rc)
+                for (int q : groups[t]) { prc[q] = rc; perr[q] = e; }
         }
     };
     if (nw == 1) {
