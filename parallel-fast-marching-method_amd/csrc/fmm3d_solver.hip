@@ -65,7 +65,7 @@ struct Grid {
 // taps beyond the vertex array are clamped into it.  Also writes slowness
 // (float64 and float32) for the interior nodes in solver layout.
 __global__ void k_gridder(Grid g, const double* __restrict__ velv, double* __restrict__ veln,
-                          double* __restrict__ slo64, float* __restrict__ slo32)
+                          double* __restrict__ slo64, float* __restrict__ slo32, int onx, int onz)
 {
     const int MR = g.nnr + 3, MX = g.nnx + 3, MZ = g.nnz + 3;
     const long total = (long)MR * MX * MZ;
@@ -99,7 +99,8 @@ __global__ void k_gridder(Grid g, const double* __restrict__ velv, double* __res
         }
         veln[i] = acc_z;
         if (kr >= 1 && kr <= g.nnr && kx >= 1 && kx <= g.nnx && kz >= 1 && kz <= g.nnz) {
-            size_t s = ((size_t)(kr - 1) * g.nnx + (kx - 1)) * g.nnz + (kz - 1);
+            // slowness rows of onx x onz nodes: g's own, or a larger container's
+            size_t s = ((size_t)(kr - 1) * onx + (kx - 1)) * onz + (kz - 1);
             double sl = 1.0 / acc_z;
             slo64[s] = sl;
             slo32[s] = (float)sl;
@@ -408,6 +409,8 @@ template <typename R> struct ColArgs {
     int* slotpend;          // per slot: some node would move
     const int* born;        // per slot: its first sweep, checked after it (the serving engine), or null
     size_t geo_n, geo_hx, geo_hz;   // per-slot strides of slo, hx, hz (slots on grids of their own), or 0 (shared)
+    const int* sdims;       // per slot (nnr, nnx, nnz) of its own grid inside the nnr x nnx x nnz
+                            // container (strides stay the container's), or null: the container's
 };
 
 // float32 helpers for the axis-sum octant form: fused multiply-subtract,
@@ -566,13 +569,17 @@ k_column(ColArgs<R> a)
     int* pg = (a.prog && (!alt || centred) && (a.ordered_until == 0 || iter <= a.ordered_until)) ? a.prog + (size_t)slot * np : nullptr;
     const int pbase = iter * a.pstride;
     const int t = threadIdx.x;
+    // this slot's own grid: nodes beyond it read as outside the grid
+    const int snr = a.sdims ? a.sdims[3 * slot] : a.nnr;
+    const int snx = a.sdims ? a.sdims[3 * slot + 1] : a.nnx;
+    const int snz = a.sdims ? a.sdims[3 * slot + 2] : a.nnz;
     if (t == 0) {
         int m = st[b];
         if (bx > 0) m = max(m, st[b - a.npz]);
         if (bx < a.npx - 1) m = max(m, st[b + a.npz]);
         if (bz > 0) m = max(m, st[b - 1]);
         if (bz < a.npz - 1) m = max(m, st[b + 1]);
-        s_act = (m >= iter - 1);
+        s_act = (m >= iter - 1) && bx * PX < snx && bz * PZ < snz;
         // an idle patch is finished with every layer of this sweep
         if (!s_act && pg) __hip_atomic_store(pg + b, pbase + a.nnr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -597,8 +604,8 @@ k_column(ColArgs<R> a)
     // walk: bottom to top, or top to bottom when a.down; "prev" layers are the
     // two this block relaxed just before (b1, b2), "next" ones come from HBM (u1, u2)
     const int dn = down ? 1 : -1;
-    for (int q = 0; q < a.nnr; ++q) {
-        const int ir = down ? q : a.nnr - 1 - q;
+    for (int q = 0; q < snr; ++q) {
+        const int ir = down ? q : snr - 1 - q;
         const size_t lay = (size_t)ir * NL;
         unsigned nfree = 0;
         // wait (bounded) until the upstream patches have finished this layer;
@@ -632,14 +639,14 @@ k_column(ColArgs<R> a)
             const int e = t + 64 * k;
             const int lx = e / PZ, lz = e % PZ;
             const int gx = x0 + lx, gz = z0 + lz;
-            const bool in = gx < a.nnx && gz < a.nnz;
+            const bool in = gx < snx && gz < snz;
             const size_t c = lay + (size_t)gx * a.nnz + gz;
             const R v = in ? T[c] : inf;
             orig[k] = v;
             cur[(lx + 2) * LZ + lz + 2] = v;
             const int n1 = ir + dn, n2 = ir + 2 * dn;
-            u1[e] = (in && n1 >= 0 && n1 < a.nnr) ? T[c + (ptrdiff_t)dn * (ptrdiff_t)NL] : inf;
-            u2[e] = (in && n2 >= 0 && n2 < a.nnr) ? T[c + (ptrdiff_t)(2 * dn) * (ptrdiff_t)NL] : inf;
+            u1[e] = (in && n1 >= 0 && n1 < snr) ? T[c + (ptrdiff_t)dn * (ptrdiff_t)NL] : inf;
+            u2[e] = (in && n2 >= 0 && n2 < snr) ? T[c + (ptrdiff_t)(2 * dn) * (ptrdiff_t)NL] : inf;
             const bool fixed = !in || ((fb[c >> 5] >> (c & 31)) & 1u);
             sl[e] = fixed ? R(-1) : slo[c];
             nfree += fixed ? 0u : 1u;
@@ -650,12 +657,12 @@ k_column(ColArgs<R> a)
             const int hx_ = e / (PZ + 4), hz_ = e % (PZ + 4);
             if (hx_ >= 2 && hx_ < PX + 2 && hz_ >= 2 && hz_ < PZ + 2) continue;
             const int gx = x0 + hx_ - 2, gz = z0 + hz_ - 2;
-            const bool in = gx >= 0 && gx < a.nnx && gz >= 0 && gz < a.nnz;
+            const bool in = gx >= 0 && gx < snx && gz >= 0 && gz < snz;
             // neighbours' values: device-coherent loads (another XCD may have written them this sweep)
             cur[hx_ * LZ + hz_] = in ? __hip_atomic_load(T + lay + (size_t)gx * a.nnz + gz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : inf;
         }
         for (int e = t; e < PX; e += 64) {
-            const int gx = min(x0 + e, a.nnx - 1);
+            const int gx = min(x0 + e, snx - 1);
             const R h = hzs[(size_t)ir * a.nnx + gx];
             shz[e] = h;
             sihz[e] = R(1) / h;
@@ -686,11 +693,11 @@ k_column(ColArgs<R> a)
                     R v1[6], v2[6];
                     bool in1[6], in2[6];
                     in1[0] = ir >= 1; in2[0] = ir >= 2;
-                    in1[1] = ir + 1 < a.nnr; in2[1] = ir + 2 < a.nnr;
+                    in1[1] = ir + 1 < snr; in2[1] = ir + 2 < snr;
                     in1[2] = ix >= 1; in2[2] = ix >= 2;
-                    in1[3] = ix + 1 < a.nnx; in2[3] = ix + 2 < a.nnx;
+                    in1[3] = ix + 1 < snx; in2[3] = ix + 2 < snx;
                     in1[4] = iz >= 1; in2[4] = iz >= 2;
-                    in1[5] = iz + 1 < a.nnz; in2[5] = iz + 2 < a.nnz;
+                    in1[5] = iz + 1 < snz; in2[5] = iz + 2 < snz;
                     // side 0 is the layer above (ir-1), side 1 the one below (ir+1)
                     v1[0] = down ? b1[pl] : u1[pl]; v2[0] = down ? b2[pl] : u2[pl];
                     v1[1] = down ? u1[pl] : b1[pl]; v2[1] = down ? u2[pl] : b2[pl];
@@ -775,6 +782,11 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
         const size_t f = i - (size_t)slot * NL;
         if (!a.slot_on[slot] || (a.born && a.born[slot] != iter)) continue;
         const int ix = (int)(f / a.nnz), iz = (int)(f - (size_t)ix * a.nnz);
+        // this slot's own grid inside the container
+        const int snr = a.sdims ? a.sdims[3 * slot] : a.nnr;
+        const int snx = a.sdims ? a.sdims[3 * slot + 1] : a.nnx;
+        const int snz = a.sdims ? a.sdims[3 * slot + 2] : a.nnz;
+        if (ix >= snx || iz >= snz) continue;
         const R* T = a.T + (size_t)slot * N;
         const uint32_t* fb = a.fixbits + (size_t)slot * NW;
         bool moved = false;
@@ -782,19 +794,19 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
         R w[5];
         w[0] = inf; w[1] = inf;
         w[2] = T[f];
-        w[3] = a.nnr > 1 ? T[NL + f] : inf;
-        w[4] = a.nnr > 2 ? T[2 * NL + f] : inf;
-        for (int ir = 0; ir < a.nnr; ++ir) {
+        w[3] = snr > 1 ? T[NL + f] : inf;
+        w[4] = snr > 2 ? T[2 * NL + f] : inf;
+        for (int ir = 0; ir < snr; ++ir) {
             const size_t c = (size_t)ir * NL + f;
             if (!((fb[c >> 5] >> (c & 31)) & 1u)) {
                 R v1[6], v2[6];
                 bool in1[6], in2[6];
                 in1[0] = ir >= 1; in2[0] = ir >= 2;
-                in1[1] = ir + 1 < a.nnr; in2[1] = ir + 2 < a.nnr;
+                in1[1] = ir + 1 < snr; in2[1] = ir + 2 < snr;
                 in1[2] = ix >= 1; in2[2] = ix >= 2;
-                in1[3] = ix + 1 < a.nnx; in2[3] = ix + 2 < a.nnx;
+                in1[3] = ix + 1 < snx; in2[3] = ix + 2 < snx;
                 in1[4] = iz >= 1; in2[4] = iz >= 2;
-                in1[5] = iz + 1 < a.nnz; in2[5] = iz + 2 < a.nnz;
+                in1[5] = iz + 1 < snz; in2[5] = iz + 2 < snz;
                 // side 0 the layer above (ir-1), side 1 the one below (ir+1);
                 // out of the grid reads as inf, as the walk's staging does
                 v1[0] = w[1]; v2[0] = w[0];
@@ -810,7 +822,7 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
                 if (nv != w[2]) moved = true;
             }
             w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
-            w[4] = ir + 3 < a.nnr ? T[c + 3 * NL] : inf;
+            w[4] = ir + 3 < snr ? T[c + 3 * NL] : inf;
         }
         if (moved) {
             a.pend[(size_t)slot * np + (ix / a.px) * a.npz + iz / a.pz] = 1;
@@ -1633,6 +1645,7 @@ struct fmm3d_ctx {
     double* ghz64 = nullptr;
     float* ghx32 = nullptr;
     float* ghz32 = nullptr;
+    int* d_sdims = nullptr;        // per slot (nnr, nnx, nnz) of its block inside the context's grid
     bool have_model = false;
     Slots sl;
     int tile_x = 8, tile_z = 32;   // lateral patch of a brick (theta x phi)
@@ -1962,6 +1975,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
     (void)hipFree(c->gslo64); (void)hipFree(c->gslo32);
     (void)hipFree(c->ghx64); (void)hipFree(c->ghz64); (void)hipFree(c->ghx32); (void)hipFree(c->ghz32);
+    (void)hipFree(c->d_sdims);
     (void)hipFree(c->hx64); (void)hipFree(c->hz64); (void)hipFree(c->hx32); (void)hipFree(c->hz32);
     (void)hipFree(c->d_updates); (void)hipFree(c->d_visits); (void)hipFree(c->ray_scratch);
     (void)hipFree(c->fr_path); (void)hipFree(c->fr_pn); (void)hipFree(c->fr_pbox);
@@ -1997,7 +2011,7 @@ extern "C" int fmm3d_set_model_vtx(fmm3d_ctx* c, const double* velv)
     if (e == hipSuccess) {
         long total = (long)(g.nnr + 3) * (g.nnx + 3) * (g.nnz + 3);
         int blocks = (int)std::min<long>((total + 255) / 256, 65536);
-        k_gridder<<<blocks, 256, 0, c->stream>>>(g, d, c->veln, c->slo64, c->slo32);
+        k_gridder<<<blocks, 256, 0, c->stream>>>(g, d, c->veln, c->slo64, c->slo32, g.nnx, g.nnz);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -2484,7 +2498,9 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     a.slotpend = s.slotpend;
     a.born = nullptr;
     a.geo_n = a.geo_hx = a.geo_hz = 0;
+    a.sdims = nullptr;
     if (c->geo_cap > 0) {
+        a.sdims = c->d_sdims;
         if (nslots > c->geo_cap) return fail(c, FMM3D_E_STATE, "solver: %d slots on a %d-block context", nslots, c->geo_cap);
         a.slo = std::is_same<R, double>::value ? (const R*)c->gslo64 : (const R*)c->gslo32;
         a.hx = std::is_same<R, double>::value ? (const R*)c->ghx64 : (const R*)c->ghx32;
@@ -3160,23 +3176,35 @@ static void grid_metrics(const Grid& g, std::vector<double>& hx, std::vector<dou
     }
 }
 
-// Slot q of a refined-block context on block grid f: its spacings, and the
-// B-spline slowness of f's nodes from the context's vertex model.
+// Slot q of a refined-block context on block grid f (at most the
+// context's grid in every dimension, laid out with the context's strides):
+// its spacings, and the B-spline slowness of f's nodes from the context's
+// vertex model.
 static int geo_slot(fmm3d_ctx* fc, int q, const Grid& f)
 {
+    const Grid& cg = fc->g;
     std::vector<double> hx, hz;
     grid_metrics(f, hx, hz);
-    std::vector<float> hx32(hx.begin(), hx.end()), hz32(hz.begin(), hz.end());
+    // container rows; entries beyond f are never read (the kernels stop at f's edges)
+    std::vector<double> chx(cg.nnr, 1.0), chz((size_t)cg.nnr * cg.nnx, 1.0);
+    for (int ir = 0; ir < f.nnr; ++ir) {
+        chx[ir] = hx[ir];
+        for (int ix = 0; ix < f.nnx; ++ix) chz[(size_t)ir * cg.nnx + ix] = hz[(size_t)ir * f.nnx + ix];
+    }
+    std::vector<float> chx32(chx.begin(), chx.end()), chz32(chz.begin(), chz.end());
     const size_t N = fc->N;
-    HIPCHK(fc, hipMemcpyAsync(fc->ghx64 + (size_t)q * f.nnr, hx.data(), hx.size() * sizeof(double), hipMemcpyHostToDevice, fc->stream));
-    HIPCHK(fc, hipMemcpyAsync(fc->ghz64 + (size_t)q * hz.size(), hz.data(), hz.size() * sizeof(double), hipMemcpyHostToDevice, fc->stream));
-    HIPCHK(fc, hipMemcpyAsync(fc->ghx32 + (size_t)q * f.nnr, hx32.data(), hx32.size() * sizeof(float), hipMemcpyHostToDevice, fc->stream));
-    HIPCHK(fc, hipMemcpyAsync(fc->ghz32 + (size_t)q * hz32.size(), hz32.data(), hz32.size() * sizeof(float), hipMemcpyHostToDevice, fc->stream));
+    const int dims[3] = { f.nnr, f.nnx, f.nnz };
+    HIPCHK(fc, hipMemcpyAsync(fc->ghx64 + (size_t)q * chx.size(), chx.data(), chx.size() * sizeof(double), hipMemcpyHostToDevice, fc->stream));
+    HIPCHK(fc, hipMemcpyAsync(fc->ghz64 + (size_t)q * chz.size(), chz.data(), chz.size() * sizeof(double), hipMemcpyHostToDevice, fc->stream));
+    HIPCHK(fc, hipMemcpyAsync(fc->ghx32 + (size_t)q * chx32.size(), chx32.data(), chx32.size() * sizeof(float), hipMemcpyHostToDevice, fc->stream));
+    HIPCHK(fc, hipMemcpyAsync(fc->ghz32 + (size_t)q * chz32.size(), chz32.data(), chz32.size() * sizeof(float), hipMemcpyHostToDevice, fc->stream));
+    HIPCHK(fc, hipMemcpyAsync(fc->d_sdims + 3 * q, dims, sizeof dims, hipMemcpyHostToDevice, fc->stream));
     const long total = (long)(f.nnr + 3) * (f.nnx + 3) * (f.nnz + 3);
     const int blocks = (int)std::min<long>((total + 255) / 256, 65536);
-    k_gridder<<<blocks, 256, 0, fc->stream>>>(f, fc->d_velv, fc->veln, fc->gslo64 + (size_t)q * N, fc->gslo32 + (size_t)q * N);
+    k_gridder<<<blocks, 256, 0, fc->stream>>>(f, fc->d_velv, fc->veln, fc->gslo64 + (size_t)q * N, fc->gslo32 + (size_t)q * N,
+                                              cg.nnx, cg.nnz);
     HIPCHK(fc, hipGetLastError());
-    // the metric vectors are pageable host memory: the copies finish before they go
+    // the host vectors are pageable memory: the copies finish before they go
     HIPCHK(fc, hipStreamSynchronize(fc->stream));
     return FMM3D_OK;
 }
@@ -3188,8 +3216,10 @@ static int geo_reserve(fmm3d_ctx* fc, int k)
     HIPCHK(fc, hipStreamSynchronize(fc->stream));
     (void)hipFree(fc->gslo64); (void)hipFree(fc->gslo32);
     (void)hipFree(fc->ghx64); (void)hipFree(fc->ghz64); (void)hipFree(fc->ghx32); (void)hipFree(fc->ghz32);
+    (void)hipFree(fc->d_sdims);
     fc->gslo64 = nullptr; fc->gslo32 = nullptr;
     fc->ghx64 = fc->ghz64 = nullptr; fc->ghx32 = fc->ghz32 = nullptr;
+    fc->d_sdims = nullptr;
     fc->geo_cap = 0;
     const Grid& f = fc->g;
     const size_t nhz = (size_t)f.nnr * f.nnx;
@@ -3199,15 +3229,26 @@ static int geo_reserve(fmm3d_ctx* fc, int k)
     HIPCHK(fc, dalloc(&fc->ghz64, (size_t)k * nhz * sizeof(double)));
     HIPCHK(fc, dalloc(&fc->ghx32, (size_t)k * f.nnr * sizeof(float)));
     HIPCHK(fc, dalloc(&fc->ghz32, (size_t)k * nhz * sizeof(float)));
+    HIPCHK(fc, dalloc(&fc->d_sdims, (size_t)k * 3 * sizeof(int)));
     fc->geo_cap = k;
     return FMM3D_OK;
 }
 
+// Node counts per axis of the refined block of one point source (0s when
+// the source lies outside the grid).
+static std::array<int, 3> block_shape(const Grid& g, const double* src3, int radius, int refine)
+{
+    int lo[3], hi[3];
+    if (!point_block(g, src3, radius, lo, hi)) return { 0, 0, 0 };
+    return { (hi[0] - lo[0]) * refine + 1, (hi[1] - lo[1]) * refine + 1, (hi[2] - lo[2]) * refine + 1 };
+}
+
 // The refined neighbourhoods of point sources members[0..k-1] (src and
-// radius index them), whose blocks have one shape, solved together: one
-// slot per source on a refined-block context, each slot with its block's
-// own origin, spacings and B-spline slowness.  Appends each source's fixed
-// set for the coarse grid to node[q] / tv[q].
+// radius index them) solved together: one slot per source on a
+// refined-block context whose grid holds every member's block (the largest
+// extent per axis), each slot with its own block's extent, origin,
+// spacings and B-spline slowness.  Appends each source's fixed set for the
+// coarse grid to node[q] / tv[q].
 static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, const std::vector<int>& members,
                        const double* src, const int* radius, int refine, int order,
                        std::vector<int64_t>* node, std::vector<double>* tv)
@@ -3217,6 +3258,7 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
     const int k = (int)members.size();
     std::vector<Grid> fg(k);
     std::vector<std::array<int, 3>> blo(k);
+    int cn[3] = { 0, 0, 0 };
     for (int m = 0; m < k; ++m) {
         const double* s3 = src + 3 * members[m];
         int lo[3], hi[3];
@@ -3232,20 +3274,21 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
         f.rgoz = g.rgoz + lo[2] * g.dnz;
         f.dnr = g.dnr / refine; f.dnx = g.dnx / refine; f.dnz = g.dnz / refine;
         blo[m] = { lo[0], lo[1], lo[2] };
-        if (f.nnr != fg[0].nnr || f.nnx != fg[0].nnx || f.nnz != fg[0].nnz)
-            return fail_to(err, FMM3D_E_STATE, "refined blocks of one group differ in shape");
+        cn[0] = std::max(cn[0], f.nnr); cn[1] = std::max(cn[1], f.nnx); cn[2] = std::max(cn[2], f.nnz);
+        if ((double)f.nnr * f.nnx * f.nnz > 64.0e6)
+            return fail_to(err, FMM3D_E_ARG, "refined block of %dx%dx%d nodes is too large", f.nnr, f.nnx, f.nnz);
     }
-    const Grid& f0 = fg[0];
-    if ((double)f0.nnr * f0.nnx * f0.nnz > 64.0e6)
-        return fail_to(err, FMM3D_E_ARG, "refined block of %dx%dx%d nodes is too large", f0.nnr, f0.nnx, f0.nnz);
+    // the context's grid: the members' largest extents (origin of the first)
+    Grid cg = fg[0];
+    cg.nnr = cn[0]; cg.nnx = cn[1]; cg.nnz = cn[2];
     fmm3d_ctx* fc = *finep;
-    if (fc && (fc->g.nnr != f0.nnr || fc->g.nnx != f0.nnx || fc->g.nnz != f0.nnz)) {
+    if (fc && (fc->g.nnr != cg.nnr || fc->g.nnx != cg.nnx || fc->g.nnz != cg.nnz)) {
         fmm3d_destroy(fc);
         *finep = fc = nullptr;
     }
     if (!fc) {
         fmm3d_ctx* n = new fmm3d_ctx();
-        n->g = f0;
+        n->g = cg;
         int rc = ctx_setup(n, c->device, finep);
         if (rc) return fail_to(err, rc, "refined grid: %s", fmm3d_last_error(nullptr));
         fc = *finep;
@@ -3253,7 +3296,7 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
         // on other threads)
         fc->use_graph = 0;
     }
-    fc->g = f0;
+    fc->g = cg;
     fc->point_order = c->point_order;
     // the vertex model (its gridding onto the shared arrays goes unused:
     // every slot has its own block)
@@ -3262,7 +3305,8 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
     if ((rc = geo_reserve(fc, k))) return fail_to(err, rc, "refined blocks: %s", fc->err.c_str());
     for (int m = 0; m < k; ++m)
         if ((rc = geo_slot(fc, m, fg[m]))) return fail_to(err, rc, "refined blocks: %s", fc->err.c_str());
-    // straight-ray times on the corners of the fine cell holding each source
+    // straight-ray times on the corners of the fine cell holding each
+    // source, at the context grid's node indices
     std::vector<int64_t> offs(k + 1, 0), fnode(8 * (size_t)k);
     std::vector<double> ftime(8 * (size_t)k);
     for (int m = 0; m < k; ++m) {
@@ -3287,7 +3331,7 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
                     const double d = sqrt((xn[0] - xs[0]) * (xn[0] - xs[0]) + (xn[1] - xs[1]) * (xn[1] - xs[1]) +
                                           (xn[2] - xs[2]) * (xn[2] - xs[2]));
                     const double sn = 1.0 / host_velocity(f, velv, hn, f.gor - f.rgor + r * f.dnr, thn, phn);
-                    fnode[e] = ((int64_t)r * f.nnx + x) * f.nnz + z;
+                    fnode[e] = ((int64_t)r * cg.nnx + x) * cg.nnz + z;
                     ftime[e] = d * 0.5 * (ss + sn);
                 }
         offs[m + 1] = 8 * (int64_t)(m + 1);
@@ -3296,7 +3340,7 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
     // 1.8 ms (61^3, refinement 5 over 10 nodes)
     rc = fmm3d_solve_fixed(fc, k, offs.data(), fnode.data(), ftime.data(), order, FMM3D_PREC_FP64, FMM3D_WALK_ALTERNATE);
     if (rc) return fail_to(err, rc, "refined solve: %s", fc->err.c_str());
-    std::vector<double> fine((size_t)f0.nnr * f0.nnx * f0.nnz);
+    std::vector<double> fine((size_t)cg.nnr * cg.nnx * cg.nnz);
     for (int m = 0; m < k; ++m) {
         rc = fmm3d_get_field(fc, m, fine.data());
         if (rc) return fail_to(err, rc, "refined field: %s", fc->err.c_str());
@@ -3308,7 +3352,7 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
         for (int r = lo[0]; r <= hi[0]; ++r)
             for (int x = lo[1]; x <= hi[1]; ++x)
                 for (int z = lo[2]; z <= hi[2]; ++z) {
-                    const size_t fq = ((size_t)(r - lo[0]) * refine * f.nnx + (size_t)(x - lo[1]) * refine) * f.nnz +
+                    const size_t fq = ((size_t)(r - lo[0]) * refine * cg.nnx + (size_t)(x - lo[1]) * refine) * cg.nnz +
                                       (size_t)(z - lo[2]) * refine;
                     node[q].push_back(((int64_t)r * g.nnx + x) * g.nnz + z);
                     tv[q].push_back(fine[fq]);
@@ -3341,31 +3385,35 @@ static int point_sets(fmm3d_ctx* c, const char* fn, int nsrc, const double* src,
     std::vector<std::vector<double>> pt(nsrc);
     std::vector<int> prc(nsrc, FMM3D_OK);
     std::vector<std::string> perr(nsrc);
-    // groups of sources whose refined blocks have one shape, solved as one
-    // batch on one refined context (at most 16 sources and 64 M nodes a
-    // group); largest blocks first, each worker taking the next group when
-    // it is free
-    std::vector<std::array<int64_t, 5>> order_by(nsrc);
+    // groups of sources solved as one batch on one refined context, whose
+    // grid holds each member's block (at most 32 sources and 128 M context
+    // nodes a group); largest blocks first, each worker taking the next
+    // group when it is free
+    std::vector<std::pair<int64_t, int>> order_by(nsrc);
+    std::vector<std::array<int, 3>> shape(nsrc);
     for (int q = 0; q < nsrc; ++q) {
-        int lo[3], hi[3];
-        std::array<int64_t, 3> sh{ 0, 0, 0 };
-        if (point_block(c->g, src + 3 * q, radius[q], lo, hi))
-            for (int a = 0; a < 3; ++a) sh[a] = (int64_t)(hi[a] - lo[a]) * refine + 1;
-        order_by[q] = { -sh[0] * sh[1] * sh[2], sh[0], sh[1], sh[2], q };
+        shape[q] = block_shape(c->g, src + 3 * q, radius[q], refine);
+        order_by[q] = { -(int64_t)shape[q][0] * shape[q][1] * shape[q][2], q };
     }
     std::sort(order_by.begin(), order_by.end());
     std::vector<std::vector<int>> groups;
     std::vector<std::array<int, 3>> gshape;
-    for (int k = 0; k < nsrc; ++k) {
-        const auto& o = order_by[k];
-        const std::array<int, 3> sh{ (int)o[1], (int)o[2], (int)o[3] };
-        const int64_t nodes = std::max<int64_t>(1, -o[0]);
-        const size_t cap = (size_t)std::max<int64_t>(1, std::min<int64_t>(16, (int64_t)64000000 / nodes));
-        if (groups.empty() || gshape.back() != sh || groups.back().size() >= cap) {
-            groups.emplace_back();
+    for (const auto& o : order_by) {
+        const int q = o.second;
+        const std::array<int, 3>& sh = shape[q];
+        bool join = !groups.empty() && sh[0] > 0 && gshape.back()[0] > 0 && groups.back().size() < 32;
+        std::array<int, 3> m{ 0, 0, 0 };
+        if (join) {
+            for (int a = 0; a < 3; ++a) m[a] = std::max(gshape.back()[a], sh[a]);
+            join = (double)(groups.back().size() + 1) * m[0] * m[1] * m[2] <= 128.0e6;
+        }
+        if (join) {
+            groups.back().push_back(q);
+            gshape.back() = m;
+        } else {
+            groups.push_back({ q });
             gshape.push_back(sh);
         }
-        groups.back().push_back((int)o[4]);
     }
     const int ng = (int)groups.size();
     const int nw = std::min(ng, c->point_workers);
