@@ -44,13 +44,15 @@ FMM3D_HIDDEN int fmm3d_engine_active(const fmm3d_ctx* c);
 // thread that prepares fixed sets beside the one that sweeps.
 FMM3D_HIDDEN fmm3d_ctx* fmm3d_engine_shadow(const fmm3d_grid_desc* g, const double* velv, int device);
 FMM3D_HIDDEN void fmm3d_engine_shadow_free(fmm3d_ctx* shadow);
-// fixed set of a point source (its refined neighbourhood, solved in
-// float64 on *fine, a refined-grid context created or resized here and
-// destroyed by the caller with fmm3d_destroy); reads only the grid, model
-// and device of c
-FMM3D_HIDDEN int fmm3d_engine_point_set(const fmm3d_ctx* c, fmm3d_ctx** fine, std::string& err, const double* src3,
-                                        int refine, int radius, int order, std::vector<int64_t>& node,
-                                        std::vector<double>& tv);
+// fixed sets of n point sources (src: n records of height, colatitude,
+// longitude), their refined neighbourhoods solved in float64 in groups on
+// *fine, a refined-grid context created or resized here and destroyed by
+// the caller with fmm3d_destroy; node[q] / tv[q] receive source q's set.
+// Reads only the grid, model and device of c
+FMM3D_HIDDEN int fmm3d_engine_point_sets(const fmm3d_ctx* c, fmm3d_ctx** fine, std::string& err, int n,
+                                         const double* src, int refine, int radius, int order,
+                                         std::vector<std::vector<int64_t>>& node,
+                                         std::vector<std::vector<double>>& tv);
 // device bytes: the model and metric tables; one slot of a precision
 FMM3D_HIDDEN size_t fmm3d_engine_model_bytes(const fmm3d_grid_desc* g);
 FMM3D_HIDDEN size_t fmm3d_engine_slot_bytes(const fmm3d_grid_desc* g, int precision);

@@ -207,9 +207,11 @@ void fmm3d_server::reap()
     }
 }
 
-// A prep worker: fixed sets of point sources, one source at a time from the
-// oldest request with sources left, beside the scheduler's sweeps.  Each
-// worker has its own refined-grid context (and stream).
+// A prep worker: fixed sets of point sources, beside the scheduler's
+// sweeps.  It takes a run of sources from the oldest request with sources
+// left -- an even share of what is left among the workers, at most 32 --
+// and solves their refined blocks together.  Each worker has its own
+// refined-grid context (and stream).
 void fmm3d_server::prep()
 {
     // the scheduler may be capturing a sweep graph meanwhile
@@ -219,12 +221,12 @@ void fmm3d_server::prep()
     fmm3d_ctx* fine = nullptr;
     fmm3d_ctx* shadow = nullptr;
     std::string shadow_key;
-    std::vector<int64_t> node;
-    std::vector<double> tv;
+    std::vector<std::vector<int64_t>> node;
+    std::vector<std::vector<double>> tv;
     std::string err;
     for (;;) {
         Request* r = nullptr;
-        int q = -1;
+        int q = -1, b = 0;
         {
             std::unique_lock<std::mutex> lk(mu);
             cv_prep.wait(lk, [&] { return stop || !prep_q.empty(); });
@@ -235,7 +237,9 @@ void fmm3d_server::prep()
                 reap();   // a failed request may have been released meanwhile
                 continue;
             }
-            q = r->next_prep++;
+            q = r->next_prep;
+            b = std::min(32, std::max(1, (r->nsrc - q + PREP_WORKERS - 1) / PREP_WORKERS));
+            r->next_prep += b;
             if (r->next_prep >= r->nsrc) prep_q.pop_front();
             r->prep_busy++;
         }
@@ -246,19 +250,19 @@ void fmm3d_server::prep()
             shadow = fmm3d_engine_shadow(&r->grid, r->velv.data(), device);
             shadow_key = key;
         }
-        node.clear();
-        tv.clear();
-        const int rc = fmm3d_engine_point_set(shadow, &fine, err, r->src.data() + 3 * q, r->refine, r->radius,
-                                              r->order, node, tv);
+        const int rc = fmm3d_engine_point_sets(shadow, &fine, err, b, r->src.data() + 3 * q, r->refine, r->radius,
+                                               r->order, node, tv);
         if (rc != FMM3D_OK) {
             trace_failure(r, rc, err.c_str());
             finish(r, rc);
         }
         std::lock_guard<std::mutex> lk(mu);
         if (rc == FMM3D_OK) {
-            r->pnode[q].swap(node);
-            r->ptv[q].swap(tv);
-            r->pready[q] = 1;
+            for (int i = 0; i < b; ++i) {
+                r->pnode[q + i].swap(node[i]);
+                r->ptv[q + i].swap(tv[i]);
+                r->pready[q + i] = 1;
+            }
             while (r->prepared < r->nsrc && r->pready[r->prepared]) r->prepared++;
             cv_work.notify_all();
         }

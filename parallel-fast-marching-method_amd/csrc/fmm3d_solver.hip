@@ -3361,17 +3361,9 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
     return FMM3D_OK;
 }
 
-// One point source's refined neighbourhood (a group of one).
-static int point_fixed_set(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, const double* src3, int refine,
-                           int radius, int order, std::vector<int64_t>& node, std::vector<double>& tv)
-{
-    const std::vector<int> one{ 0 };
-    return point_group(c, finep, err, one, src3, &radius, refine, order, &node, &tv);
-}
-
 // Fixed sets of point sources, source q refined over radius[q] nodes.
-// Sources whose refined blocks have one shape are solved together, one
-// slot each on a refined-block context; up to point_workers threads run
+// Sources are solved in groups, one slot each on a refined-block context
+// whose grid holds every member's block; up to point_workers threads run
 // such groups at once, each on its own context and stream, and the sets
 // are joined in source order.
 static int point_sets(fmm3d_ctx* c, const char* fn, int nsrc, const double* src, int refine, const int* radius,
@@ -4108,12 +4100,40 @@ void fmm3d_engine_shadow_free(fmm3d_ctx* c)
     delete c;   // holds host memory only
 }
 
-int fmm3d_engine_point_set(const fmm3d_ctx* c, fmm3d_ctx** fine, std::string& err, const double* src3, int refine,
-                           int radius, int order, std::vector<int64_t>& node, std::vector<double>& tv)
+int fmm3d_engine_point_sets(const fmm3d_ctx* c, fmm3d_ctx** fine, std::string& err, int n, const double* src,
+                            int refine, int radius, int order, std::vector<std::vector<int64_t>>& node,
+                            std::vector<std::vector<double>>& tv)
 {
     if (!c || !c->have_model) return fail_to(err, FMM3D_E_STATE, "engine: no model");
+    if (n < 1 || !src) return fail_to(err, FMM3D_E_ARG, "engine: no point sources");
     if (refine < 1 || refine > 16 || radius < 1 || radius > 64) return fail_to(err, FMM3D_E_ARG, "engine: bad refinement");
-    return point_fixed_set(c, fine, err, src3, refine, radius, order, node, tv);
+    node.assign(n, {});
+    tv.assign(n, {});
+    const std::vector<int> rad(n, radius);
+    // in order, groups of at most 32 sources and 128 M context nodes
+    std::vector<int> members;
+    std::array<int, 3> m{ 0, 0, 0 };
+    for (int q = 0; q <= n; ++q) {
+        std::array<int, 3> sh{ 0, 0, 0 }, mm = m;
+        bool join = false;
+        if (q < n) {
+            sh = block_shape(c->g, src + 3 * q, radius, refine);
+            for (int a = 0; a < 3; ++a) mm[a] = std::max(m[a], sh[a]);
+            join = members.empty() ||
+                   (members.size() < 32 && (double)(members.size() + 1) * mm[0] * mm[1] * mm[2] <= 128.0e6);
+        }
+        if (!join && !members.empty()) {
+            const int rc = point_group(c, fine, err, members, src, rad.data(), refine, order, node.data(), tv.data());
+            if (rc) return rc;
+            members.clear();
+            mm = sh;
+        }
+        if (q < n) {
+            members.push_back(q);
+            m = mm;
+        }
+    }
+    return FMM3D_OK;
 }
 
 size_t fmm3d_engine_model_bytes(const fmm3d_grid_desc* d)
