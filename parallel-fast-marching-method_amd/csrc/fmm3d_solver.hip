@@ -1697,6 +1697,8 @@ struct fmm3d_ctx {
     size_t st_offs_n = 0, st_n = 0;
     double* d_velv = nullptr;      // the vertex model on the device (gridder input)
     size_t d_velv_n = 0;
+    double* d_gather = nullptr;    // refined contexts: the coarse nodes of one block
+    size_t d_gather_n = 0;
     int* d_bad = nullptr;          // first source with a NaN boundary time (or the source count)
     double* d_sides = nullptr;     // side faces of the last teleseismic solve: theta faces, then phi faces
     int* d_side_of = nullptr;      // per source: theta side, phi side (-1 none, 0 low index, 1 high)
@@ -1976,6 +1978,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->gslo64); (void)hipFree(c->gslo32);
     (void)hipFree(c->ghx64); (void)hipFree(c->ghz64); (void)hipFree(c->ghx32); (void)hipFree(c->ghz32);
     (void)hipFree(c->d_sdims);
+    (void)hipFree(c->d_gather);
     (void)hipFree(c->hx64); (void)hipFree(c->hz64); (void)hipFree(c->hx32); (void)hipFree(c->hz32);
     (void)hipFree(c->d_updates); (void)hipFree(c->d_visits); (void)hipFree(c->ray_scratch);
     (void)hipFree(c->fr_path); (void)hipFree(c->fr_pn); (void)hipFree(c->fr_pbox);
@@ -2203,6 +2206,19 @@ __global__ void k_fill_real(R* __restrict__ p, size_t n, R v)
 __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// The coarse nodes a refined block covers: every refine-th node of the
+// block's float64 field T (laid out with row strides nnx x nnz), cr x cx x
+// cz of them, into out in (r, theta, phi) order.
+__global__ void k_gather_coarse(const double* __restrict__ T, int cr, int cx, int cz, int refine, int nnx, int nnz,
+                                double* __restrict__ out)
+{
+    const size_t n = (size_t)cr * cx * cz;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int z = (int)(i % cz), x = (int)((i / cz) % cx), r = (int)(i / ((size_t)cz * cx));
+        out[i] = T[((size_t)r * refine * nnx + (size_t)x * refine) * nnz + (size_t)z * refine];
+    }
 }
 
 template <typename R>
@@ -3340,22 +3356,37 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
     // 1.8 ms (61^3, refinement 5 over 10 nodes)
     rc = fmm3d_solve_fixed(fc, k, offs.data(), fnode.data(), ftime.data(), order, FMM3D_PREC_FP64, FMM3D_WALK_ALTERNATE);
     if (rc) return fail_to(err, rc, "refined solve: %s", fc->err.c_str());
-    std::vector<double> fine((size_t)cg.nnr * cg.nnx * cg.nnz);
+    // the coarse nodes each block covers seed the coarse solve: gathered on
+    // the device, so only they cross to the host
+    std::vector<double> cv;
     for (int m = 0; m < k; ++m) {
-        rc = fmm3d_get_field(fc, m, fine.data());
-        if (rc) return fail_to(err, rc, "refined field: %s", fc->err.c_str());
-        // the coarse nodes the block covers seed the coarse solve
         const int q = members[m];
         const Grid& f = fg[m];
         const int* lo = blo[m].data();
-        const int hi[3] = { lo[0] + (f.nnr - 1) / refine, lo[1] + (f.nnx - 1) / refine, lo[2] + (f.nnz - 1) / refine };
-        for (int r = lo[0]; r <= hi[0]; ++r)
-            for (int x = lo[1]; x <= hi[1]; ++x)
-                for (int z = lo[2]; z <= hi[2]; ++z) {
-                    const size_t fq = ((size_t)(r - lo[0]) * refine * cg.nnx + (size_t)(x - lo[1]) * refine) * cg.nnz +
-                                      (size_t)(z - lo[2]) * refine;
+        const int cr = (f.nnr - 1) / refine + 1, cx = (f.nnx - 1) / refine + 1, cz = (f.nnz - 1) / refine + 1;
+        const size_t n = (size_t)cr * cx * cz;
+        if (fc->d_gather_n < n) {
+            (void)hipFree(fc->d_gather);
+            fc->d_gather = nullptr;
+            fc->d_gather_n = 0;
+            if (dalloc(&fc->d_gather, n * sizeof(double)) != hipSuccess)
+                return fail_to(err, FMM3D_E_NOMEM, "refined field: no device memory for %zu coarse nodes", n);
+            fc->d_gather_n = n;
+        }
+        cv.resize(n);
+        k_gather_coarse<<<(int)std::min<size_t>((n + 255) / 256, 4096), 256, 0, fc->stream>>>(
+            (const double*)fc->sl.T + (size_t)m * fc->N, cr, cx, cz, refine, cg.nnx, cg.nnz, fc->d_gather);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(cv.data(), fc->d_gather, n * sizeof(double), hipMemcpyDeviceToHost, fc->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(fc->stream);
+        if (e != hipSuccess) return fail_to(err, FMM3D_E_DEVICE, "refined field: %s", hipGetErrorString(e));
+        size_t i = 0;
+        for (int r = lo[0]; r < lo[0] + cr; ++r)
+            for (int x = lo[1]; x < lo[1] + cx; ++x)
+                for (int z = lo[2]; z < lo[2] + cz; ++z, ++i) {
                     node[q].push_back(((int64_t)r * g.nnx + x) * g.nnz + z);
-                    tv[q].push_back(fine[fq]);
+                    // an unreached node reads as 0, the reference's marker
+                    tv[q].push_back(isfinite(cv[i]) ? cv[i] : 0.0);
                 }
     }
     return FMM3D_OK;
