@@ -218,6 +218,14 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *                   it, as ordered sources wait along their flow; 0 no
  *                   ordering.  Refined-block solves follow the setting
  *   "point_workers" threads solving refined blocks at once (1..16, default 4)
+ *   "point_group_mnodes" refined blocks solved together as one group: at
+ *                   most this many million nodes of the group's grid
+ *                   (1..1024, default 128)
+ *   "point_window"  alternating walks: 1 (default) a patch walks only the
+ *                   layers within two of one that changed, in it or a
+ *                   lateral neighbour, in the sweep before or so far in
+ *                   this one, and on while its own walk changes layers;
+ *                   0 whole columns
  *   "spin_limit"    polls before such a wait is given up (default 20000)
  *   "check_every"   column walk: sweeps between convergence checks (1..8,
  *                   default 4); at a check, every source whose last sweep

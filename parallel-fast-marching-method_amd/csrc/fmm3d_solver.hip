@@ -411,7 +411,15 @@ template <typename R> struct ColArgs {
     size_t geo_n, geo_hx, geo_hz;   // per-slot strides of slo, hx, hz (slots on grids of their own), or 0 (shared)
     const int* sdims;       // per slot (nnr, nnx, nnz) of its own grid inside the nnr x nnx x nnz
                             // container (strides stay the container's), or null: the container's
+    unsigned long long* lrng;   // alternating walks: per slot per patch, two entries (by sweep parity) of
+                                // (sweep << 32 | first changed layer << 16 | last changed layer), or null
 };
+
+// A layer-range entry: the sweep it belongs to and the layers lo..hi that changed.
+__host__ __device__ __forceinline__ unsigned long long lrng_pack(int sweep, int lo, int hi)
+{
+    return ((unsigned long long)(unsigned)sweep << 32) | ((unsigned long long)(lo & 0xffff) << 16) | (unsigned)(hi & 0xffff);
+}
 
 // float32 helpers for the axis-sum octant form: fused multiply-subtract,
 // hardware square root and reciprocal (about 1 ulp each).
@@ -525,6 +533,7 @@ k_column(ColArgs<R> a)
     __shared__ R b1[NP], b2[NP], u1[NP], u2[NP], sl[NP];
     __shared__ R shz[PX], sihz[PX];
     __shared__ int s_act;
+    __shared__ int s_win[2];
     const int np = a.npx * a.npz;
     // Block -> patch.  A row of patches (one slot, one theta index, every phi
     // index) runs on one XCD -- workgroups go round-robin over the 8 XCDs --
@@ -582,6 +591,38 @@ k_column(ColArgs<R> a)
         s_act = (m >= iter - 1) && bx * PX < snx && bz * PZ < snz;
         // an idle patch is finished with every layer of this sweep
         if (!s_act && pg) __hip_atomic_store(pg + b, pbase + a.nnr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // layer window (alternating walks): the layers within two of one
+        // that changed, in this patch or a lateral neighbour, in the sweep
+        // before or so far in this one.  Every node outside it was relaxed
+        // after its inputs last changed; a change inside the walk carries
+        // it on past the window (below).
+        int lo = 0, hi = snr - 1;
+        if (s_act && alt && a.lrng) {
+            const unsigned long long* lr = a.lrng + (size_t)slot * np * 2;
+            int l = 0x7fffffff, h = -1;
+            const int nb[5] = { b, bx > 0 ? b - a.npz : -1, bx < a.npx - 1 ? b + a.npz : -1,
+                                bz > 0 ? b - 1 : -1, bz < a.npz - 1 ? b + 1 : -1 };
+            for (int j = 0; j < 5; ++j) {
+                if (nb[j] < 0) continue;
+                for (int par = 0; par < 2; ++par) {
+                    const unsigned long long v = __hip_atomic_load(lr + 2 * nb[j] + par, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const int tag = (int)(unsigned)(v >> 32);
+                    if (tag == iter || tag == iter - 1) {
+                        l = min(l, (int)((v >> 16) & 0xffff));
+                        h = max(h, (int)(v & 0xffff));
+                    }
+                }
+            }
+            if (h < 0) {   // nothing changed near it: no layer to walk
+                lo = snr;
+                hi = snr - 1;
+            } else {
+                lo = max(0, l - 2);
+                hi = min(snr - 1, h + 2);
+            }
+        }
+        s_win[0] = lo;
+        s_win[1] = hi;
     }
     __syncthreads();
     if (!s_act) return;
@@ -604,7 +645,15 @@ k_column(ColArgs<R> a)
     // walk: bottom to top, or top to bottom when a.down; "prev" layers are the
     // two this block relaxed just before (b1, b2), "next" ones come from HBM (u1, u2)
     const int dn = down ? 1 : -1;
-    for (int q = 0; q < snr; ++q) {
+    // walk steps: from the window's first layer in walk order, on past its
+    // last while a layer within the two before changed
+    const bool win = alt && a.lrng;
+    const int qs = down ? s_win[0] : snr - 1 - s_win[1];
+    const int qw = down ? s_win[1] : snr - 1 - s_win[0];
+    int lastq = -1000, chg_lo = 0x7fffffff, chg_hi = -1;
+    if (pg && t == 0 && qs > 0) __hip_atomic_store(pg + b, pbase + qs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int q = qs; q < snr; ++q) {
+        if (q > qw && q - lastq > 2) break;
         const int ir = down ? q : snr - 1 - q;
         const size_t lay = (size_t)ir * NL;
         unsigned nfree = 0;
@@ -650,7 +699,11 @@ k_column(ColArgs<R> a)
             const bool fixed = !in || ((fb[c >> 5] >> (c & 31)) & 1u);
             sl[e] = fixed ? R(-1) : slo[c];
             nfree += fixed ? 0u : 1u;
-            if (q == 0) { b1[e] = inf; b2[e] = inf; }
+            // the walk's first layer: the two before it come from HBM (none at the edge)
+            if (q == qs) {
+                b1[e] = (in && q >= 1) ? T[c - (ptrdiff_t)dn * (ptrdiff_t)NL] : inf;
+                b2[e] = (in && q >= 2) ? T[c - (ptrdiff_t)(2 * dn) * (ptrdiff_t)NL] : inf;
+            }
         }
         // lateral halo ring
         for (int e = t; e < LX * (PZ + 4); e += 64) {
@@ -721,6 +774,7 @@ k_column(ColArgs<R> a)
         nvis += nfree;
         __syncthreads();
         // write back what changed; the layer becomes the one below
+        bool lay_chg = false;
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const int e = t + 64 * k;
@@ -731,10 +785,16 @@ k_column(ColArgs<R> a)
                 __hip_atomic_store(T + lay + (size_t)(x0 + lx) * a.nnz + z0 + lz, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const bool fin_n = nv < inf, fin_t = tc < inf;
                 if (fin_n != fin_t || !(fabs((double)nv - (double)tc) <= a.tol * fmax(fabs((double)nv), fabs((double)tc))))
-                    anyv = true;
+                    lay_chg = true;
             }
             b2[e] = b1[e];
             b1[e] = nv;
+        }
+        anyv |= lay_chg;
+        if (win && __ballot(lay_chg)) {
+            lastq = q;
+            chg_lo = min(chg_lo, ir);
+            chg_hi = max(chg_hi, ir);
         }
         // publish the layer once its stores have completed
         __syncthreads();
@@ -743,6 +803,11 @@ k_column(ColArgs<R> a)
             __hip_atomic_store(pg + b, pbase + q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+    // layers the walk did not reach are finished too
+    if (pg && t == 0) __hip_atomic_store(pg + b, pbase + a.nnr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (win && t == 0 && chg_hi >= 0)
+        __hip_atomic_store(a.lrng + ((size_t)slot * np + b) * 2 + (iter & 1), lrng_pack(iter, chg_lo, chg_hi),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     {
         unsigned long long w = nupd, wv = nvis;
         for (int off = 32; off > 0; off >>= 1) { w += __shfl_down(w, off, 64); wv += __shfl_down(wv, off, 64); }
@@ -780,7 +845,9 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
          i += (size_t)gridDim.x * blockDim.x) {
         const int slot = (int)(i / NL);
         const size_t f = i - (size_t)slot * NL;
-        if (!a.slot_on[slot] || (a.born && a.born[slot] != iter)) continue;
+        // alternating walks (point sources) are never at their fixed point
+        // after one sweep: they are not checked
+        if (!a.slot_on[slot] || (a.born && a.born[slot] != iter) || (a.walkalt && a.walkalt[slot])) continue;
         const int ix = (int)(f / a.nnz), iz = (int)(f - (size_t)ix * a.nnz);
         // this slot's own grid inside the container
         const int snr = a.sdims ? a.sdims[3 * slot] : a.nnr;
@@ -844,14 +911,12 @@ __global__ void __launch_bounds__(256) k_verify_apply(ColArgs<R> a, int* on)
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)a.nslots * np;
          i += (size_t)gridDim.x * blockDim.x) {
         const int slot = (int)(i / np);
-        if (!a.slot_on[slot] || (a.born && a.born[slot] != iter)) continue;
+        if (!a.slot_on[slot] || (a.born && a.born[slot] != iter) || (a.walkalt && a.walkalt[slot])) continue;
         if (!a.slotpend[slot]) {
             if (i == (size_t)slot * np) on[slot] = 0;
             continue;
         }
-        // narrowing only for flow-ordered slots: alternating walks (point
-        // sources) need their changes carried across the whole slot
-        if (a.verify_patches && !(a.walkalt && a.walkalt[slot]) && !a.pend[i]) a.stamp[i] = min(a.stamp[i], iter - 1);
+        if (a.verify_patches && !a.pend[i]) a.stamp[i] = min(a.stamp[i], iter - 1);
     }
 }
 
@@ -1616,6 +1681,7 @@ struct Slots {
     double* srcpos = nullptr; // per slot point-source position (height, colat, lon), NaN if none
     int* walkalt = nullptr;   // per slot: alternate the column walk direction
     int* pend = nullptr;      // column walk: per slot per patch, a node would move (fixed-point check)
+    unsigned long long* lrng = nullptr;   // column walk, alternating slots: per slot per patch, changed layers of two sweeps
     int* slotpend = nullptr;  // column walk: per slot, some node would move
     int* born = nullptr;      // serving engine: per slot, the sweep it joined at
 };
@@ -1677,6 +1743,7 @@ struct fmm3d_ctx {
     fmm3d_ctx* fine = nullptr;     // refined source neighbourhood (point sources)
     std::vector<fmm3d_ctx*> fines; // the same for batches of point sources: a pool kept by block shape
     int point_workers = 4;         // threads running a batch's refined-block solves
+    int point_group_mnodes = 128;  // context nodes (millions) of one group of refined blocks at most
     // continuous-batching engine
     bool eng_open = false;
     int eng_k = 1, eng_order = FMM3D_ORDER_MIXED, eng_pending = 0;
@@ -1719,6 +1786,7 @@ struct fmm3d_ctx {
     int spin_limit = 20000;        // polls (~64 cycles each) before such a wait is given up
     int max_sweeps = 0;            // column walk: stop after this many sweeps (0: at convergence)
     int point_order = 1;           // alternating walks: patches outward from the earliest fixed node, waiting on their inner neighbours
+    int point_window = 1;          // alternating walks: each patch walks the layers near the last sweeps' changes only
     int trace = 0;                 // per-sweep counters to stderr (FMM3D_TRACE)
     int timing = 0;                // time every relaxation launch
     hipEvent_t kev[2 * 8] = {};    // per-launch events (timing)
@@ -1948,7 +2016,7 @@ static void free_slots(Slots& s)
     (void)hipFree(s.T); (void)hipFree(s.P); (void)hipFree(s.fix); (void)hipFree(s.stamp);
     (void)hipFree(s.lastchg); (void)hipFree(s.on); (void)hipFree(s.toff);
     (void)hipFree(s.valchg); (void)hipFree(s.prog); (void)hipFree(s.flow); (void)hipFree(s.srcpos); (void)hipFree(s.walkalt);
-    (void)hipFree(s.pend); (void)hipFree(s.slotpend); (void)hipFree(s.born);
+    (void)hipFree(s.pend); (void)hipFree(s.slotpend); (void)hipFree(s.born); (void)hipFree(s.lrng);
     s = Slots();
 }
 
@@ -2063,6 +2131,7 @@ static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
     HIPCHK(c, dalloc(&s.srcpos, (size_t)nsrc * 3 * sizeof(double)));
     HIPCHK(c, dalloc(&s.walkalt, (size_t)nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.pend, (size_t)nsrc * nbmax * sizeof(int)));
+    HIPCHK(c, dalloc(&s.lrng, (size_t)nsrc * nbmax * 2 * sizeof(unsigned long long)));
     HIPCHK(c, dalloc(&s.slotpend, (size_t)nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.born, (size_t)nsrc * sizeof(int)));
     s.cap = nsrc;
@@ -2206,6 +2275,17 @@ __global__ void k_fill_real(R* __restrict__ p, size_t n, R v)
 __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Layer ranges of slots whose first sweep is `first`: every layer changed
+// in the sweep before (so that sweep walks whole columns), nothing in the
+// other parity's entry.
+__global__ void k_lrng_init(unsigned long long* __restrict__ p, size_t n, int first)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        p[2 * i + ((first - 1) & 1)] = lrng_pack(first - 1, 0, 0xffff);
+        p[2 * i + (first & 1)] = lrng_pack(-1000, 0, 0);
+    }
 }
 
 // The coarse nodes a refined block covers: every refine-th node of the
@@ -2513,6 +2593,7 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     a.pend = s.pend;
     a.slotpend = s.slotpend;
     a.born = nullptr;
+    a.lrng = c->point_window ? s.lrng : nullptr;
     a.geo_n = a.geo_hx = a.geo_hz = 0;
     a.sdims = nullptr;
     if (c->geo_cap > 0) {
@@ -2577,8 +2658,9 @@ static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k, bo
             hipError_t le = launch_column<R, FAST, SHIFT>(c, a, blocks, kd);
             HIPCHK(c, le);
             if (c->timing) HIPCHK(c, hipEventRecord(c->kev[2 * j + 1], c->stream));
-            // engine slots join at a chunk's first sweep: their checks follow it
-            if (kd == 1 && a.verify > 0 && (!engine || j == 0)) {
+            // engine slots join at a chunk's first sweep: their checks follow
+            // it; batches of alternating walks alone are not checked
+            if (kd == 1 && a.verify > 0 && (!engine || j == 0) && (engine || c->walk_alt != 1)) {
                 le = launch_verify<R, FAST, SHIFT>(c, a);
                 HIPCHK(c, le);
             }
@@ -2602,10 +2684,11 @@ static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k, bo
     }
     a.iter = 0;
     a.iterp = c->d_iter;
-    std::vector<char> sig(sizeof a + 2 * sizeof(int));
+    std::vector<char> sig(sizeof a + 3 * sizeof(int));
     memcpy(sig.data(), &a, sizeof a);
-    const int tag[2] = { blocks, (int)sizeof(R) * 100 + kind * 10000000 + c->strategy * 1000000 +
-                                     c->plane_ch * 100000 + c->col_px * 10000 + c->col_pz };
+    // the walk mode decides whether the graph holds the fixed-point check
+    const int tag[3] = { blocks, (int)sizeof(R) * 100 + kind * 10000000 + c->strategy * 1000000 +
+                                     c->plane_ch * 100000 + c->col_px * 10000 + c->col_pz, c->walk_alt };
     memcpy(sig.data() + sizeof a, tag, sizeof tag);
     if (!c->gexec || sig != c->g_sig) {
         if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
@@ -2617,7 +2700,7 @@ static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k, bo
         hipError_t e = hipSuccess;
         for (int j = 0; j < n && e == hipSuccess; ++j) {
             e = launch_column<R, FAST, SHIFT>(c, a, blocks, kind);
-            if (e == hipSuccess && kind == 1 && a.verify > 0) e = launch_verify<R, FAST, SHIFT>(c, a);
+            if (e == hipSuccess && kind == 1 && a.verify > 0 && c->walk_alt != 1) e = launch_verify<R, FAST, SHIFT>(c, a);
             if (e == hipSuccess) {
                 k_next_iter<<<1, 64, 0, c->stream>>>(c->d_iter);
                 e = hipGetLastError();
@@ -2660,6 +2743,7 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     const int maxit = 4 * (g.nnr + g.nnx + g.nnz) + 200;
     std::vector<int> lc(nsrc);
     k_stamp_init<<<std::min(4096, (blocks + 255) / 256), 256, 0, c->stream>>>(nsrc, np, 1, 1, 1, s.stamp);
+    k_lrng_init<<<(nsrc * np + 255) / 256, 256, 0, c->stream>>>(s.lrng, (size_t)nsrc * np, 1);
     k_fill_int<<<(nsrc * np + 255) / 256, 256, 0, c->stream>>>(s.pend, nsrc * np, 0);
     k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.slotpend, nsrc, 0);
     k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.lastchg, nsrc, 0);
@@ -3428,7 +3512,7 @@ static int point_sets(fmm3d_ctx* c, const char* fn, int nsrc, const double* src,
         std::array<int, 3> m{ 0, 0, 0 };
         if (join) {
             for (int a = 0; a < 3; ++a) m[a] = std::max(gshape.back()[a], sh[a]);
-            join = (double)(groups.back().size() + 1) * m[0] * m[1] * m[2] <= 128.0e6;
+            join = (double)(groups.back().size() + 1) * m[0] * m[1] * m[2] <= c->point_group_mnodes * 1.0e6;
         }
         if (join) {
             groups.back().push_back(q);
@@ -3899,6 +3983,7 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
         // every patch column of the new slot runs in the next sweep
         k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.stamp + (size_t)slot * np, np, c->eng_k - 1);
         k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.prog + (size_t)slot * np, np, 0);
+        k_lrng_init<<<(np + 255) / 256, 256, 0, c->stream>>>(s.lrng + (size_t)slot * np * 2, np, c->eng_k);
         k_fill_int<<<1, 1, 0, c->stream>>>(s.lastchg + slot, 1, c->eng_k);
         k_fill_int<<<1, 1, 0, c->stream>>>(s.walkalt + slot, 1, walkalt ? 1 : 0);
         k_fill_int<<<1, 1, 0, c->stream>>>(s.on + slot, 1, 1);
@@ -3920,7 +4005,7 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
 __global__ void k_slot_setup(int slot, int np, int k, int walkalt, int* __restrict__ stamp, int* __restrict__ prog,
                              int* __restrict__ pend, int* __restrict__ lastchg, int* __restrict__ walk,
                              int* __restrict__ born, int* __restrict__ slotpend, int* __restrict__ on,
-                             double* __restrict__ srcpos)
+                             double* __restrict__ srcpos, unsigned long long* __restrict__ lrng)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < np) {
@@ -3928,6 +4013,8 @@ __global__ void k_slot_setup(int slot, int np, int k, int walkalt, int* __restri
         stamp[p] = k - 1;
         prog[p] = 0;
         pend[p] = 0;
+        lrng[2 * p + ((k - 1) & 1)] = lrng_pack(k - 1, 0, 0xffff);
+        lrng[2 * p + (k & 1)] = lrng_pack(-1000, 0, 0);
     }
     if (i == 0) {
         lastchg[slot] = k;
@@ -3970,7 +4057,7 @@ int fmm3d_engine_admit_face(fmm3d_ctx* c, int slot, const double* face)
         // the slot's per-patch and per-slot state in one launch; every patch
         // column of the new slot runs in the next sweep
         k_slot_setup<<<(np + 255) / 256, 256, 0, c->stream>>>(slot, np, c->eng_k, 0, s.stamp, s.prog, s.pend, s.lastchg,
-                                                             s.walkalt, s.born, s.slotpend, s.on, s.srcpos);
+                                                             s.walkalt, s.born, s.slotpend, s.on, s.srcpos, s.lrng);
         e = hipGetLastError();
     }
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "engine admit: %s", hipGetErrorString(e));
@@ -4241,9 +4328,15 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "graph")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "graph must be 0 or 1");
         c->use_graph = iv;
+    } else if (!strcmp(key, "point_group_mnodes")) {
+        if (iv < 1 || iv > 1024) return fail(c, FMM3D_E_ARG, "point_group_mnodes must be in 1..1024");
+        c->point_group_mnodes = iv;
     } else if (!strcmp(key, "point_workers")) {
         if (iv < 1 || iv > 16) return fail(c, FMM3D_E_ARG, "point_workers must be in 1..16");
         c->point_workers = iv;
+    } else if (!strcmp(key, "point_window")) {
+        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "point_window must be 0 or 1");
+        c->point_window = iv;
     } else if (!strcmp(key, "point_order")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "point_order must be 0 or 1");
         c->point_order = iv;
