@@ -226,6 +226,9 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *                   lateral neighbour, in the sweep before or so far in
  *                   this one, and on while its own walk changes layers;
  *                   0 whole columns
+ *   "point_warm"    refined blocks: 1 (default) a float32 solve first, then
+ *                   float64 sweeps from its field to the float64 fixed
+ *                   point; 0 float64 from the start
  *   "spin_limit"    polls before such a wait is given up (default 20000)
  *   "check_every"   column walk: sweeps between convergence checks (1..8,
  *                   default 4); at a check, every source whose last sweep

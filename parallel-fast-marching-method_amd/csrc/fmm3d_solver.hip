@@ -1667,6 +1667,7 @@ struct Slots {
     int cap = 0;           // allocated slots
     int n = 0;             // resident sources
     int prec = FMM3D_PREC_FP64;
+    size_t es = 0;         // bytes per node of T as allocated (a float64 allocation also holds float32 fields)
     void* T = nullptr;     // fields
     void* P = nullptr;     // ordering fields
     uint32_t* fix = nullptr;
@@ -1766,6 +1767,9 @@ struct fmm3d_ctx {
     size_t d_velv_n = 0;
     double* d_gather = nullptr;    // refined contexts: the coarse nodes of one block
     size_t d_gather_n = 0;
+    float* d_warm = nullptr;       // refined contexts: float32 fields of a warm start
+    double* d_warm_off = nullptr;  // and their offsets
+    size_t d_warm_n = 0;
     int* d_bad = nullptr;          // first source with a NaN boundary time (or the source count)
     double* d_sides = nullptr;     // side faces of the last teleseismic solve: theta faces, then phi faces
     int* d_side_of = nullptr;      // per source: theta side, phi side (-1 none, 0 low index, 1 high)
@@ -1787,6 +1791,7 @@ struct fmm3d_ctx {
     int max_sweeps = 0;            // column walk: stop after this many sweeps (0: at convergence)
     int point_order = 1;           // alternating walks: patches outward from the earliest fixed node, waiting on their inner neighbours
     int point_window = 1;          // alternating walks: each patch walks the layers near the last sweeps' changes only
+    int point_warm = 1;            // refined blocks: a float32 solve first, then float64 sweeps from its field
     int trace = 0;                 // per-sweep counters to stderr (FMM3D_TRACE)
     int timing = 0;                // time every relaxation launch
     hipEvent_t kev[2 * 8] = {};    // per-launch events (timing)
@@ -2047,6 +2052,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->ghx64); (void)hipFree(c->ghz64); (void)hipFree(c->ghx32); (void)hipFree(c->ghz32);
     (void)hipFree(c->d_sdims);
     (void)hipFree(c->d_gather);
+    (void)hipFree(c->d_warm); (void)hipFree(c->d_warm_off);
     (void)hipFree(c->hx64); (void)hipFree(c->hz64); (void)hipFree(c->hx32); (void)hipFree(c->hz32);
     (void)hipFree(c->d_updates); (void)hipFree(c->d_visits); (void)hipFree(c->ray_scratch);
     (void)hipFree(c->fr_path); (void)hipFree(c->fr_pn); (void)hipFree(c->fr_pbox);
@@ -2113,10 +2119,13 @@ static Bricks bricks_of(const fmm3d_ctx* c)
 static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
 {
     Slots& s = c->sl;
-    if (s.cap >= nsrc && s.prec == prec) return FMM3D_OK;
+    const size_t es = prec == FMM3D_PREC_FP32 ? sizeof(float) : sizeof(double);
+    if (s.cap >= nsrc && (s.prec == prec || s.es >= es)) {
+        s.prec = prec;
+        return FMM3D_OK;
+    }
     free_slots(s);
     const size_t N = c->N;
-    const size_t es = prec == FMM3D_PREC_FP32 ? sizeof(float) : sizeof(double);
     // stamps for the largest brick count any supported shape gives (4-layer bricks, 4x8 patches)
     const size_t nbmax = (size_t)((c->g.nnr + 3) / 4) * ((c->g.nnx + 3) / 4) * ((c->g.nnz + 7) / 8);
     HIPCHK(c, dalloc(&s.T, (size_t)nsrc * N * es));
@@ -2136,6 +2145,7 @@ static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
     HIPCHK(c, dalloc(&s.born, (size_t)nsrc * sizeof(int)));
     s.cap = nsrc;
     s.prec = prec;
+    s.es = es;
     return FMM3D_OK;
 }
 
@@ -2275,6 +2285,21 @@ __global__ void k_fill_real(R* __restrict__ p, size_t n, R v)
 __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Free nodes of float64 fields T take the float32 fields W plus their
+// slot's offset (fixed nodes keep their times): the start of a float64
+// relaxation from a float32 one.
+__global__ void k_warm_start(double* __restrict__ T, const float* __restrict__ W, const double* __restrict__ woff,
+                             const uint32_t* __restrict__ fix, int nslots, size_t N)
+{
+    const size_t NW = (N + 31) / 32;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)nslots * N; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t slot = i / N, k = i - slot * N;
+        if ((fix[slot * NW + (k >> 5)] >> (k & 31)) & 1u) continue;
+        const float w = W[i];
+        T[i] = w < __builtin_huge_valf() ? woff[slot] + (double)w : __builtin_huge_val();
+    }
 }
 
 // Layer ranges of slots whose first sweep is `first`: every layer changed
@@ -3398,6 +3423,7 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
     }
     fc->g = cg;
     fc->point_order = c->point_order;
+    fc->point_window = c->point_window;
     // the vertex model (its gridding onto the shared arrays goes unused:
     // every slot has its own block)
     int rc = fmm3d_set_model_vtx(fc, velv);
@@ -3437,8 +3463,39 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
         offs[m + 1] = 8 * (int64_t)(m + 1);
     }
     // float64: in float32 the refined block moved the coarse field by up to
-    // 1.8 ms (61^3, refinement 5 over 10 nodes)
-    rc = fmm3d_solve_fixed(fc, k, offs.data(), fnode.data(), ftime.data(), order, FMM3D_PREC_FP64, FMM3D_WALK_ALTERNATE);
+    // 1.8 ms (61^3, refinement 5 over 10 nodes).  With point_warm the
+    // float64 sweeps start from a float32 solve's field.
+    if (c->point_warm) {
+        rc = fmm3d_solve_fixed(fc, k, offs.data(), fnode.data(), ftime.data(), order, FMM3D_PREC_FP32, FMM3D_WALK_ALTERNATE);
+        if (rc) return fail_to(err, rc, "refined float32 solve: %s", fc->err.c_str());
+        const size_t n = (size_t)k * fc->N;
+        if (fc->d_warm_n < n) {
+            (void)hipFree(fc->d_warm); (void)hipFree(fc->d_warm_off);
+            fc->d_warm = nullptr; fc->d_warm_off = nullptr; fc->d_warm_n = 0;
+            if (dalloc(&fc->d_warm, n * sizeof(float)) != hipSuccess || dalloc(&fc->d_warm_off, (size_t)k * sizeof(double) + 256) != hipSuccess)
+                return fail_to(err, FMM3D_E_NOMEM, "refined warm start: no device memory");
+            fc->d_warm_n = n;
+        }
+        hipError_t e = hipMemcpyAsync(fc->d_warm, fc->sl.T, n * sizeof(float), hipMemcpyDeviceToDevice, fc->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(fc->d_warm_off, fc->sl.toff, (size_t)k * sizeof(double), hipMemcpyDeviceToDevice, fc->stream);
+        if (e != hipSuccess) return fail_to(err, FMM3D_E_DEVICE, "refined warm start: %s", hipGetErrorString(e));
+        // the float64 fields: fixed sets as in a cold solve, free nodes from the float32 field
+        std::vector<double> toff;
+        std::vector<int> flow;
+        if ((rc = fixed_prepare(fc, "refined solve", k, offs.data(), fnode.data(), ftime.data(), toff, flow, fc->point_order != 0)) ||
+            (rc = ensure_slots(fc, k, FMM3D_PREC_FP64)))
+            return fail_to(err, rc, "refined solve: %s", fc->err.c_str());
+        fc->sl.n = 0;
+        fc->walk_alt = 1;
+        if ((rc = fixed_upload(fc, "refined solve", 0, k, offs.data(), fnode.data(), ftime.data(), FMM3D_PREC_FP64, toff, flow)))
+            return fail_to(err, rc, "refined solve: %s", fc->err.c_str());
+        k_warm_start<<<(int)std::min<size_t>((n + 255) / 256, 16384), 256, 0, fc->stream>>>(
+            (double*)fc->sl.T, fc->d_warm, fc->d_warm_off, fc->sl.fix, k, fc->N);
+        if ((e = hipGetLastError()) != hipSuccess) return fail_to(err, FMM3D_E_DEVICE, "refined warm start: %s", hipGetErrorString(e));
+        rc = solve_run(fc, k, order, FMM3D_PREC_FP64);
+    } else {
+        rc = fmm3d_solve_fixed(fc, k, offs.data(), fnode.data(), ftime.data(), order, FMM3D_PREC_FP64, FMM3D_WALK_ALTERNATE);
+    }
     if (rc) return fail_to(err, rc, "refined solve: %s", fc->err.c_str());
     // the coarse nodes each block covers seed the coarse solve: gathered on
     // the device, so only they cross to the host
@@ -4334,6 +4391,9 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "point_workers")) {
         if (iv < 1 || iv > 16) return fail(c, FMM3D_E_ARG, "point_workers must be in 1..16");
         c->point_workers = iv;
+    } else if (!strcmp(key, "point_warm")) {
+        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "point_warm must be 0 or 1");
+        c->point_warm = iv;
     } else if (!strcmp(key, "point_window")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "point_window must be 0 or 1");
         c->point_window = iv;

@@ -98,21 +98,23 @@ def test_point_workers_and_context_pool(cfg0):
 
 def test_point_window_and_group_size_keep_the_field(cfg0):
     """Layer windows on or off, refined blocks in one group or one per
-    group: the same fixed point, and the oracle's."""
+    group, float64 refined sweeps from a float32 solve or from the start:
+    the same fixed point, and the oracle's."""
     g, velv, veln, srcs, rcv, ref = cfg0
     runs = {}
     with lib.Context(g) as c:
         c.set_model(velv)
-        for pw, mn in ((1, 128), (0, 128), (1, 1)):
+        for pw, mn, warm in ((1, 128, 1), (0, 128, 1), (1, 1, 1), (1, 128, 0)):
             c.set_option("point_window", pw)
             c.set_option("point_group_mnodes", mn)
+            c.set_option("point_warm", warm)
             c.solve_point(srcs, 5, 10, lib.ORDER_MIXED, lib.PREC_FP64)
-            runs[(pw, mn)] = [c.field(s) for s in range(len(srcs))]
-        for key, bad in (("point_window", 2), ("point_group_mnodes", 0)):
+            runs[(pw, mn, warm)] = [c.field(s) for s in range(len(srcs))]
+        for key, bad in (("point_window", 2), ("point_group_mnodes", 0), ("point_warm", -1)):
             with pytest.raises(lib.FmmError) as e:
                 c.set_option(key, bad)
             assert e.value.code == lib.E_ARG
-    base = runs[(1, 128)]
+    base = runs[(1, 128, 1)]
     for key, run in runs.items():
         for s in range(len(srcs)):
             assert float(np.abs(run[s] - base[s]).max()) <= 1e-9, (key, s)
