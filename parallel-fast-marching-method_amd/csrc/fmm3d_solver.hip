@@ -2618,7 +2618,10 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     a.pend = s.pend;
     a.slotpend = s.slotpend;
     a.born = nullptr;
-    a.lrng = c->point_window ? s.lrng : nullptr;
+    // float64 only: float32 walks count a change only above their settle
+    // tolerance (about 1e-6 relative), and windows built on those changes
+    // leave sub-tolerance drift behind (up to 2.4 ms at receivers, 257^3)
+    a.lrng = (c->point_window && std::is_same<R, double>::value) ? s.lrng : nullptr;
     a.geo_n = a.geo_hx = a.geo_hz = 0;
     a.sdims = nullptr;
     if (c->geo_cap > 0) {

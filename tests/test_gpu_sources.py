@@ -130,6 +130,25 @@ def test_point_sources_fp32_within_one_millisecond(cfg0):
             assert float(np.abs(c.field(s) - ref[s]).max()) <= 1e-3, s
 
 
+def test_point_sources_fp32_receivers_201():
+    """32 local sources (5-80 km deep, refinement 5 over 5 nodes) on a 201^3
+    grid (0-600 km, 15x15 deg, 25^3 vertices, +-5 %): float32 receiver times
+    within 1 ms of float64."""
+    g = synth.make_grid((201, 201, 201), 600.0, (15.0, 15.0), nv=(25, 25, 25))
+    velv = synth.make_velv(g, 0.05, 5)
+    src = np.stack([synth.point_source_rad(g, s) for s in synth.local_sources(g, 32, 5)])
+    rcv = synth.receivers_rad(synth.receivers(g, 100, 5))
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        c.solve_point(src, 5, 5, lib.ORDER_MIXED, lib.PREC_FP64)
+        t64 = c.receivers(rcv).astype(np.float64)
+        c.solve_point(src, 5, 5, lib.ORDER_MIXED, lib.PREC_FP32)
+        t32 = c.receivers(rcv).astype(np.float64)
+    d = float(np.nanmax(np.abs(t32 - t64)))
+    print(f"float32 vs float64 point-source receivers at 201^3: max |dt| {d:.3e} s")
+    assert np.isfinite(t64).all() and d <= 1e-3
+
+
 def test_side_face_injection_matches_relaxation():
     """Bottom face plus incoming side faces with 1-D tau-p times: the GPU field
     equals the oracle's relaxation of the same fixed set (float64), and the
