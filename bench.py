@@ -365,8 +365,8 @@ def serving_mix(a):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (sources shard across GPUs "
-              f"with no exchange, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported: this build runs on one GPU "
+              f"(sources would shard across GPUs with no exchange between them)", file=sys.stderr)
         sys.exit(2)
     if a.workload == "mixed256":
         print(json.dumps(mixed256(a)), flush=True)
