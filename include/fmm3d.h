@@ -221,7 +221,8 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *   "point_group_mnodes" refined blocks solved together as one group: at
  *                   most this many million nodes of the group's grid
  *                   (1..1024, default 128)
- *   "point_window"  alternating float64 walks: 1 (default) a patch walks only the
+ *   "point_window"  alternating float64 walks, and the float32 warm-up of
+ *                   refined blocks: 1 (default) a patch walks only the
  *                   layers within two of one that changed, in it or a
  *                   lateral neighbour, in the sweep before or so far in
  *                   this one, and on while its own walk changes layers;

@@ -2618,10 +2618,12 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     a.pend = s.pend;
     a.slotpend = s.slotpend;
     a.born = nullptr;
-    // float64 only: float32 walks count a change only above their settle
-    // tolerance (about 1e-6 relative), and windows built on those changes
-    // leave sub-tolerance drift behind (up to 2.4 ms at receivers, 257^3)
-    a.lrng = (c->point_window && std::is_same<R, double>::value) ? s.lrng : nullptr;
+    // float64, and the float32 warm-up solves of refined blocks (float64
+    // sweeps follow them): float32 walks count a change only above their
+    // settle tolerance (about 1e-6 relative), and windows built on those
+    // changes leave sub-tolerance drift behind (up to 2.4 ms at receivers
+    // of a 257^3 coarse solve)
+    a.lrng = (c->point_window && (std::is_same<R, double>::value || c->geo_cap > 0)) ? s.lrng : nullptr;
     a.geo_n = a.geo_hx = a.geo_hz = 0;
     a.sdims = nullptr;
     if (c->geo_cap > 0) {
