@@ -49,6 +49,7 @@ extern "C" {
 #define FMM3D_PREC_FP64     0   /* float64 node updates                     */
 #define FMM3D_PREC_FP32     1   /* float32 updates on times offset by the source's boundary minimum */
 #define FMM3D_WALK_ALTERNATE 1  /* flag: column walk alternates bottom-up / top-down (point sources) */
+#define FMM3D_REFERENCE_PARTITION 2  /* fmm3d_solve_tele flag: the reference's 2x2x2 partitioned solve */
 
 typedef struct {
     int nvr, nvx, nvz;          /* vertex counts, border excluded          */
@@ -96,7 +97,14 @@ int fmm3d_set_model_vtx(fmm3d_ctx *ctx, const double *velv);
  * faces: nsrc records of nnx*nnz float64, phi fastest (itimes.out).
  * order: FMM3D_ORDER_*; precision: FMM3D_PREC_*.  The nsrc fields stay
  * resident in the context (replacing those of the previous solve) for the
- * receiver and Frechet calls.  flags: 0. */
+ * receiver and Frechet calls.
+ * flags: 0 solves the upwind system on the whole grid as one domain (the
+ * field a single-rank heap march gives); FMM3D_REFERENCE_PARTITION
+ * (float64 only) reproduces the reference program's own result: its 2x2x2
+ * blocks with stencils ending at block faces, face relaxation from two-deep
+ * ghosts, the per-block march from the reclassified state, and passes until
+ * the largest change is at most the float32 literal 1e-5
+ * (fm3dt_parallel.f90:157-416, 708-710, 1100-2825, 6127-6171, 6397-6435). */
 int fmm3d_solve_tele(fmm3d_ctx *ctx, int nsrc, const double *faces, int order,
                      int precision, int flags);
 
@@ -181,6 +189,11 @@ int fmm3d_frechet_pinned(fmm3d_ctx *ctx, int nrc, const double *rcv, const int64
 /* Copy resident field `src` to the host as float64 (nnr, nnx, nnz), phi
  * fastest (the reference's ttn after the final exchange, F:6439-6448). */
 int fmm3d_get_field(fmm3d_ctx *ctx, int src, double *ttn);
+
+/* Outer passes per source of the last FMM3D_REFERENCE_PARTITION solve of
+ * nsrc sources (the reference's `count`, F:6398-6430); FMM3D_E_STATE after
+ * any other solve. */
+int fmm3d_get_passes(const fmm3d_ctx *ctx, int nsrc, int *passes);
 
 /* Statistics of the last solve. */
 int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);

@@ -471,13 +471,12 @@ static size_t face_index(const dom_t *D, int k, int io, int ii)
  * the ghost side always counts, second order iff near ghost > far ghost;
  * only octants on the ghost side of the face axis.  Acceptance as F:1378
  * (F:2817 for the down face). */
-static void face_update(dom_t *D, int k, int ir, int ix, int iz)
+static double face_value(const dom_t *D, int k, int ir, int ix, int iz)
 {
     const or_grid *g = D->g;
     const int c[3] = { ir, ix, iz };
     const int A = FACE_AXIS[k];
-    double old = D->t[DIDX(D, ir, ix, iz)];
-    double tc = old;
+    double tc = D->t[DIDX(D, ir, ix, iz)];
     double ri = D->ri[ir];
     double h[3] = { g->dnr, ri * g->dnx, ri * D->sth[ix] * g->dnz };
     int ord[3][2];
@@ -521,17 +520,57 @@ static void face_update(dom_t *D, int k, int ir, int ix, int iz)
                 double T = or_octant_solve(o, a1, a2, h, s);
                 if (T < best) best = T;
             }
+    return best;
+}
+
+/* acceptance as F:1378 (F:2817 for the down face): the new value goes into
+ * the block's heap */
+static void face_accept(dom_t *D, int k, int ir, int ix, int iz, double best)
+{
+    size_t node = DIDX(D, ir, ix, iz);
+    double old = D->t[node];
     int accept = (k == 5) ? (old > best || old == 0.0) : (old >= best || old == 0.0);
     if (accept) {
-        size_t node = DIDX(D, ir, ix, iz);
         D->t[node] = best;
         if (D->st[node] <= 0) heap_push(D, (int)node);
         else heap_up(D, D->st[node]);
     }
 }
 
-/* face relaxation + reclassification of one block (F:157-416) */
-static void order_overlap(dom_t *D)
+static void face_update(dom_t *D, int k, int ir, int ix, int iz)
+{
+    face_accept(D, k, ir, ix, iz, face_value(D, k, ir, ix, iz));
+}
+
+/* Reclassification after the face relaxation (F:387-415): if any face node
+ * was accepted (the heap is not empty) the block marches this pass; every
+ * other known node at or below the heap minimum is alive, at or above the
+ * heap maximum far (its value kept), otherwise back in the band. */
+static void reclassify(dom_t *D)
+{
+    if (D->nh >= 1) {
+        D->sw_fmm = 1;
+        double mini = D->t[D->heap[1]], maxi = 0.0;
+        for (int i = 1; i <= D->nh; ++i) if (D->t[D->heap[i]] > maxi) maxi = D->t[D->heap[i]];
+        for (int ir = 1; ir <= D->n[0]; ++ir)
+            for (int ix = 1; ix <= D->n[1]; ++ix)
+                for (int iz = 1; iz <= D->n[2]; ++iz) {
+                    size_t q = DIDX(D, ir, ix, iz);
+                    if (D->st[q] > 0) continue;
+                    if (D->t[q] > 0.0) {
+                        if (D->t[q] <= mini) D->st[q] = 0;
+                        else if (D->t[q] >= maxi) D->st[q] = -1;
+                        else heap_push(D, (int)q);
+                    } else {
+                        D->st[q] = -1;
+                    }
+                }
+    }
+}
+
+/* face relaxation of one block in the reference's order: per face, a heap of
+ * the face nodes keyed by the near ghost, popped in key order (F:157-386) */
+static void face_relax_heap(dom_t *D)
 {
     for (int k = 0; k < 6; ++k) {
         if (!D->srecv[k]) continue;
@@ -578,24 +617,190 @@ static void order_overlap(dom_t *D)
         }
         free(fh);
     }
-    if (D->nh >= 1) {
-        D->sw_fmm = 1;
-        double mini = D->t[D->heap[1]], maxi = 0.0;
-        for (int i = 1; i <= D->nh; ++i) if (D->t[D->heap[i]] > maxi) maxi = D->t[D->heap[i]];
-        for (int ir = 1; ir <= D->n[0]; ++ir)
+}
+
+/* ------------------------- partitioned mode as block relaxation (the GPU's) */
+
+/* The face relaxation of face_relax_heap as dataflow rounds, the form the GPU
+ * runs.  A face node in the heap reads, through face_update, its in-plane
+ * first and second neighbours; in the heap order it sees the new value of
+ * every such neighbour with a smaller key and the old value of the others.
+ * So a node is ready once its heap neighbours of smaller key are done, and
+ * each round updates every ready node.  Ties in the key are taken in face
+ * index order (the reference's heap breaks them by its sift order). */
+static void face_relax_rounds(dom_t *D)
+{
+    static const int NB[8][2] = { {-1, 0}, {1, 0}, {-2, 0}, {2, 0}, {0, -1}, {0, 1}, {0, -2}, {0, 2} };
+    for (int k = 0; k < 6; ++k) {
+        if (!D->srecv[k]) continue;
+        int outer, inner;
+        face_axes(k, &outer, &inner);
+        const int A = FACE_AXIS[k];
+        const int fixed = FACE_SIDE[k] < 0 ? 1 : D->n[A];
+        const int no = D->n[outer], ni = D->n[inner], nf = no * ni;
+        const double *key = D->gh[k][0];
+        unsigned char *inh = calloc(nf, 1), *done = calloc(nf, 1), *now = calloc(nf, 1);
+        double *val = malloc(nf * sizeof(double));
+        for (int fi = 0; fi < nf; ++fi) {
+            int p[3];
+            p[A] = fixed; p[outer] = fi / ni + 1; p[inner] = fi % ni + 1;
+            double tf = D->t[DIDX(D, p[0], p[1], p[2])];
+            inh[fi] = key[fi] < tf || tf == 0.0;
+        }
+        for (;;) {
+            int any = 0;
+            for (int fi = 0; fi < nf; ++fi) {
+                now[fi] = 0;
+                if (!inh[fi] || done[fi]) continue;
+                const int io = fi / ni, ii = fi % ni;
+                int ready = 1;
+                for (int q = 0; q < 8 && ready; ++q) {
+                    const int jo = io + NB[q][0], ji = ii + NB[q][1];
+                    if (jo < 0 || jo >= no || ji < 0 || ji >= ni) continue;
+                    const int fj = jo * ni + ji;
+                    if (inh[fj] && !done[fj] && (key[fj] < key[fi] || (key[fj] == key[fi] && fj < fi))) ready = 0;
+                }
+                now[fi] = ready;
+                any |= ready;
+            }
+            if (!any) break;
+            /* all of a round's values from the field before it (a ready node
+             * reads its larger-key heap neighbours' old values), then applied */
+            for (int fi = 0; fi < nf; ++fi) {
+                if (!now[fi]) continue;
+                int p[3];
+                p[A] = fixed; p[outer] = fi / ni + 1; p[inner] = fi % ni + 1;
+                val[fi] = face_value(D, k, p[0], p[1], p[2]);
+            }
+            for (int fi = 0; fi < nf; ++fi) {
+                if (!now[fi]) continue;
+                int p[3];
+                p[A] = fixed; p[outer] = fi / ni + 1; p[inner] = fi % ni + 1;
+                face_accept(D, k, p[0], p[1], p[2], val[fi]);
+                done[fi] = 1;
+            }
+        }
+        free(inh); free(done); free(now); free(val);
+    }
+}
+
+/* The heap march of one block (local_fmm, F:78-155) as a relaxation: the
+ * pop-order update.  The march freezes every node alive at its start (st 0)
+ * and gives every other node the minimum of its starting value and the
+ * octant candidates computed each time a neighbour pops, from the
+ * neighbours alive at that time.  A neighbour pops before the node iff its
+ * value is below the node's value so far, so the update walks the known
+ * neighbours in increasing value, and a neighbour joins the stencil while
+ * it is below the running minimum; after each join the octants through it
+ * give a candidate.  Second order on an axis iff the second neighbour is
+ * known and below the first (F:645-707); neighbours outside the block are
+ * out of bounds (F:708-710).  Unlike the value-based form (a neighbour
+ * counts iff below the node's own value), the result depends on the
+ * neighbours only, so the relaxation cannot cycle; it settles where the
+ * march does.  T: current values (+inf unknown). */
+static double pop_update(const dom_t *D, const double *T, int ir, int ix, int iz, double t0)
+{
+    const or_grid *g = D->g;
+    const int c[3] = { ir, ix, iz };
+    const double ri = D->ri[ir];
+    const double h[3] = { g->dnr, ri * g->dnx, ri * D->sth[ix] * g->dnz };
+    const double s = D->slo[DIDX(D, ir, ix, iz)];
+    double v1[6] = { 0 }, v2[6] = { 0 };
+    int side[6], sec[6], ord6[6], m = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int sd = 0; sd < 2; ++sd) {
+            const int j = 2 * a + sd, d = sd ? 1 : -1;
+            int p1[3] = { c[0], c[1], c[2] }, p2[3] = { c[0], c[1], c[2] };
+            p1[a] += d; p2[a] += 2 * d;
+            side[j] = -1; sec[j] = 0;                       /* -1 out of bounds */
+            if (p1[a] < 1 || p1[a] > D->n[a]) continue;
+            side[j] = 0;                                    /* 0 not (yet) in the stencil */
+            v1[j] = T[DIDX(D, p1[0], p1[1], p1[2])];
+            if (!(v1[j] < INFINITY)) continue;
+            if (D->fom && p2[a] >= 1 && p2[a] <= D->n[a]) {
+                v2[j] = T[DIDX(D, p2[0], p2[1], p2[2])];
+                sec[j] = v2[j] < INFINITY && v1[j] > v2[j];
+            }
+            ord6[m++] = j;
+        }
+    for (int i = 1; i < m; ++i) {                           /* ascending value, ties by side */
+        int j = ord6[i], q = i;
+        while (q > 0 && v1[ord6[q - 1]] > v1[j]) { ord6[q] = ord6[q - 1]; --q; }
+        ord6[q] = j;
+    }
+    double cur = t0;
+    for (int i = 0; i < m; ++i) {
+        const int j = ord6[i];
+        if (!(v1[j] < cur)) break;
+        side[j] = sec[j] ? 2 : 1;
+        const int aj = j / 2;
+        for (int u = 0; u < 2; ++u)
+            for (int w = 0; w < 2; ++w) {
+                int pick[3];
+                pick[aj] = j;
+                pick[(aj + 1) % 3] = 2 * ((aj + 1) % 3) + u;
+                pick[(aj + 2) % 3] = 2 * ((aj + 2) % 3) + w;
+                int o[3];
+                double a1[3], a2[3];
+                int bad = 0;
+                for (int a = 0; a < 3; ++a) {
+                    const int q = pick[a];
+                    if (side[q] < 0) { bad = 1; break; }
+                    o[a] = side[q]; a1[a] = v1[q]; a2[a] = v2[q];
+                }
+                if (bad) continue;
+                const double Tv = or_octant_solve(o, a1, a2, h, s);
+                if (Tv < cur) cur = Tv;
+            }
+    }
+    return cur;
+}
+
+/* Relax one block from the state the march would start from (alive nodes
+ * fixed, every other node bounded by its starting value, unknown = +inf),
+ * Gauss-Seidel in column order with the walk direction alternating, until a
+ * sweep changes nothing; then every reached node is alive, as after the
+ * march.  Returns the sweeps made. */
+static long block_relax(dom_t *D)
+{
+    const size_t N = (size_t)(D->n[0] + 2) * D->NX * D->NZ;
+    double *T = malloc(N * sizeof(double)), *T0 = malloc(N * sizeof(double));
+    unsigned char *fx = calloc(N, 1);
+    for (size_t q = 0; q < N; ++q) { T[q] = INFINITY; T0[q] = INFINITY; }
+    for (int ir = 1; ir <= D->n[0]; ++ir)
+        for (int ix = 1; ix <= D->n[1]; ++ix)
+            for (int iz = 1; iz <= D->n[2]; ++iz) {
+                const size_t q = DIDX(D, ir, ix, iz);
+                T0[q] = D->t[q] > 0.0 ? D->t[q] : INFINITY;
+                T[q] = T0[q];
+                fx[q] = D->st[q] == 0;
+            }
+    long it;
+    for (it = 1; it <= 100000; ++it) {
+        int moved = 0;
+        const int down = !(it & 1);
+        for (int qq = 0; qq < D->n[0]; ++qq) {
+            const int ir = down ? 1 + qq : D->n[0] - qq;
             for (int ix = 1; ix <= D->n[1]; ++ix)
                 for (int iz = 1; iz <= D->n[2]; ++iz) {
-                    size_t q = DIDX(D, ir, ix, iz);
-                    if (D->st[q] > 0) continue;
-                    if (D->t[q] > 0.0) {
-                        if (D->t[q] <= mini) D->st[q] = 0;
-                        else if (D->t[q] >= maxi) D->st[q] = -1;
-                        else heap_push(D, (int)q);
-                    } else {
-                        D->st[q] = -1;
-                    }
+                    const size_t k = DIDX(D, ir, ix, iz);
+                    if (fx[k]) continue;
+                    const double nv = pop_update(D, T, ir, ix, iz, T0[k]);
+                    if (nv != T[k]) { T[k] = nv; moved = 1; }
                 }
+        }
+        if (!moved) break;
     }
+    for (int ir = 1; ir <= D->n[0]; ++ir)
+        for (int ix = 1; ix <= D->n[1]; ++ix)
+            for (int iz = 1; iz <= D->n[2]; ++iz) {
+                const size_t q = DIDX(D, ir, ix, iz);
+                D->t[q] = T[q] < INFINITY ? T[q] : 0.0;
+                D->st[q] = T[q] < INFINITY ? 0 : -1;
+            }
+    D->nh = 0;
+    free(T); free(T0); free(fx);
+    return it;
 }
 
 /* copy the two planes of block B next to face k of block A into A's ghosts */
@@ -626,8 +831,8 @@ static void exchange_face(dom_t *A, const dom_t *B, int k)
  * process in lockstep; the MPI exchanges are deterministic, so this is the
  * reference's result.  velv: vertex array (node velocities are evaluated
  * per block with the block's offset arithmetic).  Returns passes made. */
-int or_fmm_tele_part(const or_grid *g, const double *velv, const double *face, int fom,
-                     double *ttn, int max_pass)
+static int part_solve(const or_grid *g, const double *velv, const double *face, int fom,
+                      double *ttn, int max_pass, int relax)
 {
     const int P = 2;
     int nn[3] = { g->nnr, g->nnx, g->nnz };
@@ -684,7 +889,11 @@ int or_fmm_tele_part(const or_grid *g, const double *velv, const double *face, i
         int sw = 0;
         for (int o = 0; o < 8; ++o)
             if (Dm[o].coord[0] == P - 1 && have[o] && tmin[b] > tmin[o]) sw = 1;
-        if (!sw) { heap_push(&Dm[b], mn[b]); march(&Dm[b]); Dm[b].sw_fmm = 1; }
+        if (!sw) {
+            heap_push(&Dm[b], mn[b]);
+            if (relax) block_relax(&Dm[b]); else march(&Dm[b]);
+            Dm[b].sw_fmm = 1;
+        }
     }
     const double tol = (double)1.0e-5f;
     double err = 0.0;
@@ -713,7 +922,11 @@ int or_fmm_tele_part(const or_grid *g, const double *velv, const double *face, i
         for (int b = 0; b < 8; ++b)
             for (int k = 0; k < 6; ++k)
                 if (nbr[b][k] >= 0 && Dm[nbr[b][k]].dsend) exchange_face(&Dm[b], &Dm[nbr[b][k]], k);
-        for (int b = 0; b < 8; ++b) { order_overlap(&Dm[b]); march(&Dm[b]); }
+        for (int b = 0; b < 8; ++b) {
+            if (relax) face_relax_rounds(&Dm[b]); else face_relax_heap(&Dm[b]);
+            reclassify(&Dm[b]);
+            if (Dm[b].nh > 0) { if (relax) block_relax(&Dm[b]); else march(&Dm[b]); }
+        }
         err = 0.0;
         for (int b = 0; b < 8; ++b) {
             dom_t *D = &Dm[b];
@@ -729,6 +942,22 @@ int or_fmm_tele_part(const or_grid *g, const double *velv, const double *face, i
     }
     for (int b = 0; b < 8; ++b) { dom_extract(&Dm[b], ttn); dom_free(&Dm[b]); }
     return pass;
+}
+
+int or_fmm_tele_part(const or_grid *g, const double *velv, const double *face, int fom,
+                     double *ttn, int max_pass)
+{
+    return part_solve(g, velv, face, fom, ttn, max_pass, 0);
+}
+
+/* The same partitioned solve with the two sequential parts in the parallel
+ * form the GPU runs: each block's heap march as the pop-order relaxation
+ * (block_relax) and each face heap as dataflow rounds (face_relax_rounds).
+ * Returns the passes made. */
+int or_relax_tele_part(const or_grid *g, const double *velv, const double *face, int fom,
+                       double *ttn, int max_pass)
+{
+    return part_solve(g, velv, face, fom, ttn, max_pass, 1);
 }
 
 static void metric_tables(const or_grid *g, double *ri, double *sth)

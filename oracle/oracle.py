@@ -55,6 +55,8 @@ def lib():
         L.or_fmm_tele.restype = C.c_long
         L.or_fmm_tele_part.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, dp, C.c_int]
         L.or_fmm_tele_part.restype = C.c_int
+        L.or_relax_tele_part.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, dp, C.c_int]
+        L.or_relax_tele_part.restype = C.c_int
         L.or_relax_tele.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, dp]
         L.or_relax_tele.restype = C.c_long
         L.or_relax1_tele.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, dp]
@@ -103,6 +105,15 @@ def fmm_tele_part(g, velv: np.ndarray, face: np.ndarray, fom: int = 1, max_pass:
     ttn = np.zeros((g.nnr, g.nnx, g.nnz))
     n = lib().or_fmm_tele_part(C.byref(or_grid(g)), np.ascontiguousarray(velv, dtype=np.float64),
                                np.ascontiguousarray(face, dtype=np.float64), fom, ttn, max_pass)
+    return ttn, n
+
+
+def relax_tele_part(g, velv: np.ndarray, face: np.ndarray, fom: int = 1, max_pass: int = 1000):
+    """The partitioned solve in the GPU's parallel form (pop-order block
+    relaxation, face updates in dataflow rounds); returns (ttn, passes)."""
+    ttn = np.zeros((g.nnr, g.nnx, g.nnz))
+    n = lib().or_relax_tele_part(C.byref(or_grid(g)), np.ascontiguousarray(velv, dtype=np.float64),
+                                 np.ascontiguousarray(face, dtype=np.float64), fom, ttn, max_pass)
     return ttn, n
 
 

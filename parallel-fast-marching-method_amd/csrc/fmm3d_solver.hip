@@ -1689,6 +1689,10 @@ struct Slots {
 
 }  // namespace
 
+struct PartState;                  // the reference-partition mode's device state (fmm3d_partition.inl)
+static void part_free(PartState* ps);
+static void part_model_changed(PartState* ps);
+
 struct fmm3d_ctx {
     Grid g;
     int device = 0;
@@ -1801,6 +1805,8 @@ struct fmm3d_ctx {
     int64_t* d_updates = nullptr;
     unsigned long long* d_visits = nullptr;
     float* ray_scratch = nullptr;
+    PartState* part = nullptr;     // FMM3D_REFERENCE_PARTITION solves (allocated at first use)
+    bool part_last = false;        // the resident fields come from a partitioned solve
     size_t ray_scratch_workers = 0;
     // Frechet row staging (per ray `fr_rcap` entries), counts, offsets and the packed rows
     int32_t* fr_cols = nullptr;
@@ -2047,6 +2053,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv); (void)hipFree(c->eng_faces);
     (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out); (void)hipFree(c->eng_islot);
     free_slots(c->sl);
+    part_free(c->part);
     (void)hipFree(c->veln); (void)hipFree(c->slo64); (void)hipFree(c->slo32);
     (void)hipFree(c->gslo64); (void)hipFree(c->gslo32);
     (void)hipFree(c->ghx64); (void)hipFree(c->ghz64); (void)hipFree(c->ghx32); (void)hipFree(c->ghz32);
@@ -2094,6 +2101,7 @@ extern "C" int fmm3d_set_model_vtx(fmm3d_ctx* c, const double* velv)
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "gridder: %s", hipGetErrorString(e));
     c->have_model = true;
+    part_model_changed(c->part);
     return FMM3D_OK;
 }
 
@@ -2119,6 +2127,7 @@ static Bricks bricks_of(const fmm3d_ctx* c)
 static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
 {
     Slots& s = c->sl;
+    c->part_last = false;   // every solve starts here
     const size_t es = prec == FMM3D_PREC_FP32 ? sizeof(float) : sizeof(double);
     if (s.cap >= nsrc && (s.prec == prec || s.es >= es)) {
         s.prec = prec;
@@ -3041,6 +3050,45 @@ static int tele_upload(fmm3d_ctx* c, const char* fn, int nsrc, const double* fac
     return FMM3D_OK;
 }
 
+#include "fmm3d_partition.inl"
+
+// The reference-partition teleseismic solve (bottom faces, float64).
+static int solve_tele_part(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, int order)
+{
+    const Grid& g = c->g;
+    const size_t F = (size_t)g.nnx * g.nnz;
+    for (size_t i = 0; i < (size_t)nsrc * F; ++i)
+        if (isnan(faces[i])) return fail(c, FMM3D_E_ARG, "%s: NaN boundary time (source %zu)", fn, i / F);
+    int rc = ensure_slots(c, nsrc, FMM3D_PREC_FP64);
+    if (rc) return rc;
+    c->sl.n = 0;
+    if (c->d_faces_n < (size_t)nsrc * F) {
+        (void)hipFree(c->d_faces);
+        c->d_faces = nullptr;
+        c->d_faces_n = 0;
+        HIPCHK(c, dalloc(&c->d_faces, (size_t)nsrc * F * sizeof(double)));
+        c->d_faces_n = (size_t)nsrc * F;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_faces, faces, (size_t)nsrc * F * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->stats = fmm3d_stats{};
+    HIPCHK(c, hipMemsetAsync(c->d_visits, 0, 3 * sizeof(unsigned long long), c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    rc = part_run(c, nsrc, faces, order);
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    c->stats.solve_ms = ms;
+    unsigned long long vis[3] = { 0, 0, 0 };
+    HIPCHK(c, hipMemcpy(vis, c->d_visits, sizeof vis, hipMemcpyDeviceToHost));
+    c->stats.node_updates = (int64_t)vis[1];
+    c->stats.tile_r = 1; c->stats.tile_x = PT_P; c->stats.tile_z = PT_P;
+    c->sl.n = nsrc;
+    c->part_last = true;
+    return FMM3D_OK;
+}
+
 // Teleseismic solves from bottom faces and, when xf is given, one theta
 // and one phi side face per source.
 static int solve_tele_impl(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, const double* xf,
@@ -3049,11 +3097,19 @@ static int solve_tele_impl(fmm3d_ctx* c, const char* fn, int nsrc, const double*
     if (!c) return fail(c, FMM3D_E_ARG, "%s: null context", fn);
     if (order != FMM3D_ORDER_FIRST && order != FMM3D_ORDER_MIXED) return fail(c, FMM3D_E_ARG, "%s: bad order %d", fn, order);
     if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "%s: bad precision %d", fn, precision);
-    if (flags != 0) return fail(c, FMM3D_E_ARG, "%s: flags must be 0", fn);
+    if (flags != 0 && flags != FMM3D_REFERENCE_PARTITION)
+        return fail(c, FMM3D_E_ARG, "%s: flags must be 0 or FMM3D_REFERENCE_PARTITION", fn);
     if (!c->have_model) return fail(c, FMM3D_E_STATE, "%s: no velocity model (call fmm3d_set_model_vtx)", fn);
     int rc = tele_check(c, fn, nsrc, faces, xf, zf, side);
     if (rc) return rc;
     HIPCHK(c, hipSetDevice(c->device));
+    if (flags == FMM3D_REFERENCE_PARTITION) {
+        if (precision != FMM3D_PREC_FP64)
+            return fail(c, FMM3D_E_ARG, "%s: FMM3D_REFERENCE_PARTITION solves in float64 only", fn);
+        if (xf) return fail(c, FMM3D_E_ARG, "%s: FMM3D_REFERENCE_PARTITION takes bottom faces only", fn);
+        c->walk_alt = 0;
+        return solve_tele_part(c, fn, nsrc, faces, order);
+    }
     rc = ensure_slots(c, nsrc, precision);
     if (rc) return rc;
     c->sl.n = 0;
@@ -4416,6 +4472,16 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else {
         return fail(c, FMM3D_E_ARG, "fmm3d_set_option: unknown key '%s'", key);
     }
+    return FMM3D_OK;
+}
+
+extern "C" int fmm3d_get_passes(const fmm3d_ctx* c, int nsrc, int* passes)
+{
+    if (!c || !passes) return fail(const_cast<fmm3d_ctx*>(c), FMM3D_E_ARG, "fmm3d_get_passes: null argument");
+    if (!c->part_last || !c->part || (int)c->part->passes.size() != nsrc || c->sl.n != nsrc)
+        return fail(const_cast<fmm3d_ctx*>(c), FMM3D_E_STATE,
+                    "fmm3d_get_passes: the last solve was not a reference-partition solve of %d sources", nsrc);
+    for (int s = 0; s < nsrc; ++s) passes[s] = c->part->passes[s];
     return FMM3D_OK;
 }
 

@@ -22,12 +22,13 @@ SRC_TELE, SRC_POINT = 0, 1
 ORDER_FIRST, ORDER_MIXED = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 WALK_ALTERNATE = 1
+REFERENCE_PARTITION = 2
 
 EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model_vtx",
            "fmm3d_solve_fixed", "fmm3d_solve_point", "fmm3d_solve_tele_sides", "fmm3d_solve_mixed",
            "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_result",
            "fmm3d_server_stats", "fmm3d_server_destroy",
-           "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_frechet", "fmm3d_frechet_pinned", "fmm3d_get_field",
+           "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_frechet", "fmm3d_frechet_pinned", "fmm3d_get_field", "fmm3d_get_passes",
            "fmm3d_get_stats", "fmm3d_set_option")
 
 
@@ -109,6 +110,7 @@ def load(path: Optional[str] = None):
                                        C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_float)),
                                        C.POINTER(C.c_int64)]
     L.fmm3d_get_field.argtypes = [vp, C.c_int, dp]
+    L.fmm3d_get_passes.argtypes = [vp, C.c_int, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
     L.fmm3d_get_stats.argtypes = [vp, C.POINTER(StatsC)]
     L.fmm3d_set_option.argtypes = [vp, C.c_char_p, C.c_double]
     for f in EXPORTS:
@@ -162,14 +164,24 @@ class Context:
     def set_model(self, velv: np.ndarray) -> None:
         self._check(self.L.fmm3d_set_model_vtx(self._h, np.ascontiguousarray(velv, dtype=np.float64)))
 
-    def solve_tele(self, faces: np.ndarray, order: int = ORDER_MIXED, precision: int = PREC_FP64) -> None:
+    def solve_tele(self, faces: np.ndarray, order: int = ORDER_MIXED, precision: int = PREC_FP64,
+                   partition: bool = False) -> None:
+        """Bottom-face teleseismic solves; partition=True reproduces the
+        reference's own 2x2x2 partitioned result (float64 only)."""
         faces = np.ascontiguousarray(faces, dtype=np.float64)
         if faces.ndim == 2:
             faces = faces[None]
         if faces.shape[1:] != (self.g.nnx, self.g.nnz):
             raise ValueError(f"faces must be (nsrc, {self.g.nnx}, {self.g.nnz}), got {faces.shape}")
-        self._check(self.L.fmm3d_solve_tele(self._h, faces.shape[0], faces, order, precision, 0))
+        flags = REFERENCE_PARTITION if partition else 0
+        self._check(self.L.fmm3d_solve_tele(self._h, faces.shape[0], faces, order, precision, flags))
         self.nsrc = faces.shape[0]
+
+    def passes(self) -> np.ndarray:
+        """Outer passes per source of the last partitioned solve."""
+        out = np.zeros(self.nsrc, np.int32)
+        self._check(self.L.fmm3d_get_passes(self._h, self.nsrc, out))
+        return out
 
     def solve_tele_sides(self, faces: np.ndarray, xface: np.ndarray, zface: np.ndarray, side: np.ndarray,
                          order: int = ORDER_MIXED, precision: int = PREC_FP64) -> None:
