@@ -94,13 +94,16 @@ MODULE fm3dt_run
       CHARACTER(LEN=20) :: sources, itimes, receivers, gridv, rtravel, frechet
       REAL(C_DOUBLE) :: earth
       INTEGER :: fom, fsrt, cfd
-      INTEGER :: dev, batch, prec
+      INTEGER :: dev, batch, prec, flags
    END TYPE run_config
 CONTAINS
    ! fm3dt_parallel.in (fm3dt_parallel.f90:6082-6102) and the environment:
    ! FMM3D_DEVICE (default 0), FMM3D_PRECISION=fp32 for the float32 solver
    ! (default float64, the reference's arithmetic), FMM3D_BATCH (sources per
-   ! device call, default 64)
+   ! device call, default 64).  In float64 the default solve reproduces the
+   ! reference's own output, its 2x2x2 partitioned result
+   ! (FMM3D_REFERENCE_PARTITION); FMM3D_DOMAIN=whole solves the grid as one
+   ! domain instead (the float32 solver always does).
    SUBROUTINE read_config(cfg)
       TYPE(run_config), INTENT(OUT) :: cfg
       CHARACTER(LEN=20) :: cdum
@@ -136,6 +139,10 @@ CONTAINS
       cfg%prec = 0
       CALL GET_ENVIRONMENT_VARIABLE('FMM3D_PRECISION', env, STATUS=ios)
       IF (ios == 0 .AND. TRIM(env) == 'fp32') cfg%prec = 1
+      cfg%flags = 0
+      IF (cfg%prec == 0) cfg%flags = 2   ! FMM3D_REFERENCE_PARTITION
+      CALL GET_ENVIRONMENT_VARIABLE('FMM3D_DOMAIN', env, STATUS=ios)
+      IF (ios == 0 .AND. TRIM(env) == 'whole') cfg%flags = 0
    END SUBROUTINE read_config
 
    ! the .vtx model (fm3dt_parallel.f90:6103-6126): header verbatim, values r fastest
@@ -224,7 +231,8 @@ CONTAINS
             ALLOCATE(faces(INT(g%nnx, 8) * g%nnz * nb), tt(nrc * nb))
             READ(itu, POS=1 + face_rec * 8 * INT(g%nnx, 8) * g%nnz) faces
             face_rec = face_rec + nb
-            CALL check(fmm3d_solve_tele(ctx, INT(nb, C_INT), faces, INT(cfg%fom, C_INT), INT(cfg%prec, C_INT), 0_C_INT), &
+            CALL check(fmm3d_solve_tele(ctx, INT(nb, C_INT), faces, INT(cfg%fom, C_INT), INT(cfg%prec, C_INT), &
+                                        INT(cfg%flags, C_INT)), &
                        ctx, 'fmm3d_solve_tele')
             IF (cfg%fsrt == 1) THEN
                CALL check(fmm3d_receivers(ctx, INT(nrc, C_INT), rcv, tt), ctx, 'fmm3d_receivers')

@@ -2,9 +2,11 @@
 separate forward runs (fm3dt_gpu), with a stub subinv that rescales the
 model by an amount that depends on the iteration it reads from subiter.in.
 Both must leave the same files: rtimes.dat (the first pass), subiter.in,
-and the last pass's rtravel.out and frechet.out (float64 solves that stop
-at a relative change of 1e-14: times within 1e-5 s, Frechet columns equal
-and values within 1e-5 relative)."""
+and the last pass's rtravel.out and frechet.out (times within 1e-5 s,
+Frechet columns equal and values within 1e-5 relative).  The last pass is
+also checked against the oracle: the reference's partitioned solve
+(or_fmm_tele_part) on the model subinv left, whose receiver times the
+drop-in's default solve reproduces bit for bit."""
 import os
 import shutil
 import subprocess
@@ -14,6 +16,7 @@ import numpy as np
 import pytest
 
 from fmm3d import build, io, synth
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -34,9 +37,10 @@ with open("stub.log", "a") as log:
 
 def _case(d):
     g, velv, srcs, rcv = synth.tele_case((31, 33, 35), 400.0, (10.0, 10.0), (10, 11, 12), 0.05, 4, 10, 31)
-    synth.write_case(str(d), g, velv, srcs, rcv, fom=1, cfd=1)
+    info = synth.write_case(str(d), g, velv, srcs, rcv, fom=1, cfd=1)
     shutil.copy(d / "gridc.vtx", d / "gridi.vtx")
     (d / "stub.py").write_text(STUB)
+    return info["faces"]
 
 
 def _frechet_close(a, b):
@@ -53,7 +57,8 @@ def test_resident_loop_matches_subsh_sequence(tmp_path):
     stub = f"{sys.executable} stub.py"
     a, b = tmp_path / "resident", tmp_path / "subsh"
     a.mkdir(); b.mkdir()
-    _case(a); _case(b)
+    faces = _case(a)
+    _case(b)
     print("fm3dt_invert, 2 iterations", flush=True)
     r = subprocess.run([build.INVERT_EXE, "2", stub], cwd=str(a), capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -80,6 +85,11 @@ def test_resident_loop_matches_subsh_sequence(tmp_path):
     # the model changed, and so did the times: the loop re-gridded each pass
     assert np.abs(la - first).max() > 1e-3
     _frechet_close(a / "frechet.out", b / "frechet.out")
+    # the last pass against the oracle on the final model
+    g, velv = io.read_vtx(str(a / "gridc.vtx"))
+    rcv = io.read_receivers(str(a / "receivers.dat"))[0]
+    want = np.concatenate([O.srtimes(g, O.fmm_tele_part(g, velv, f, 1)[0], rcv) for f in faces])
+    assert np.array_equal(la, want), float(np.abs(la - want).max())
 
 
 def test_resident_loop_stops_when_subinv_fails(tmp_path):
