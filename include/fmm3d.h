@@ -167,6 +167,18 @@ int fmm3d_solve_mixed(fmm3d_ctx *ctx, int ntele, const double *faces, const doub
  * A receiver outside the grid yields NaN. */
 int fmm3d_receivers(fmm3d_ctx *ctx, int nrc, const double *rcv, float *t_out);
 
+/* fmm3d_receivers without any device allocation: the receiver coordinates
+ * and the times are staged in a fixed device array that the library holds
+ * on each device it runs on until the process ends (found at context
+ * creation), so nrc*3*8 bytes (rounded up to 256) + nsrc*nrc*4 bytes must
+ * not exceed 8 MiB, else FMM3D_E_ARG.  Same results as fmm3d_receivers.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Every call on a device shares that
+ * array, so calls that overlap on one GPU (two host threads, two contexts)
+ * race; serialise them. */
+int fmm3d_receivers_noalloc(fmm3d_ctx *ctx, int nrc, const double *rcv, float *t_out);
+
 /* Rays and Frechet rows for every resident source and receiver
  * (parallel_rpaths + the writer's row assembly, fm3dt_parallel.f90:
  * 7250-7777, 6503-6599).  Rows are source-major then receiver order;

@@ -1806,6 +1806,7 @@ struct fmm3d_ctx {
     unsigned long long* d_visits = nullptr;
     float* ray_scratch = nullptr;
     PartState* part = nullptr;     // FMM3D_REFERENCE_PARTITION solves (allocated at first use)
+    void* scratch = nullptr;       // this device's fmm3d_receivers_noalloc staging (device address)
     bool part_last = false;        // the resident fields come from a partitioned solve
     size_t ray_scratch_workers = 0;
     // Frechet row staging (per ray `fr_rcap` entries), counts, offsets and the packed rows
@@ -1946,6 +1947,35 @@ static int upload_metrics(fmm3d_ctx* c)
     return FMM3D_OK;
 }
 
+// ------------------------------------------------------------ allocation-free staging
+
+// fmm3d_receivers_noalloc stages receiver coordinates and times here rather
+// than in device memory allocated per call.  Each device holds its own copy
+// until the process ends; every call on a device shares it.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)8 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[MAX_DEVICES] = {};   // device address of `scratch`, per device
+
+// The device address of this device's scratch array, looked up once per
+// device with that device current.  In host code the array's name is only
+// a stand-in for the symbol: kernels and copies take this address.
+static hipError_t scratch_address(int device, void** out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_dev[device] = p;
+    }
+    *out = g_scratch_dev[device];
+    return hipSuccess;
+}
+
 // Device resources of a context whose grid is set (also used for the
 // refined sub-grid of point sources).
 static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
@@ -1965,6 +1995,7 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
             return rc;                                                                         \
         }                                                                                      \
     } while (0)
+    CREATECHK(scratch_address(device, &c->scratch));
     CREATECHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     CREATECHK(hipEventCreate(&c->ev0));
     CREATECHK(hipEventCreate(&c->ev1));
@@ -3815,6 +3846,39 @@ extern "C" int fmm3d_receivers(fmm3d_ctx* c, int nrc, const double* rcv, float* 
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(drcv); (void)hipFree(dout);
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "receivers: %s", hipGetErrorString(e));
+    return FMM3D_OK;
+}
+
+// fmm3d_receivers with its staging in the device's scratch array: no
+// device memory is allocated or freed.
+extern "C" int fmm3d_receivers_noalloc(fmm3d_ctx* c, int nrc, const double* rcv, float* t_out)
+{
+    if (!c || (nrc > 0 && (!rcv || !t_out)) || nrc < 0) return fail(c, FMM3D_E_ARG, "fmm3d_receivers_noalloc: bad arguments");
+    if (c->sl.n <= 0) return fail(c, FMM3D_E_STATE, "fmm3d_receivers_noalloc: no resident solution");
+    if (nrc == 0) return FMM3D_OK;
+    const int ns = c->sl.n;
+    const size_t rb = ((size_t)nrc * 3 * sizeof(double) + 255) & ~(size_t)255;
+    const size_t need = rb + (size_t)ns * nrc * sizeof(float);
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(c, FMM3D_E_ARG, "fmm3d_receivers_noalloc: %d receivers x %d sources need %zu bytes of staging, "
+                    "above its maximum of %zu", nrc, ns, need, NOALLOC_SCRATCH_BYTES);
+    if (!c->scratch) return fail(c, FMM3D_E_STATE, "fmm3d_receivers_noalloc: no scratch address on this device");
+    HIPCHK(c, hipSetDevice(c->device));
+    unsigned char* base = (unsigned char*)c->scratch;
+    double* drcv = (double*)base;
+    float* dout = (float*)(base + rb);
+    hipError_t e = hipMemcpyAsync(drcv, rcv, (size_t)nrc * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    const int total = ns * nrc;
+    if (e == hipSuccess) {
+        if (c->sl.prec == FMM3D_PREC_FP64)
+            k_receivers<double><<<(total + 255) / 256, 256, 0, c->stream>>>(c->g, (const double*)c->sl.T, c->sl.toff, ns, nrc, drcv, dout);
+        else
+            k_receivers<float><<<(total + 255) / 256, 256, 0, c->stream>>>(c->g, (const float*)c->sl.T, c->sl.toff, ns, nrc, drcv, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(t_out, dout, (size_t)total * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "receivers (no allocation): %s", hipGetErrorString(e));
     return FMM3D_OK;
 }
 

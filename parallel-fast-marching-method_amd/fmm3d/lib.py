@@ -28,7 +28,7 @@ EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model
            "fmm3d_solve_fixed", "fmm3d_solve_point", "fmm3d_solve_tele_sides", "fmm3d_solve_mixed",
            "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_result",
            "fmm3d_server_stats", "fmm3d_server_destroy",
-           "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_frechet", "fmm3d_frechet_pinned", "fmm3d_get_field", "fmm3d_get_passes",
+           "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_receivers_noalloc", "fmm3d_frechet", "fmm3d_frechet_pinned", "fmm3d_get_field", "fmm3d_get_passes",
            "fmm3d_get_stats", "fmm3d_set_option")
 
 
@@ -105,6 +105,7 @@ def load(path: Optional[str] = None):
     # nullable arrays go as plain addresses
     L.fmm3d_solve_mixed.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, C.c_int]
     L.fmm3d_receivers.argtypes = [vp, C.c_int, dp, fp]
+    L.fmm3d_receivers_noalloc.argtypes = [vp, C.c_int, dp, fp]
     L.fmm3d_frechet.argtypes = [vp, C.c_int, dp, lp, ip, fp, C.c_int64]
     L.fmm3d_frechet_pinned.argtypes = [vp, C.c_int, dp, C.POINTER(C.POINTER(C.c_int64)),
                                        C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_float)),
@@ -263,6 +264,14 @@ class Context:
         rcv = np.ascontiguousarray(rcv, dtype=np.float64)
         out = np.zeros(self.nsrc * len(rcv), dtype=np.float32)
         self._check(self.L.fmm3d_receivers(self._h, len(rcv), rcv, out))
+        return out.reshape(self.nsrc, len(rcv))
+
+    def receivers_noalloc(self, rcv: np.ndarray) -> np.ndarray:
+        """fmm3d_receivers_noalloc: the same times, staged in the library's
+        fixed device array (at most 8 MiB of coordinates and times)."""
+        rcv = np.ascontiguousarray(rcv, dtype=np.float64)
+        out = np.zeros(self.nsrc * len(rcv), dtype=np.float32)
+        self._check(self.L.fmm3d_receivers_noalloc(self._h, len(rcv), rcv, out))
         return out.reshape(self.nsrc, len(rcv))
 
     def frechet(self, rcv: np.ndarray, cap: int = 0, pinned: bool = False) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
