@@ -133,6 +133,28 @@ def one_step(ctx, work, rcv, prec):
     return t, st, int(rp[-1])
 
 
+def row_errors(ctx, work, rcv, nv):
+    """Frechet rows traced on the float32 field against those on the float64
+    field of the same batch: per row the relative 2-norm of the difference,
+    ||r32 - r64|| / ||r64||; median, p99, max and the share of rows above
+    1e-4."""
+    from scipy.sparse import csr_matrix
+
+    def rows(prec):
+        work.solve(ctx, prec)
+        rp, cols, vals = ctx.frechet(rcv)
+        return csr_matrix((vals.astype(np.float64), cols.astype(np.int64) - 1, rp), shape=(len(rp) - 1, nv))
+    a, b = rows(lib.PREC_FP64), rows(lib.PREC_FP32)
+    d = np.sqrt(np.asarray((b - a).multiply(b - a).sum(axis=1)).ravel())
+    n = np.sqrt(np.asarray(a.multiply(a).sum(axis=1)).ravel())
+    ok = n > 0
+    rel = d[ok] / n[ok]
+    return {"rows": int(ok.sum()), "median": float(np.median(rel)), "p99": float(np.percentile(rel, 99)),
+            "max": float(rel.max()), "frac_above_1e-4": round(float((rel > 1e-4).mean()), 4),
+            "definition": "per row ||r32 - r64||_2 / ||r64||_2, rows traced on the float32 and the float64 "
+                          "fields of the timed batch"}
+
+
 def serving(g, velv, faces, rcv, seconds, t_src):
     """The serving loop under Poisson arrivals: requests of 4 float32 plane-wave
     sources (bottom faces, receiver times only) on the resident 201^3 model,
@@ -274,7 +296,15 @@ def mixed256(a):
                                    f"{a.nrc} receivers; receiver times + Frechet rows"},
             "solve_wall_s": med("solve_wall_s"), "device_solve_ms": med("device_solve_ms"),
             "sweeps": runs[-1]["sweeps"], "frechet_entries": runs[-1]["entries"],
-            "max_receiver_err_s_vs_fp64": err, "fp64_step_s": round(r64["step_s"], 3),
+            "max_receiver_err_s_vs_fp64": err,
+        "frechet_row_err_vs_fp64": rows_err,
+        "graph_timed": True,
+        "ms_per_step_direct_launches": round(1e3 * side["direct_elapsed"] / a.steps, 2),
+        "reference_partition_fp64": {
+            "source_solves_per_s": round(npart / part_s, 3), "sources": npart,
+            "passes": part_passes.tolist(),
+            "what": "the reference's own 2x2x2-partitioned float64 solve (FMM3D_REFERENCE_PARTITION, bottom "
+                    "faces), receivers and rows; reproduces the reference program's rtravel.out"}, "fp64_step_s": round(r64["step_s"], 3),
             "tele_only": {k: round(tele_only[k], 3) for k in ("solve_wall_s", "device_solve_ms", "step_s")} | {"sweeps": tele_only["sweeps"]},
             "point_only": {k: round(point_only[k], 3) for k in ("solve_wall_s", "device_solve_ms", "step_s")} | {"sweeps": point_only["sweeps"]}}
 
@@ -383,11 +413,13 @@ def main():
     rcv = synth.receivers_rad(rcv_deg)
     ctx = lib.Context(g)
     ctx.set_model(velv)
-    ctx.set_option("timing", 1)
 
     def measure(work, steps, warmup, plain_n):
-        """float32 error against float64, the timed float32 batches, and the
-        plain float64 solver (one source per call) on the same sources"""
+        """float32 error against float64, the timed float32 batches (sweeps
+        replayed from captured hipGraphs), the relaxation kernel's launches
+        timed with HIP events on the solver's stream over as many batches
+        again (direct launches), and the plain float64 solver (one source
+        per call) on the same sources"""
         t64, _, _ = one_step(ctx, work, rcv, lib.PREC_FP64)
         t32, _, _ = one_step(ctx, work, rcv, lib.PREC_FP32)
         r = {"t32": t32, "err": float(np.nanmax(np.abs(t32.astype(np.float64) - t64))), "lat": [],
@@ -399,13 +431,20 @@ def main():
             s0 = time.perf_counter()
             _, st, n = one_step(ctx, work, rcv, lib.PREC_FP32)
             r["lat"].append(1e3 * (time.perf_counter() - s0))
+            r["nnz"] += n
+        r["elapsed"] = time.perf_counter() - t0
+        r["value"] = a.batch * steps / r["elapsed"]
+        # per-launch kernel time: every relaxation launch between HIP events
+        ctx.set_option("timing", 1)
+        d0 = time.perf_counter()
+        for _ in range(steps):
+            _, st, n = one_step(ctx, work, rcv, lib.PREC_FP32)
             r["upd"] += st["node_updates"]
             r["nvis"] += st["node_visits"]
             r["kms"] += st["kernel_ms"]
             r["launches"] += st["kernel_launches"]
-            r["nnz"] += n
-        r["elapsed"] = time.perf_counter() - t0
-        r["value"] = a.batch * steps / r["elapsed"]
+        r["direct_elapsed"] = time.perf_counter() - d0
+        ctx.set_option("timing", 0)
         plain_t = []
         for s in range(plain_n):
             s0 = time.perf_counter()
@@ -423,6 +462,18 @@ def main():
     upd, nvis, kms, launches, nnz = side["upd"], side["nvis"], side["kms"], side["launches"], side["nnz"]
     faces = bottom_work.faces
 
+    rows_err = row_errors(ctx, Workload(g, srcs, True), rcv, (g.nvr + 2) * (g.nvx + 2) * (g.nvz + 2))
+    # the reference's own result on the GPU: the 2x2x2-partitioned float64
+    # solve (bottom faces) with receivers and rows, on the first sources
+    npart = min(8, a.batch)
+    part_w = bottom_work.take(0, npart)
+    one_step_part = lambda: (ctx.solve_tele(part_w.faces, lib.ORDER_MIXED, lib.PREC_FP64, partition=True),
+                             ctx.receivers(rcv), ctx.frechet(rcv, pinned=True))
+    one_step_part()
+    p0 = time.perf_counter()
+    one_step_part()
+    part_s = time.perf_counter() - p0
+    part_passes = ctx.passes()
     avg_launch_ms = kms / max(launches, 1)
     bytes_per_launch = BYTES_PER_NODE_VISIT * nvis / max(launches, 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
@@ -447,6 +498,14 @@ def main():
         "plain_fp64_source_solves_per_s": round(plain, 3) if plain else None,
         "ratio_vs_plain_fp64": round(value / plain, 2) if plain else None,
         "max_receiver_err_s_vs_fp64": err,
+        "frechet_row_err_vs_fp64": rows_err,
+        "graph_timed": True,
+        "ms_per_step_direct_launches": round(1e3 * side["direct_elapsed"] / a.steps, 2),
+        "reference_partition_fp64": {
+            "source_solves_per_s": round(npart / part_s, 3), "sources": npart,
+            "passes": part_passes.tolist(),
+            "what": "the reference's own 2x2x2-partitioned float64 solve (FMM3D_REFERENCE_PARTITION, bottom "
+                    "faces), receivers and rows; reproduces the reference program's rtravel.out"},
         "latency_ms_per_batch": {"p50": round(float(np.percentile(lat, 50)), 2),
                                  "p99": round(float(np.percentile(lat, 99)), 2)},
         "frechet_entries_per_step": nnz // max(a.steps, 1),
@@ -454,6 +513,10 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(MEASURED_BYTES_PER_NODE_VISIT * nvis / max(launches, 1)),
+                     "traffic_source": "extrapolated: HBM bytes per node visit from the rocprofv3 FETCH_SIZE/"
+                                       "WRITE_SIZE pass in profiles/ times this run's node visits per launch",
+                     "launch_timing": "HIP events around each k_column launch on the solver's stream, over "
+                                      "as many batches again as the timed ones (direct launches)",
                      "avg_launch_ms": round(avg_launch_ms, 4),
                      "node_visits_per_launch": int(nvis / max(launches, 1)),
                      "algorithmic_bytes_per_node_visit": BYTES_PER_NODE_VISIT,
