@@ -309,6 +309,7 @@ typedef struct {
     int nrc;
     const double *rcv;          /* nrc records (height km, colatitude rad, longitude rad) */
     int order, precision;       /* FMM3D_ORDER_*, FMM3D_PREC_* */
+    int rows;                   /* 1: also trace the rays and keep the Frechet rows (fmm3d_server_rows) */
 } fmm3d_request;
 
 typedef struct {
@@ -329,6 +330,15 @@ int fmm3d_server_submit(fmm3d_server *srv, const fmm3d_request *req, int64_t *ti
 /* FMM3D_OK once done (or failed: result reports it), FMM3D_E_TIMEOUT if not
  * within timeout_s seconds (0: poll). */
 int fmm3d_server_wait(fmm3d_server *srv, int64_t ticket, double timeout_s);
+/* Frechet rows of a finished request submitted with rows = 1, in the layout
+ * of fmm3d_frechet (source-major then receiver order; row_ptr has
+ * nsrc*nrc+1 entries; columns ascending): the reference's rows for every
+ * source and receiver (parallel_rpaths + the writer's merge,
+ * fm3dt_parallel.f90:7250-7777, 6503-6599).  FMM3D_E_CAPACITY with
+ * row_ptr[nsrc*nrc] = entries needed when cap is too small; FMM3D_E_STATE
+ * for a request without rows.  Call before fmm3d_server_result, which
+ * releases the ticket. */
+int fmm3d_server_rows(fmm3d_server *srv, int64_t ticket, int64_t *row_ptr, int32_t *cols, float *vals, int64_t cap);
 /* Receiver times (nsrc*nrc float32, source-major) and the request's latency;
  * releases the ticket.  Returns the request's status. */
 int fmm3d_server_result(fmm3d_server *srv, int64_t ticket, float *times, double *latency_ms);

@@ -52,6 +52,10 @@ struct Request {
     int order = FMM3D_ORDER_MIXED, precision = FMM3D_PREC_FP32;
     std::vector<double> velv, faces, src, rcv;
     std::vector<float> times;
+    bool want_rows = false;                // trace rays and keep Frechet rows (fmm3d_request.rows)
+    std::vector<std::vector<int64_t>> rrp; // per source: nrc+1 row offsets
+    std::vector<std::vector<int32_t>> rcols;
+    std::vector<std::vector<float>> rvals;
     int admitted = 0, done = 0;
     int status = FMM3D_OK;
     std::atomic<bool> finished{false};     // set under the mutex, read by the scheduler without it
@@ -373,8 +377,20 @@ void fmm3d_server::run()
                 rrcv.push_back(r->rcv.data());
                 rout.push_back(r->times.data() + (size_t)m->owner[slot].second * r->nrc);
             }
-            const int rst = fmm3d_engine_receivers_many(m->ctx, (int)rslots.size(), rslots.data(), rnrc.data(),
-                                                        rrcv.data(), rout.data());
+            int rst = fmm3d_engine_receivers_many(m->ctx, (int)rslots.size(), rslots.data(), rnrc.data(),
+                                                  rrcv.data(), rout.data());
+            // rays and Frechet rows of the converged slots whose request asked for them
+            for (int slot : done) {
+                Request* r = m->owner[slot].first;
+                if (!r || !r->want_rows || rst != FMM3D_OK) continue;
+                const int q = m->owner[slot].second;
+                const int frc = fmm3d_engine_frechet(m->ctx, slot, r->nrc, r->rcv.data(), r->rrp[q], r->rcols[q], r->rvals[q]);
+                if (frc != FMM3D_OK) {
+                    trace_failure(r, frc, fmm3d_last_error(m->ctx));
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (r->status == FMM3D_OK) r->status = frc;
+                }
+            }
             for (int pass = 0; pass < 2; ++pass)
                 for (int slot : (pass == 0 ? done : failed)) {
                     Request* r = m->owner[slot].first;
@@ -446,6 +462,12 @@ extern "C" int fmm3d_server_submit(fmm3d_server* s, const fmm3d_request* q, int6
     else r->src.assign(q->src, q->src + 3 * (size_t)q->nsrc);
     r->rcv.assign(q->rcv, q->rcv + 3 * (size_t)q->nrc);
     r->times.assign((size_t)q->nsrc * q->nrc, NAN);
+    r->want_rows = q->rows != 0;
+    if (r->want_rows) {
+        r->rrp.assign(q->nsrc, std::vector<int64_t>((size_t)q->nrc + 1, 0));
+        r->rcols.resize(q->nsrc);
+        r->rvals.resize(q->nsrc);
+    }
     if (q->kind == FMM3D_SRC_POINT) {
         r->pnode.resize(q->nsrc);
         r->ptv.resize(q->nsrc);
@@ -477,6 +499,37 @@ extern "C" int fmm3d_server_wait(fmm3d_server* s, int64_t ticket, double timeout
     if (timeout_s <= 0.0) return FMM3D_E_TIMEOUT;
     const bool ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] { return r->finished.load(); });
     return ok ? FMM3D_OK : FMM3D_E_TIMEOUT;
+}
+
+extern "C" int fmm3d_server_rows(fmm3d_server* s, int64_t ticket, int64_t* row_ptr, int32_t* cols, float* vals,
+                                 int64_t cap)
+{
+    if (!s || !row_ptr || cap < 0) return FMM3D_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return FMM3D_E_ARG;
+    Request* r = it->second.get();
+    if (!r->finished) return FMM3D_E_TIMEOUT;
+    if (r->released || !r->want_rows) return FMM3D_E_STATE;
+    if (r->status != FMM3D_OK) return r->status;
+    // source-major, receiver order: the layout of fmm3d_frechet
+    row_ptr[0] = 0;
+    int64_t k = 0;
+    for (int q = 0; q < r->nsrc; ++q)
+        for (int i = 0; i < r->nrc; ++i, ++k) row_ptr[k + 1] = row_ptr[k] + (r->rrp[q][i + 1] - r->rrp[q][i]);
+    const int64_t need = row_ptr[k];
+    if (need > cap) return FMM3D_E_CAPACITY;
+    if (need > 0 && (!cols || !vals)) return FMM3D_E_ARG;
+    int64_t at = 0;
+    for (int q = 0; q < r->nsrc; ++q) {
+        const size_t n = r->rcols[q].size();
+        if (n) {
+            memcpy(cols + at, r->rcols[q].data(), n * sizeof(int32_t));
+            memcpy(vals + at, r->rvals[q].data(), n * sizeof(float));
+        }
+        at += (int64_t)n;
+    }
+    return FMM3D_OK;
 }
 
 extern "C" int fmm3d_server_result(fmm3d_server* s, int64_t ticket, float* times, double* latency_ms)

@@ -3557,37 +3557,42 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
     // float64: in float32 the refined block moved the coarse field by up to
     // 1.8 ms (61^3, refinement 5 over 10 nodes).  With point_warm the
     // float64 sweeps start from a float32 solve's field.
-    if (c->point_warm) {
+    // A failure anywhere in the warm start (the float32 solve, its staging,
+    // the float64 sweeps from its field) falls back to the cold float64 solve.
+    bool warm_ok = false;
+    while (c->point_warm) {
         rc = fmm3d_solve_fixed(fc, k, offs.data(), fnode.data(), ftime.data(), order, FMM3D_PREC_FP32, FMM3D_WALK_ALTERNATE);
-        if (rc) return fail_to(err, rc, "refined float32 solve: %s", fc->err.c_str());
+        if (rc) break;
         const size_t n = (size_t)k * fc->N;
         if (fc->d_warm_n < n) {
             (void)hipFree(fc->d_warm); (void)hipFree(fc->d_warm_off);
             fc->d_warm = nullptr; fc->d_warm_off = nullptr; fc->d_warm_n = 0;
             if (dalloc(&fc->d_warm, n * sizeof(float)) != hipSuccess || dalloc(&fc->d_warm_off, (size_t)k * sizeof(double) + 256) != hipSuccess)
-                return fail_to(err, FMM3D_E_NOMEM, "refined warm start: no device memory");
+                break;
             fc->d_warm_n = n;
         }
         hipError_t e = hipMemcpyAsync(fc->d_warm, fc->sl.T, n * sizeof(float), hipMemcpyDeviceToDevice, fc->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(fc->d_warm_off, fc->sl.toff, (size_t)k * sizeof(double), hipMemcpyDeviceToDevice, fc->stream);
-        if (e != hipSuccess) return fail_to(err, FMM3D_E_DEVICE, "refined warm start: %s", hipGetErrorString(e));
+        if (e != hipSuccess) break;
         // the float64 fields: fixed sets as in a cold solve, free nodes from the float32 field
         std::vector<double> toff;
         std::vector<int> flow;
         if ((rc = fixed_prepare(fc, "refined solve", k, offs.data(), fnode.data(), ftime.data(), toff, flow, fc->point_order != 0)) ||
             (rc = ensure_slots(fc, k, FMM3D_PREC_FP64)))
-            return fail_to(err, rc, "refined solve: %s", fc->err.c_str());
+            break;
         fc->sl.n = 0;
         fc->walk_alt = 1;
         if ((rc = fixed_upload(fc, "refined solve", 0, k, offs.data(), fnode.data(), ftime.data(), FMM3D_PREC_FP64, toff, flow)))
-            return fail_to(err, rc, "refined solve: %s", fc->err.c_str());
+            break;
         k_warm_start<<<(int)std::min<size_t>((n + 255) / 256, 16384), 256, 0, fc->stream>>>(
             (double*)fc->sl.T, fc->d_warm, fc->d_warm_off, fc->sl.fix, k, fc->N);
-        if ((e = hipGetLastError()) != hipSuccess) return fail_to(err, FMM3D_E_DEVICE, "refined warm start: %s", hipGetErrorString(e));
+        if ((e = hipGetLastError()) != hipSuccess) break;
         rc = solve_run(fc, k, order, FMM3D_PREC_FP64);
-    } else {
-        rc = fmm3d_solve_fixed(fc, k, offs.data(), fnode.data(), ftime.data(), order, FMM3D_PREC_FP64, FMM3D_WALK_ALTERNATE);
+        warm_ok = rc == FMM3D_OK;
+        break;
     }
+    if (!warm_ok)
+        rc = fmm3d_solve_fixed(fc, k, offs.data(), fnode.data(), ftime.data(), order, FMM3D_PREC_FP64, FMM3D_WALK_ALTERNATE);
     if (rc) return fail_to(err, rc, "refined solve: %s", fc->err.c_str());
     // the coarse nodes each block covers seed the coarse solve: gathered on
     // the device, so only they cross to the host
@@ -3882,18 +3887,16 @@ extern "C" int fmm3d_receivers_noalloc(fmm3d_ctx* c, int nrc, const double* rcv,
     return FMM3D_OK;
 }
 
-extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* row_ptr, int32_t* cols,
-                             float* vals, int64_t cap)
+// Rays ray_begin .. ray_end-1 (ray slot*nrc + i: slot's field, receiver i)
+// with their Frechet rows; row_ptr has ray_end-ray_begin+1 entries.
+static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin, int ray_end, int64_t* row_ptr,
+                         int32_t* cols, float* vals, int64_t cap)
 {
-    if (!c || nrc < 0 || (nrc > 0 && (!rcv || !row_ptr)) || cap < 0) return fail(c, FMM3D_E_ARG, "fmm3d_frechet: bad arguments");
-    if (c->sl.n <= 0) return fail(c, FMM3D_E_STATE, "fmm3d_frechet: no resident solution");
-    if (!c->have_model) return fail(c, FMM3D_E_STATE, "fmm3d_frechet: no velocity model");
     HIPCHK(c, hipSetDevice(c->device));
     const Grid& g = c->g;
-    const int ns = c->sl.n;
-    const int nrays = ns * nrc;
+    const int nrays = ray_end - ray_begin;
     row_ptr[0] = 0;
-    if (nrays == 0) return FMM3D_OK;
+    if (nrays <= 0) return FMM3D_OK;
     const size_t NV = (size_t)(g.nvr + 2) * (g.nvx + 2) * (g.nvz + 2);
     // ray workers: one wavefront each, with a dense float32 scratch of NV
     const size_t workers = std::min<size_t>(16384, (size_t)nrays);
@@ -3922,10 +3925,10 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
     std::vector<int64_t> off;
     int64_t total = 0;
     bool short_cap = false;
-    for (int r0 = 0; r0 < nrays && e == hipSuccess;) {
+    for (int r0 = ray_begin; r0 < ray_end && e == hipSuccess;) {
         const int64_t rcap = std::min<int64_t>(c->fr_rcap, (int64_t)NV);
         const int64_t pcap = c->fr_pcap;
-        const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>({ (int64_t)(nrays - r0), (int64_t)(budget / (rcap * 8)),
+        const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>({ (int64_t)(ray_end - r0), (int64_t)(budget / (rcap * 8)),
                                                                         (int64_t)(pbudget / (pcap * sizeof(RaySample))) }));
         if (c->fr_path_n < (size_t)chunk * pcap) {
             (void)hipFree(c->fr_path);
@@ -3992,7 +3995,7 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
         int64_t ctot = 0;
         for (int q = 0; q < nr; ++q) {
             const int64_t n = std::max(cnt[q], 0);
-            row_ptr[r0 + q + 1] = row_ptr[r0 + q] + n;
+            row_ptr[r0 - ray_begin + q + 1] = row_ptr[r0 - ray_begin + q] + n;
             off[q] = ctot;
             ctot += n;
         }
@@ -4026,6 +4029,15 @@ extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* 
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "frechet: %s", hipGetErrorString(e));
     if (short_cap) return fail(c, FMM3D_E_CAPACITY, "fmm3d_frechet: capacity %lld < %lld entries", (long long)cap, (long long)row_ptr[nrays]);
     return FMM3D_OK;
+}
+
+extern "C" int fmm3d_frechet(fmm3d_ctx* c, int nrc, const double* rcv, int64_t* row_ptr, int32_t* cols,
+                             float* vals, int64_t cap)
+{
+    if (!c || nrc < 0 || (nrc > 0 && (!rcv || !row_ptr)) || cap < 0) return fail(c, FMM3D_E_ARG, "fmm3d_frechet: bad arguments");
+    if (c->sl.n <= 0) return fail(c, FMM3D_E_STATE, "fmm3d_frechet: no resident solution");
+    if (!c->have_model) return fail(c, FMM3D_E_STATE, "fmm3d_frechet: no velocity model");
+    return frechet_range(c, nrc, rcv, 0, c->sl.n * nrc, row_ptr, cols, vals, cap);
 }
 
 // fmm3d_frechet into the context's own pinned host arrays: the copies from
@@ -4373,6 +4385,29 @@ int fmm3d_engine_receivers(fmm3d_ctx* c, int slot, int nrc, const double* rcv, f
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "engine receivers: %s", hipGetErrorString(e));
     return FMM3D_OK;
+}
+
+// One slot's rays and Frechet rows for nrc receivers, into host vectors
+// (rp: nrc+1 offsets); capacity grows to the exact size on a second try.
+int fmm3d_engine_frechet(fmm3d_ctx* c, int slot, int nrc, const double* rcv, std::vector<int64_t>& rp,
+                         std::vector<int32_t>& cols, std::vector<float>& vals)
+{
+    if (!c || slot < 0 || slot >= c->sl.cap || nrc < 0) return fail(c, FMM3D_E_ARG, "engine frechet: bad arguments");
+    rp.assign((size_t)nrc + 1, 0);
+    int64_t cap = std::max<int64_t>((int64_t)cols.capacity(), (int64_t)nrc * 512 + 1024);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        cols.resize(cap);
+        vals.resize(cap);
+        const int rc = frechet_range(c, nrc, rcv, slot * nrc, (slot + 1) * nrc, rp.data(), cols.data(), vals.data(), cap);
+        if (rc == FMM3D_OK) {
+            cols.resize(rp[nrc]);
+            vals.resize(rp[nrc]);
+            return FMM3D_OK;
+        }
+        if (rc != FMM3D_E_CAPACITY) return rc;
+        cap = rp[nrc];
+    }
+    return fail(c, FMM3D_E_CAPACITY, "engine frechet: capacity retry failed");
 }
 
 int fmm3d_engine_retire(fmm3d_ctx* c, int slot)

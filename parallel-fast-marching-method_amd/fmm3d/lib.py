@@ -27,7 +27,7 @@ REFERENCE_PARTITION = 2
 EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model_vtx",
            "fmm3d_solve_fixed", "fmm3d_solve_point", "fmm3d_solve_tele_sides", "fmm3d_solve_mixed",
            "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_result",
-           "fmm3d_server_stats", "fmm3d_server_destroy",
+           "fmm3d_server_stats", "fmm3d_server_destroy", "fmm3d_server_rows",
            "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_receivers_noalloc", "fmm3d_frechet", "fmm3d_frechet_pinned", "fmm3d_get_field", "fmm3d_get_passes",
            "fmm3d_get_stats", "fmm3d_set_option")
 
@@ -54,7 +54,7 @@ class RequestC(C.Structure):
     _fields_ = [("grid", GridDescC), ("velv", C.c_void_p), ("model_id", C.c_uint64), ("kind", C.c_int),
                 ("nsrc", C.c_int), ("faces", C.c_void_p), ("src", C.c_void_p), ("refine", C.c_int),
                 ("radius", C.c_int), ("nrc", C.c_int), ("rcv", C.c_void_p), ("order", C.c_int),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("rows", C.c_int)]
 
 
 class ServerInfoC(C.Structure):
@@ -98,6 +98,7 @@ def load(path: Optional[str] = None):
     L.fmm3d_server_submit.argtypes = [vp, C.POINTER(RequestC), C.POINTER(C.c_int64)]
     L.fmm3d_server_wait.argtypes = [vp, C.c_int64, C.c_double]
     L.fmm3d_server_result.argtypes = [vp, C.c_int64, fp, C.POINTER(C.c_double)]
+    L.fmm3d_server_rows.argtypes = [vp, C.c_int64, lp, ip, fp, C.c_int64]
     L.fmm3d_server_stats.argtypes = [vp, C.POINTER(ServerInfoC)]
     L.fmm3d_server_destroy.argtypes = [vp]
     L.fmm3d_solve_fixed.argtypes = [vp, C.c_int, lp, lp, dp, C.c_int, C.c_int, C.c_int]
@@ -335,7 +336,7 @@ class Server:
         self._keep = {}
 
     def submit(self, g, velv, model_id: int, *, faces=None, src=None, refine: int = 5, radius: int = 10,
-               rcv=None, order: int = ORDER_MIXED, precision: int = PREC_FP32) -> int:
+               rcv=None, order: int = ORDER_MIXED, precision: int = PREC_FP32, rows: bool = False) -> int:
         velv = np.ascontiguousarray(velv, dtype=np.float64)
         rcv = np.ascontiguousarray(np.zeros((0, 3)) if rcv is None else rcv, dtype=np.float64)
         r = RequestC()
@@ -355,12 +356,31 @@ class Server:
         r.refine, r.radius = refine, radius
         r.nrc, r.rcv = rcv.shape[0], rcv.ctypes.data if rcv.size else None
         r.order, r.precision = order, precision
+        r.rows = 1 if rows else 0
         t = C.c_int64()
         rc = self.L.fmm3d_server_submit(self._h, C.byref(r), C.byref(t))
         if rc != OK:
             raise FmmError(rc, "fmm3d_server_submit rejected the request")
         self._keep[t.value] = (r.nsrc, r.nrc)
         return t.value
+
+    def rows(self, ticket: int):
+        """Frechet rows of a finished request submitted with rows=True:
+        (row_ptr, cols, vals) as Context.frechet; call before result()."""
+        nsrc, nrc = self._keep[ticket]
+        rp = np.zeros(nsrc * nrc + 1, np.int64)
+        cap = nsrc * nrc * 512 + 1024
+        for _ in range(2):
+            cols = np.zeros(cap, np.int32)
+            vals = np.zeros(cap, np.float32)
+            rc = self.L.fmm3d_server_rows(self._h, ticket, rp, cols, vals, cap)
+            if rc == OK:
+                n = int(rp[-1])
+                return rp, cols[:n], vals[:n]
+            if rc != E_CAPACITY:
+                raise FmmError(rc, "fmm3d_server_rows failed")
+            cap = int(rp[-1])
+        raise FmmError(E_CAPACITY, "fmm3d_server_rows capacity retry failed")
 
     def wait(self, ticket: int, timeout_s: float = 60.0) -> bool:
         rc = self.L.fmm3d_server_wait(self._h, ticket, timeout_s)
