@@ -157,10 +157,11 @@ def row_errors(ctx, work, rcv, nv):
 
 def serving(g, velv, faces, rcv, seconds, t_src):
     """The serving loop under Poisson arrivals: requests of 4 float32 plane-wave
-    sources (bottom faces, receiver times only) on the resident 201^3 model,
-    offered at 70 % of the batched bottom-face throughput; p50/p99 request
-    latency and the throughput the server sustains.  Results are checked
-    against the batched solve of the same sources."""
+    sources (bottom faces) on the resident 201^3 model, each with receiver
+    times and Frechet rows (the source-solve of the headline), offered at 70 %
+    of the batched bottom-face throughput; p50/p99 request latency and the
+    throughput the server sustains.  Results are checked against the batched
+    solve of the same sources."""
     rng = np.random.default_rng(5)
     per = 4
     rate = 0.7 * (1.0 / t_src) / per              # requests per second
@@ -170,8 +171,9 @@ def serving(g, velv, faces, rcv, seconds, t_src):
     tickets = []
     with lib.Server(hbm_budget=64e9, max_slots=64) as S:
         # warm the model: the first request creates the resident context
-        t = S.submit(g, velv, 11, faces=faces[:per], rcv=rcv)
+        t = S.submit(g, velv, 11, faces=faces[:per], rcv=rcv, rows=True)
         S.wait(t, 600.0)
+        S.rows(t)
         S.result(t)
 
         t0 = time.perf_counter()
@@ -181,10 +183,11 @@ def serving(g, velv, faces, rcv, seconds, t_src):
             d = next_t - time.perf_counter()
             if d > 0:
                 time.sleep(d)
-            tickets.append((S.submit(g, velv, 11, faces=faces[picks[i]], rcv=rcv), picks[i]))
-        lat, errs = [], []
+            tickets.append((S.submit(g, velv, 11, faces=faces[picks[i]], rcv=rcv, rows=True), picks[i]))
+        lat, errs, nnz = [], [], 0
         for t, idx in tickets:
             S.wait(t, 600.0)
+            nnz += int(S.rows(t)[0][-1])
             out, l = S.result(t)
             lat.append(l)
             errs.append(out)
@@ -192,7 +195,8 @@ def serving(g, velv, faces, rcv, seconds, t_src):
         st = S.stats()
     return {"requests": n, "sources_per_request": per, "offered_req_per_s": round(rate, 2),
             "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
-            "source_solves_per_s": round(n * per / span, 2), "server_hbm_gb": round(st["hbm_bytes"] / 1e9, 2)}, tickets, errs
+            "source_solves_per_s": round(n * per / span, 2), "server_hbm_gb": round(st["hbm_bytes"] / 1e9, 2),
+            "frechet_entries": nnz, "includes_rays": True}, tickets, errs
 
 
 def cpu_baseline_reference(g, velv, srcs, rcv_deg):

@@ -281,6 +281,26 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  * Returns FMM3D_E_ARG for an unknown key or a value out of range. */
 int fmm3d_set_option(fmm3d_ctx *ctx, const char *key, double value);
 
+/* Velocity-perturbation slice of a vertex model, on the device: the
+ * reference's slicer (gmtplot/gmtslicet.f90:4-540).  The vertex model velv
+ * minus velv_ref (or velv itself when velv_ref is NULL), both
+ * (nvz+2)(nvx+2)(nvr+2) r fastest, is B-splined onto the propagation grid
+ * of `grid` and one plane of it is sampled with a second B-spline diced
+ * dice1 x dice2 per cell:
+ *   kind 0: depth slice at height `where` km, dicing (theta, phi);
+ *   kind 1: N-S slice at longitude `where` deg, dicing (r, theta);
+ *   kind 2: E-W slice at latitude `where` deg, dicing (r, phi).
+ * n1 x n2 values go to out in the order gmtslicet writes them (outer loop
+ * over the second axis, inner over the first from its far end);
+ * FMM3D_E_CAPACITY (with *n1, *n2 set) when cap is smaller.  FMM3D_E_ARG
+ * when the slice or a B-spline tap falls outside the grids (gmtslicet
+ * reads out of bounds there).  No context: the call owns its device memory
+ * and stream for its duration.  fmm3d_slice_last_error() gives the message
+ * of the calling thread's last failed slice. */
+int fmm3d_slice_velocity(const fmm3d_grid_desc *grid, int device, const double *velv, const double *velv_ref,
+                         int kind, double where, int dice1, int dice2, double *out, int64_t cap, int *n1, int *n2);
+const char *fmm3d_slice_last_error(void);
+
 /* ---------------------------------------------------------------------------
  * Serving loop (no counterpart in the reference, which runs one source after
  * another per process, fm3dt_parallel.f90:6371).  A server owns one device

@@ -21,6 +21,7 @@
  *                   refined neighbourhood (build-only, no reference)
  *   or_srtimes      trilinear receiver times, float32 sum    F:7040-7248
  *   or_rpaths       steepest-descent rays + Frechet rows     F:7250-7777
+ *   or_slice        velocity-perturbation slices of gmtplot/gmtslicet.f90:4-540
  *
  * Layouts.  Fields are (nnr, nnx, nnz) with phi fastest -- the reference
  * array order ttn(iz,ix,ir) -- and carry no halo at the interface; veln is
@@ -1679,4 +1680,179 @@ static long rpaths(const or_grid *g, const double *veln, const double *ttn, int 
 #undef FDM
     free(fdm);
     return nent;
+}
+
+/* ------------------------------------------------ model slices (gmtslicet) */
+
+/* B-spline basis as gmtplot/gmtslicet.f90 writes it (lines 283-300, 346-375):
+ * (1-u)^3/6, (4-6u^2+3u^3)/6, (1+3u+3u^2-3u^3)/6, u^3/6 */
+static void slice_basis(double u, double w[4])
+{
+    w[0] = (1.0 - u) * (1.0 - u) * (1.0 - u) / 6.0;
+    w[1] = (4.0 - 6.0 * (u * u) + 3.0 * (u * u * u)) / 6.0;
+    w[2] = (1.0 + 3.0 * u + 3.0 * (u * u) - 3.0 * (u * u * u)) / 6.0;
+    w[3] = u * u * u / 6.0;
+}
+
+/* Velocity (perturbation) slices of gmtslicet (gmtplot/gmtslicet.f90:4-540):
+ * the vertex model minus the reference vertex model (vref, or none), the
+ * cubic B-spline on the propagation grid with a one-node border in theta
+ * and phi (velv, lines 273-306), then a second B-spline diced d1 x d2 per
+ * cell on a plane of it:
+ *   kind 0  depth slice at height `where` km, dicing (theta, phi)  (327-386)
+ *   kind 1  N-S slice at longitude `where` deg, dicing (r, theta)  (398-460)
+ *   kind 2  E-W slice at latitude `where` deg, dicing (r, phi)     (470-531)
+ * out receives the values in the order gmtslicet writes them (outer loop
+ * over the second slice axis, inner over the first from its far end);
+ * n1, n2 = the first and second axis lengths.  Vertex arrays are
+ * (nvz+2, nvx+2, nvr+2) with r fastest.  Returns 0, or -1 when a tap falls
+ * outside the arrays (gmtslicet reads out of bounds there). */
+int or_slice(const or_grid *g, const double deg[4], const double *vtrue, const double *vref, int kind,
+             double where, int d1, int d2, double *out, int *n1, int *n2)
+{
+    /* deg: the .vtx header's propagation origin and spacing in degrees,
+     * (lat of the N edge, lat spacing, lon of the W edge, lon spacing):
+     * the N-S and E-W slices are placed with them (lines 398, 470) */
+    const int NR = g->nvr + 2, NX = g->nvx + 2, NZ = g->nvz + 2;
+    const int rr = g->nnr, rt = g->nnx, rp = g->nnz;
+    const size_t nv = (size_t)NR * NX * NZ;
+    double *vv = malloc(nv * sizeof(double));
+    for (size_t i = 0; i < nv; ++i) vv[i] = vtrue[i] - (vref ? vref[i] : 0.0);
+    /* velv(k = 1..rnnr, j = 0..rnnt+1, i = 0..rnnp+1) */
+    const int VJ = rt + 2, VI = rp + 2;
+#define VELV(k, j, i) velv[((size_t)((i)) * VJ + (j)) * rr + ((k) - 1)]
+    double *velv = malloc((size_t)rr * VJ * VI * sizeof(double));
+    int bad = 0;
+    for (int k = 1; k <= rr; ++k) {
+        const int ri = (int)((g->gor - g->rgor + (k - 1) * g->dnr) / g->dvr) + 1;
+        const double distr = -(g->rgor - (k - 1) * g->dnr) + (g->gor - (ri - 1) * g->dvr);
+        double ai[4], bi[4], ci[4];
+        slice_basis(distr / g->dvr, ai);
+        for (int j = 0; j <= rt + 1; ++j) {
+            const int xi = (int)((g->rgox + (j - 1) * g->dnx - g->gox) / g->dvx) + 1;
+            const double distx = g->rgox + (j - 1) * g->dnx - g->gox - (xi - 1) * g->dvx;
+            slice_basis(distx / g->dvx, bi);
+            for (int i = 0; i <= rp + 1; ++i) {
+                const int zi = (int)((g->rgoz + (i - 1) * g->dnz - g->goz) / g->dvz) + 1;
+                const double distz = g->rgoz + (i - 1) * g->dnz - g->goz - (zi - 1) * g->dvz;
+                slice_basis(distz / g->dvz, ci);
+                if (ri - 1 < 0 || ri + 2 >= NR || xi - 1 < 0 || xi + 2 >= NX || zi - 1 < 0 || zi + 2 >= NZ) {
+                    bad = 1;
+                    VELV(k, j, i) = 0.0;
+                    continue;
+                }
+                double sumi = 0.0;
+                for (int i1 = 0; i1 < 4; ++i1) {
+                    double sumj = 0.0;
+                    for (int j1 = 0; j1 < 4; ++j1) {
+                        double sumk = 0.0;
+                        const double *col = vv + ((size_t)(zi - 1 + i1) * NX + (xi - 1 + j1)) * NR;
+                        for (int k1 = 0; k1 < 4; ++k1) sumk = sumk + ai[k1] * col[ri - 1 + k1];
+                        sumj = sumj + bi[j1] * sumk;
+                    }
+                    sumi = sumi + ci[i1] * sumj;
+                }
+                VELV(k, j, i) = sumi;
+            }
+        }
+    }
+    /* dicing weights: ui for the first slice axis, vi for the second, wi across the plane */
+    double *ui = malloc((size_t)(d1 + 1) * 4 * sizeof(double)), *vi = malloc((size_t)(d2 + 1) * 4 * sizeof(double));
+    for (int i = 1; i <= d1 + 1; ++i) slice_basis((double)(i - 1) / (double)d1, ui + 4 * (i - 1));
+    for (int i = 1; i <= d2 + 1; ++i) slice_basis((double)(i - 1) / (double)d2, vi + 4 * (i - 1));
+    double wi[4];
+    int rnode;
+    if (kind == 0) {
+        rnode = (int)((g->rgor - where) / g->dnr) + 1;
+        slice_basis(fabs(g->rgor - where - (rnode - 1) * g->dnr) / g->dnr, wi);
+        *n1 = (rt - 1) * d1 + 1; *n2 = (rp - 1) * d2 + 1;
+    } else if (kind == 1) {
+        rnode = (int)((where - deg[2]) / deg[3]) + 1;
+        slice_basis(fabs(where - deg[2] - (rnode - 1) * deg[3]) / deg[3], wi);
+        *n1 = (rr - 3) * d1 + 1; *n2 = (rt - 1) * d2 + 1;
+    } else {
+        rnode = (int)((deg[0] - where) / deg[1]) + 1;
+        slice_basis(fabs(deg[0] - where - (rnode - 1) * deg[1]) / deg[1], wi);
+        *n1 = (rr - 3) * d1 + 1; *n2 = (rp - 1) * d2 + 1;
+    }
+    const int nx1 = *n1;
+    for (int p = 0; p < nx1 * *n2; ++p) out[p] = 0.0;
+#define OUT(a, b) out[(size_t)((b) - 1) * nx1 + (nx1 - (a))]
+    if (kind == 0) {
+        if (rnode - 1 < 1 || rnode + 2 > rr) bad = 1;
+        else
+            for (int i = 1; i <= rp - 1; ++i) {
+                const int conp = i == rp - 1 ? d2 + 1 : d2;
+                for (int j = 1; j <= rt - 1; ++j) {
+                    const int cont = j == rt - 1 ? d1 + 1 : d1;
+                    for (int l = 1; l <= conp; ++l)
+                        for (int m = 1; m <= cont; ++m) {
+                            double sumi = 0.0;
+                            for (int i1 = 1; i1 <= 4; ++i1) {
+                                double sumj = 0.0;
+                                for (int j1 = 1; j1 <= 4; ++j1) {
+                                    double sumk = 0.0;
+                                    for (int k1 = 1; k1 <= 4; ++k1)
+                                        sumk = sumk + wi[k1 - 1] * VELV(rnode - 2 + k1, j - 2 + j1, i - 2 + i1);
+                                    sumj = sumj + ui[4 * (m - 1) + j1 - 1] * sumk;
+                                }
+                                sumi = sumi + vi[4 * (l - 1) + i1 - 1] * sumj;
+                            }
+                            OUT(d1 * (j - 1) + m, d2 * (i - 1) + l) = sumi;
+                        }
+                }
+            }
+    } else if (kind == 1) {
+        if (rnode - 1 < 0 || rnode + 2 > rp + 1) bad = 1;
+        else
+            for (int j = 1; j <= rt - 1; ++j) {
+                const int cont = j == rt - 1 ? d2 + 1 : d2;
+                for (int k = 2; k <= rr - 2; ++k) {
+                    const int conr = k == rr - 2 ? d1 + 1 : d1;
+                    for (int m = 1; m <= cont; ++m)
+                        for (int n = 1; n <= conr; ++n) {
+                            double sumi = 0.0;
+                            for (int i1 = 1; i1 <= 4; ++i1) {
+                                double sumj = 0.0;
+                                for (int j1 = 1; j1 <= 4; ++j1) {
+                                    double sumk = 0.0;
+                                    for (int k1 = 1; k1 <= 4; ++k1)
+                                        sumk = sumk + ui[4 * (n - 1) + k1 - 1] * VELV(k - 2 + k1, j - 2 + j1, rnode - 2 + i1);
+                                    sumj = sumj + vi[4 * (m - 1) + j1 - 1] * sumk;
+                                }
+                                sumi = sumi + wi[i1 - 1] * sumj;
+                            }
+                            OUT(d1 * (k - 2) + n, d2 * (j - 1) + m) = sumi;
+                        }
+                }
+            }
+    } else {
+        if (rnode - 1 < 0 || rnode + 2 > rt + 1) bad = 1;
+        else
+            for (int i = 1; i <= rp - 1; ++i) {
+                const int conp = i == rp - 1 ? d2 + 1 : d2;
+                for (int k = 2; k <= rr - 2; ++k) {
+                    const int conr = k == rr - 2 ? d1 + 1 : d1;
+                    for (int l = 1; l <= conp; ++l)
+                        for (int n = 1; n <= conr; ++n) {
+                            double sumi = 0.0;
+                            for (int i1 = 1; i1 <= 4; ++i1) {
+                                double sumj = 0.0;
+                                for (int j1 = 1; j1 <= 4; ++j1) {
+                                    double sumk = 0.0;
+                                    for (int k1 = 1; k1 <= 4; ++k1)
+                                        sumk = sumk + ui[4 * (n - 1) + k1 - 1] * VELV(k - 2 + k1, rnode - 2 + j1, i - 2 + i1);
+                                    sumj = sumj + wi[j1 - 1] * sumk;
+                                }
+                                sumi = sumi + vi[4 * (l - 1) + i1 - 1] * sumj;
+                            }
+                            OUT(d1 * (k - 2) + n, d2 * (i - 1) + l) = sumi;
+                        }
+                }
+            }
+    }
+#undef OUT
+#undef VELV
+    free(vv); free(velv); free(ui); free(vi);
+    return bad ? -1 : 0;
 }

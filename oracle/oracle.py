@@ -76,6 +76,9 @@ def lib():
         L.or_srtimes.restype = C.c_int
         L.or_rpaths.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_int, dp, lp, ip, fp, C.c_long, ip, ip]
         L.or_rpaths.restype = C.c_long
+        L.or_slice.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, dp,
+                               C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_slice.restype = C.c_int
         L.or_octant_solve.argtypes = [ip, dp, dp, dp, C.c_double]
         L.or_octant_solve.restype = C.c_double
         _lib = L
@@ -233,3 +236,26 @@ def rpaths(g, veln: np.ndarray, ttn: np.ndarray, rcv: np.ndarray, cap: int = 0, 
 def octant_solve(ord3, t1, t2, h, slown) -> float:
     return lib().or_octant_solve(np.asarray(ord3, dtype=np.int32), np.asarray(t1, dtype=np.float64),
                                  np.asarray(t2, dtype=np.float64), np.asarray(h, dtype=np.float64), slown)
+
+
+def slice_velocity(g, vtrue, vref, kind, where, d1, d2):
+    """gmtslicet's velocity-perturbation slice (kind 0 depth at height
+    `where` km, 1 N-S at longitude `where` deg, 2 E-W at latitude `where`
+    deg; dicing d1, d2): the values in gmtslicet's output order, and
+    (n1, n2).  None when a tap falls outside the arrays."""
+    deg = np.array([g.rgoxd, g.dnxd, g.rgozd, g.dnzd], np.float64)
+    if kind == 0:
+        n = ((g.nnx - 1) * d1 + 1) * ((g.nnz - 1) * d2 + 1)
+    elif kind == 1:
+        n = ((g.nnr - 3) * d1 + 1) * ((g.nnx - 1) * d2 + 1)
+    else:
+        n = ((g.nnr - 3) * d1 + 1) * ((g.nnz - 1) * d2 + 1)
+    out = np.zeros(n)
+    n1, n2 = C.c_int(), C.c_int()
+    vr = None if vref is None else np.ascontiguousarray(vref, dtype=np.float64)
+    rc = lib().or_slice(C.byref(or_grid(g)), deg, np.ascontiguousarray(vtrue, dtype=np.float64),
+                        None if vr is None else vr.ctypes.data, kind, float(where), d1, d2, out,
+                        C.byref(n1), C.byref(n2))
+    if rc != 0:
+        return None
+    return out, (n1.value, n2.value)

@@ -16,6 +16,7 @@ INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "libfmm3d_hip.so")
 EXE = os.path.join(ROOT, "host", "fm3dt_gpu")
 INVERT_EXE = os.path.join(ROOT, "host", "fm3dt_invert")
+SLICE_EXE = os.path.join(ROOT, "host", "gmtslice_gpu")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
@@ -29,7 +30,7 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["fmm3d_solver.hip", "fmm3d_server.hip"]
+SOURCES = ["fmm3d_solver.hip", "fmm3d_server.hip", "fmm3d_slice.hip"]
 
 
 def build_hip(verbose: bool = False) -> str:
@@ -47,14 +48,14 @@ def build_hip(verbose: bool = False) -> str:
 
 def build_host() -> str:
     """Fortran hosts (ISO_C_BINDING onto libfmm3d_hip.so): the drop-in
-    fm3dt_gpu and the resident inversion loop fm3dt_invert, both on the
-    shared module file fm3dt_core.f90."""
+    fm3dt_gpu, the resident inversion loop fm3dt_invert and the model
+    slicer gmtslice_gpu, all on the shared module file fm3dt_core.f90."""
     core = os.path.join(HOST, "fm3dt_core.f90")
     if not os.path.exists(core) or not os.path.exists(FLANG):
         return ""
     moddir = os.path.join(HOST, "_mod")
     os.makedirs(moddir, exist_ok=True)
-    for main, exe in (("fm3dt_gpu.f90", EXE), ("fm3dt_invert.f90", INVERT_EXE)):
+    for main, exe in (("fm3dt_gpu.f90", EXE), ("fm3dt_invert.f90", INVERT_EXE), ("gmtslice_gpu.f90", SLICE_EXE)):
         cmd = [FLANG, "-O2", "-J", moddir, core, os.path.join(HOST, main), "-o", exe, "-L" + PKG, "-lfmm3d_hip",
                "-Wl,-rpath,$ORIGIN/../fmm3d", "-Wl,-rpath," + PKG]
         r = subprocess.run(cmd, capture_output=True, text=True)

@@ -29,7 +29,7 @@ EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model
            "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_result",
            "fmm3d_server_stats", "fmm3d_server_destroy", "fmm3d_server_rows",
            "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_receivers_noalloc", "fmm3d_frechet", "fmm3d_frechet_pinned", "fmm3d_get_field", "fmm3d_get_passes",
-           "fmm3d_get_stats", "fmm3d_set_option")
+           "fmm3d_get_stats", "fmm3d_set_option", "fmm3d_slice_velocity", "fmm3d_slice_last_error")
 
 
 class GridDescC(C.Structure):
@@ -112,11 +112,14 @@ def load(path: Optional[str] = None):
                                        C.POINTER(C.POINTER(C.c_int32)), C.POINTER(C.POINTER(C.c_float)),
                                        C.POINTER(C.c_int64)]
     L.fmm3d_get_field.argtypes = [vp, C.c_int, dp]
+    L.fmm3d_slice_velocity.argtypes = [C.POINTER(GridDescC), C.c_int, dp, vp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                       dp, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.fmm3d_slice_last_error.restype = C.c_char_p
     L.fmm3d_get_passes.argtypes = [vp, C.c_int, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
     L.fmm3d_get_stats.argtypes = [vp, C.POINTER(StatsC)]
     L.fmm3d_set_option.argtypes = [vp, C.c_char_p, C.c_double]
     for f in EXPORTS:
-        if f not in ("fmm3d_last_error",):
+        if f not in ("fmm3d_last_error", "fmm3d_slice_last_error"):
             getattr(L, f).restype = C.c_int
     _lib = L
     return L
@@ -412,3 +415,23 @@ class Server:
 
     def __exit__(self, *a):
         self.close()
+
+
+def slice_velocity(g: GridDesc, velv: np.ndarray, velv_ref: Optional[np.ndarray], kind: int, where: float,
+                   dice1: int, dice2: int, device: int = 0):
+    """fmm3d_slice_velocity: the values of one gmtslicet slice in its write
+    order, and (n1, n2)."""
+    L = load()
+    velv = np.ascontiguousarray(velv, dtype=np.float64)
+    ref = None if velv_ref is None else np.ascontiguousarray(velv_ref, dtype=np.float64)
+    n1, n2 = C.c_int(), C.c_int()
+    rc = L.fmm3d_slice_velocity(C.byref(grid_c(g)), device, velv, None if ref is None else ref.ctypes.data, kind,
+                                float(where), dice1, dice2, np.zeros(1), 0, C.byref(n1), C.byref(n2))
+    if rc != E_CAPACITY:
+        raise FmmError(rc, L.fmm3d_slice_last_error().decode())
+    out = np.zeros(n1.value * n2.value)
+    rc = L.fmm3d_slice_velocity(C.byref(grid_c(g)), device, velv, None if ref is None else ref.ctypes.data, kind,
+                                float(where), dice1, dice2, out, out.size, C.byref(n1), C.byref(n2))
+    if rc != OK:
+        raise FmmError(rc, L.fmm3d_slice_last_error().decode())
+    return out, (n1.value, n2.value)

@@ -58,6 +58,21 @@ MODULE fmm3d_c
          REAL(C_FLOAT) :: vals(*)
          INTEGER(C_INT64_T), VALUE :: cap
       END FUNCTION
+      INTEGER(C_INT) FUNCTION fmm3d_slice_velocity(g, device, velv, velv_ref, kind, where, dice1, dice2, &
+                                                  out, cap, n1, n2) BIND(C, NAME='fmm3d_slice_velocity')
+         IMPORT :: C_INT, C_PTR, C_DOUBLE, C_INT64_T, fmm3d_grid_desc
+         TYPE(fmm3d_grid_desc), INTENT(IN) :: g
+         INTEGER(C_INT), VALUE :: device, kind, dice1, dice2
+         REAL(C_DOUBLE), INTENT(IN) :: velv(*)
+         TYPE(C_PTR), VALUE :: velv_ref
+         REAL(C_DOUBLE), VALUE :: where
+         REAL(C_DOUBLE) :: out(*)
+         INTEGER(C_INT64_T), VALUE :: cap
+         INTEGER(C_INT) :: n1, n2
+      END FUNCTION
+      TYPE(C_PTR) FUNCTION fmm3d_slice_last_error() BIND(C, NAME='fmm3d_slice_last_error')
+         IMPORT :: C_PTR
+      END FUNCTION
       INTEGER(C_SIZE_T) FUNCTION c_strlen(s) BIND(C, NAME='strlen')
          IMPORT :: C_SIZE_T, C_PTR
          TYPE(C_PTR), VALUE :: s
