@@ -301,6 +301,27 @@ int fmm3d_slice_velocity(const fmm3d_grid_desc *grid, int device, const double *
                          int kind, double where, int dice1, int dice2, double *out, int64_t cap, int *n1, int *n2);
 const char *fmm3d_slice_last_error(void);
 
+/* The Frechet matrix G on the device and the sparse products of subinv's
+ * subspace step (subinv.f90:612-620 G^T d; 759-790 and 829-850 G a / Cd,
+ * then G^T of it).  subinv itself stays on the CPU; these are for an
+ * inversion loop that keeps G resident.  G: nrows rows (traveltimes) in
+ * CSR, row_ptr[0..nrows], 1-based columns (vertices, frechet.out's) below
+ * ncols and ascending within a row, float32 values.  The transpose is
+ * built on the device with the rows of each column ascending, as subinv
+ * builds it (subinv.f90:457-475).  Products sum in float32 entry by entry
+ * in subinv's order, so they are subinv's values:
+ *   fmm3d_gmat_gtx: y[ncols] = G^T x[nrows]
+ *   fmm3d_gmat_gx:  y[nrows] = (G x[ncols]) / cd[nrows] (cd may be NULL: 1)
+ * fmm3d_gmat_last_error(NULL) gives the message of the calling thread's
+ * last failed fmm3d_gmat_create. */
+typedef struct fmm3d_gmat fmm3d_gmat;
+int fmm3d_gmat_create(int device, int64_t nrows, int64_t ncols, const int64_t *row_ptr, const int32_t *cols,
+                      const float *vals, fmm3d_gmat **out);
+int fmm3d_gmat_gtx(fmm3d_gmat *g, const float *x, float *y);
+int fmm3d_gmat_gx(fmm3d_gmat *g, const float *x, const float *cd, float *y);
+const char *fmm3d_gmat_last_error(const fmm3d_gmat *g);
+int fmm3d_gmat_destroy(fmm3d_gmat *g);
+
 /* ---------------------------------------------------------------------------
  * Serving loop (no counterpart in the reference, which runs one source after
  * another per process, fm3dt_parallel.f90:6371).  A server owns one device

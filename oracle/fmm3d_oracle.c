@@ -22,6 +22,8 @@
  *   or_srtimes      trilinear receiver times, float32 sum    F:7040-7248
  *   or_rpaths       steepest-descent rays + Frechet rows     F:7250-7777
  *   or_slice        velocity-perturbation slices of gmtplot/gmtslicet.f90:4-540
+ *   or_gmat_gtx/gx  subinv's sparse products G^T x, (G x)/cd  subinv.f90:457-475,
+ *                   612-616, 759-767
  *
  * Layouts.  Fields are (nnr, nnx, nnz) with phi fastest -- the reference
  * array order ttn(iz,ix,ir) -- and carry no halo at the interface; veln is
@@ -1855,4 +1857,46 @@ int or_slice(const or_grid *g, const double deg[4], const double *vtrue, const d
 #undef VELV
     free(vv); free(velv); free(ui); free(vi);
     return bad ? -1 : 0;
+}
+
+/* ----------------------------------------- subinv's sparse products (float32) */
+
+/* y = G^T x as subinv computes it: the transpose built column by column
+ * with a per-row cursor (subinv.f90:457-475: for each column, every row
+ * whose next entry is in that column gives its entry, rows ascending),
+ * then per column a single-precision sum in that order (subinv.f90:612-616,
+ * 773-776).  G: CSR with 1-based columns.  Entries of a row that are not in
+ * ascending column order are never reached, as in subinv. */
+void or_gmat_gtx(long nrows, long ncols, const long *row_ptr, const int *cols, const float *vals,
+                 const float *x, float *y)
+{
+    long *stpv = malloc((nrows + 1) * sizeof(long));
+    for (long j = 0; j < nrows; ++j) stpv[j] = 1;
+    for (long i = 1; i <= ncols; ++i) {
+        float rsum1 = 0.0f;
+        for (long j = 1; j <= nrows; ++j) {
+            const long n = row_ptr[j] - row_ptr[j - 1];
+            if (stpv[j - 1] <= n) {
+                const long k = row_ptr[j - 1] + stpv[j - 1] - 1;
+                if (cols[k] == i) {
+                    rsum1 = rsum1 + vals[k] * x[j - 1];
+                    stpv[j - 1]++;
+                }
+            }
+        }
+        y[i - 1] = rsum1;
+    }
+    free(stpv);
+}
+
+/* y = (G x) / cd as subinv computes it: per row a single-precision sum in
+ * the row's order, divided by cd (subinv.f90:759-767); cd NULL: 1. */
+void or_gmat_gx(long nrows, const long *row_ptr, const int *cols, const float *vals, const float *x,
+                const float *cd, float *y)
+{
+    for (long j = 0; j < nrows; ++j) {
+        float rsum1 = 0.0f;
+        for (long k = row_ptr[j]; k < row_ptr[j + 1]; ++k) rsum1 = rsum1 + vals[k] * x[cols[k] - 1];
+        y[j] = cd ? rsum1 / cd[j] : rsum1;
+    }
 }

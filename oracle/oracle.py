@@ -79,6 +79,11 @@ def lib():
         L.or_slice.argtypes = [C.POINTER(OrGrid), dp, dp, C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int, dp,
                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.or_slice.restype = C.c_int
+        f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+        L.or_gmat_gtx.argtypes = [C.c_long, C.c_long, lp, ip, f32, f32, f32]
+        L.or_gmat_gtx.restype = None
+        L.or_gmat_gx.argtypes = [C.c_long, lp, ip, f32, f32, C.c_void_p, f32]
+        L.or_gmat_gx.restype = None
         L.or_octant_solve.argtypes = [ip, dp, dp, dp, C.c_double]
         L.or_octant_solve.restype = C.c_double
         _lib = L
@@ -259,3 +264,23 @@ def slice_velocity(g, vtrue, vref, kind, where, d1, d2):
     if rc != 0:
         return None
     return out, (n1.value, n2.value)
+
+
+def gmat_gtx(row_ptr, cols, vals, ncols, x):
+    """subinv's G^T x (float32, its transpose order)."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    y = np.zeros(ncols, np.float32)
+    lib().or_gmat_gtx(len(rp) - 1, ncols, rp, np.ascontiguousarray(cols, dtype=np.int32),
+                      np.ascontiguousarray(vals, dtype=np.float32), np.ascontiguousarray(x, dtype=np.float32), y)
+    return y
+
+
+def gmat_gx(row_ptr, cols, vals, x, cd=None):
+    """subinv's (G x) / cd (float32, row order)."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+    y = np.zeros(len(rp) - 1, np.float32)
+    c = None if cd is None else np.ascontiguousarray(cd, dtype=np.float32)
+    lib().or_gmat_gx(len(rp) - 1, rp, np.ascontiguousarray(cols, dtype=np.int32),
+                     np.ascontiguousarray(vals, dtype=np.float32), np.ascontiguousarray(x, dtype=np.float32),
+                     None if c is None else c.ctypes.data, y)
+    return y

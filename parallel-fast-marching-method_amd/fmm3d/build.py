@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-Wno-unused-result",
 ]
 
-SOURCES = ["fmm3d_solver.hip", "fmm3d_server.hip", "fmm3d_slice.hip"]
+SOURCES = ["fmm3d_solver.hip", "fmm3d_server.hip", "fmm3d_slice.hip", "fmm3d_gmat.hip"]
 
 
 def build_hip(verbose: bool = False) -> str:

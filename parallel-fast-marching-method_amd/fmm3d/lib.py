@@ -29,7 +29,8 @@ EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model
            "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_result",
            "fmm3d_server_stats", "fmm3d_server_destroy", "fmm3d_server_rows",
            "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_receivers_noalloc", "fmm3d_frechet", "fmm3d_frechet_pinned", "fmm3d_get_field", "fmm3d_get_passes",
-           "fmm3d_get_stats", "fmm3d_set_option", "fmm3d_slice_velocity", "fmm3d_slice_last_error")
+           "fmm3d_get_stats", "fmm3d_set_option", "fmm3d_slice_velocity", "fmm3d_slice_last_error",
+           "fmm3d_gmat_create", "fmm3d_gmat_gtx", "fmm3d_gmat_gx", "fmm3d_gmat_last_error", "fmm3d_gmat_destroy")
 
 
 class GridDescC(C.Structure):
@@ -115,11 +116,18 @@ def load(path: Optional[str] = None):
     L.fmm3d_slice_velocity.argtypes = [C.POINTER(GridDescC), C.c_int, dp, vp, C.c_int, C.c_double, C.c_int, C.c_int,
                                        dp, C.c_int64, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.fmm3d_slice_last_error.restype = C.c_char_p
+    fp32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    L.fmm3d_gmat_create.argtypes = [C.c_int, C.c_int64, C.c_int64, lp, ip, fp32, C.POINTER(vp)]
+    L.fmm3d_gmat_gtx.argtypes = [vp, fp32, fp32]
+    L.fmm3d_gmat_gx.argtypes = [vp, fp32, vp, fp32]
+    L.fmm3d_gmat_last_error.argtypes = [vp]
+    L.fmm3d_gmat_last_error.restype = C.c_char_p
+    L.fmm3d_gmat_destroy.argtypes = [vp]
     L.fmm3d_get_passes.argtypes = [vp, C.c_int, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
     L.fmm3d_get_stats.argtypes = [vp, C.POINTER(StatsC)]
     L.fmm3d_set_option.argtypes = [vp, C.c_char_p, C.c_double]
     for f in EXPORTS:
-        if f not in ("fmm3d_last_error", "fmm3d_slice_last_error"):
+        if f not in ("fmm3d_last_error", "fmm3d_slice_last_error", "fmm3d_gmat_last_error"):
             getattr(L, f).restype = C.c_int
     _lib = L
     return L
@@ -435,3 +443,46 @@ def slice_velocity(g: GridDesc, velv: np.ndarray, velv_ref: Optional[np.ndarray]
     if rc != OK:
         raise FmmError(rc, L.fmm3d_slice_last_error().decode())
     return out, (n1.value, n2.value)
+
+
+class FrechetMatrix:
+    """fmm3d_gmat_*: the Frechet matrix resident on the device, with
+    subinv's products G^T x and (G x) / cd."""
+
+    def __init__(self, row_ptr, cols, vals, ncols: int, device: int = 0):
+        self.L = load()
+        self.rp = np.ascontiguousarray(row_ptr, dtype=np.int64)
+        self.nrows, self.ncols = len(self.rp) - 1, int(ncols)
+        self._h = C.c_void_p()
+        rc = self.L.fmm3d_gmat_create(device, self.nrows, self.ncols, self.rp,
+                                      np.ascontiguousarray(cols, dtype=np.int32),
+                                      np.ascontiguousarray(vals, dtype=np.float32), C.byref(self._h))
+        if rc != OK:
+            raise FmmError(rc, self.L.fmm3d_gmat_last_error(None).decode())
+
+    def gtx(self, x) -> np.ndarray:
+        y = np.zeros(self.ncols, np.float32)
+        rc = self.L.fmm3d_gmat_gtx(self._h, np.ascontiguousarray(x, dtype=np.float32), y)
+        if rc != OK:
+            raise FmmError(rc, self.L.fmm3d_gmat_last_error(self._h).decode())
+        return y
+
+    def gx(self, x, cd=None) -> np.ndarray:
+        y = np.zeros(self.nrows, np.float32)
+        c = None if cd is None else np.ascontiguousarray(cd, dtype=np.float32)
+        rc = self.L.fmm3d_gmat_gx(self._h, np.ascontiguousarray(x, dtype=np.float32),
+                                  None if c is None else c.ctypes.data, y)
+        if rc != OK:
+            raise FmmError(rc, self.L.fmm3d_gmat_last_error(self._h).decode())
+        return y
+
+    def close(self):
+        if self._h:
+            self.L.fmm3d_gmat_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
