@@ -108,6 +108,23 @@ __global__ void k_gridder(Grid g, const double* __restrict__ velv, double* __res
     }
 }
 
+// Mean slowness of each layer (one block per layer): the vertical profile
+// of the float32 teleseismic reference.
+__global__ void __launch_bounds__(256) k_layer_mean(const double* __restrict__ slo, size_t NL, double* __restrict__ out)
+{
+    __shared__ double sm[256];
+    const double* p = slo + (size_t)blockIdx.x * NL;
+    double acc = 0.0;
+    for (size_t i = threadIdx.x; i < NL; i += 256) acc += p[i];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sm[0] / (double)NL;
+}
+
 // ------------------------------------------------------------ solver sweeps
 
 template <typename R> struct SweepArgs {
@@ -413,6 +430,60 @@ template <typename R> struct ColArgs {
                             // container (strides stay the container's), or null: the container's
     unsigned long long* lrng;   // alternating walks: per slot per patch, two entries (by sweep parity) of
                                 // (sweep << 32 | first changed layer << 16 | last changed layer), or null
+    const double* ref;      // float32 fields: per slot nnr + 2 reference values (A[ir], gx, gz), or null
+    double tfloor;          // slots with a reference: a change counts above tol * max(|value|, tfloor)
+    float qscale;           // slots with a reference: updated values are rounded to multiples of 1/qscale (0: not)
+};
+
+// Reference value of node (ir, ix, iz) from a slot's reference row
+// (A[0..nnr-1], gx, gz); 0 without one.
+__device__ __forceinline__ double ref_at(const double* __restrict__ rs, int nnr, size_t ir, size_t ix, size_t iz)
+{
+    return rs ? rs[ir] + rs[nnr] * (double)ix + rs[nnr + 1] * (double)iz : 0.0;
+}
+
+// An updated value stored against a reference, rounded to the nearest
+// multiple of 1/qs (qs a power of two; 0: as computed).  The reference's
+// steps are multiples of 2^-20 s, so with qs >= 2^20 a stored value plus a
+// step is exact while under 2^24/qs s in magnitude and the update's
+// neighbour comparisons agree from both sides; the fixed grid also bounds
+// how fine a change the relaxation resolves.  inf stays inf.
+template <typename R> __device__ __forceinline__ R ref_round(R v, R qs)
+{
+    return qs != R(0) ? rint(v * qs) * (R(1) / qs) : v;
+}
+
+// Steps of a slot's reference from node (ir, ix, iz) to its stencil
+// neighbours: a float32 neighbour value plus its step is the neighbour's
+// time relative to the centre node's reference.  Every step is a multiple
+// of 2^-20 s small enough to be exact in float32.  float64 fields carry no
+// reference and take no steps.
+template <typename R> struct RefSteps {
+    R up1 = R(0), up2 = R(0), dn1 = R(0), dn2 = R(0);   // layers ir-1, ir-2, ir+1, ir+2
+    R gx = R(0), gz = R(0);                             // one node along theta, phi
+    __device__ __forceinline__ void lateral(const double* rs, int nnr)
+    {
+        if (std::is_same<R, float>::value && rs) { gx = (R)rs[nnr]; gz = (R)rs[nnr + 1]; }
+    }
+    __device__ __forceinline__ void layer(const double* rs, int ir, int snr)
+    {
+        if (std::is_same<R, float>::value && rs) {
+            const double a0 = rs[ir];
+            up1 = ir >= 1 ? (R)(rs[ir - 1] - a0) : R(0);
+            up2 = ir >= 2 ? (R)(rs[ir - 2] - a0) : R(0);
+            dn1 = ir + 1 < snr ? (R)(rs[ir + 1] - a0) : R(0);
+            dn2 = ir + 2 < snr ? (R)(rs[ir + 2] - a0) : R(0);
+        }
+    }
+    // v1/v2 in side order r-, r+, x-, x+, z-, z+
+    __device__ __forceinline__ void apply(R v1[6], R v2[6]) const
+    {
+        if (std::is_same<R, float>::value) {
+            v1[0] += up1; v2[0] += up2; v1[1] += dn1; v2[1] += dn2;
+            v1[2] -= gx; v2[2] -= gx + gx; v1[3] += gx; v2[3] += gx + gx;
+            v1[4] -= gz; v2[4] -= gz + gz; v1[5] += gz; v2[5] += gz + gz;
+        }
+    }
 };
 
 // A layer-range entry: the sweep it belongs to and the layers lo..hi that changed.
@@ -639,6 +710,15 @@ k_column(ColArgs<R> a)
     const R inf = Lim<R>::inf();
     const int x0 = bx * PX, z0 = bz * PZ;
     const R hr = a.hr, ihr = R(1) / a.hr;
+    const double* rs = a.ref ? a.ref + (size_t)slot * (a.nnr + 2) : nullptr;
+    RefSteps<R> rst;
+    rst.lateral(rs, a.nnr);
+    // a change counts above tol times the larger magnitude; with a reference
+    // the stored values sit near zero and carry rounding at the scale of the
+    // reference's steps, so the magnitude is taken as at least tfloor
+    const bool hasref = rs && rs[0] != 0.0;
+    const double tfloor = hasref ? a.tfloor : 0.0;
+    const R rq = hasref ? (R)a.qscale : R(0);
     bool anyv = false;
     unsigned nupd = 0, nvis = 0;
     R orig[S];
@@ -721,6 +801,7 @@ k_column(ColArgs<R> a)
             sihz[e] = R(1) / h;
         }
         const R hxl = hxs[ir], ihx = R(1) / hxl;
+        rst.layer(rs, ir, snr);
         __syncthreads();
         // the passes stop after one that changes nothing: the layer is then
         // at a fixpoint of the update and later passes would not move it
@@ -758,9 +839,10 @@ k_column(ColArgs<R> a)
                     v1[3] = cur[own + LZ]; v2[3] = cur[own + 2 * LZ];
                     v1[4] = cur[own - 1]; v2[4] = cur[own - 2];
                     v1[5] = cur[own + 1]; v2[5] = cur[own + 2];
+                    rst.apply(v1, v2);
                     const R h[3] = { hr, hxl, shz[lx] };
                     const R ih[3] = { ihr, ihx, sihz[lx] };
-                    const R nv = col_update<R, FAST, SHIFT>(a.fom, tc, v1, v2, in1, in2, h, ih, slown);
+                    const R nv = ref_round(col_update<R, FAST, SHIFT>(a.fom, tc, v1, v2, in1, in2, h, ih, slown), rq);
                     // every lane has read its stencil before any lane writes (one wavefront)
                     __builtin_amdgcn_wave_barrier();
                     if (nv != tc) { cur[own] = nv; moved = true; }
@@ -784,7 +866,8 @@ k_column(ColArgs<R> a)
             if (nv != tc) {
                 __hip_atomic_store(T + lay + (size_t)(x0 + lx) * a.nnz + z0 + lz, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const bool fin_n = nv < inf, fin_t = tc < inf;
-                if (fin_n != fin_t || !(fabs((double)nv - (double)tc) <= a.tol * fmax(fabs((double)nv), fabs((double)tc))))
+                if (fin_n != fin_t ||
+                    !(fabs((double)nv - (double)tc) <= a.tol * fmax(fmax(fabs((double)nv), fabs((double)tc)), tfloor)))
                     lay_chg = true;
             }
             b2[e] = b1[e];
@@ -856,6 +939,10 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
         if (ix >= snx || iz >= snz) continue;
         const R* T = a.T + (size_t)slot * N;
         const uint32_t* fb = a.fixbits + (size_t)slot * NW;
+        const double* rs = a.ref ? a.ref + (size_t)slot * (a.nnr + 2) : nullptr;
+        RefSteps<R> rst;
+        rst.lateral(rs, a.nnr);
+        const R qs = (rs && rs[0] != 0.0) ? (R)a.qscale : R(0);
         bool moved = false;
         // window w[0..4] = layers ir-2 .. ir+2 of this column
         R w[5];
@@ -882,10 +969,12 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
                 v1[3] = in1[3] ? T[c + a.nnz] : inf; v2[3] = in2[3] ? T[c + 2 * (size_t)a.nnz] : inf;
                 v1[4] = in1[4] ? T[c - 1] : inf; v2[4] = in2[4] ? T[c - 2] : inf;
                 v1[5] = in1[5] ? T[c + 1] : inf; v2[5] = in2[5] ? T[c + 2] : inf;
+                rst.layer(rs, ir, snr);
+                rst.apply(v1, v2);
                 const R hxl = a.hx[slot * a.geo_hx + ir], hzl = a.hz[slot * a.geo_hz + (size_t)ir * a.nnx + ix];
                 const R h[3] = { hr, hxl, hzl };
                 const R ih[3] = { ihr, R(1) / hxl, R(1) / hzl };
-                const R nv = col_update<R, FAST, SHIFT>(a.fom, w[2], v1, v2, in1, in2, h, ih, a.slo[slot * a.geo_n + c]);
+                const R nv = ref_round(col_update<R, FAST, SHIFT>(a.fom, w[2], v1, v2, in1, in2, h, ih, a.slo[slot * a.geo_n + c]), qs);
                 if (nv != w[2]) moved = true;
             }
             w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
@@ -1210,16 +1299,25 @@ k_plane(ColArgs<R> a)
 
 // ------------------------------------------------------------ receivers
 
+// Time at node (ir, ix, iz), index i, of a slot's field: float32 fields add
+// the slot's offset and reference (rs, or null) to the stored value.
 template <typename R>
-__device__ __forceinline__ double field_at(const R* T, double toff, size_t i) { return toff + (double)T[i]; }
+__device__ __forceinline__ double field_at(const R* T, double toff, const double* rs, int nnr, int ir, int ix, int iz,
+                                           size_t i)
+{
+    return toff + ref_at(rs, nnr, ir, ix, iz) + (double)T[i];
+}
 template <>
-__device__ __forceinline__ double field_at<double>(const double* T, double, size_t i) { return T[i]; }
+__device__ __forceinline__ double field_at<double>(const double* T, double, const double*, int, int, int, int, size_t i)
+{
+    return T[i];
+}
 
 // trilinear receiver time with a float32 accumulator (fm3dt_parallel.f90:7063-7106);
 // NaN outside the grid
 template <typename R>
-__device__ __forceinline__ float receiver_time(const Grid& g, const R* __restrict__ F, double to, double rr, double rx,
-                                               double rz)
+__device__ __forceinline__ float receiver_time(const Grid& g, const R* __restrict__ F, double to, const double* rs,
+                                               double rr, double rx, double rz)
 {
     int irr = (int)floor((g.rgor - rr) / g.dnr) + 1;
     int irx = (int)floor((rx - g.rgox) / g.dnx) + 1;
@@ -1236,34 +1334,38 @@ __device__ __forceinline__ float receiver_time(const Grid& g, const R* __restric
             for (int l = 0; l < 2; ++l) {
                 double w = (1.0 - fabs((l * g.dnz - drz) / g.dnz)) * (1.0 - fabs((k * g.dnx - drx) / g.dnx));
                 w = w * (1.0 - fabs((j * g.dnr - drr) / g.dnr));
-                size_t idx = ((size_t)(irr - 1 + j) * g.nnx + (irx - 1 + k)) * g.nnz + (irz - 1 + l);
-                acc = (float)((double)acc + field_at<R>(F, to, idx) * w);
+                const int nr_ = irr - 1 + j, nx_ = irx - 1 + k, nz_ = irz - 1 + l;
+                const size_t idx = ((size_t)nr_ * g.nnx + nx_) * g.nnz + nz_;
+                acc = (float)((double)acc + field_at<R>(F, to, rs, g.nnr, nr_, nx_, nz_, idx) * w);
             }
     return acc;
 }
 
 template <typename R>
-__global__ void k_receivers(Grid g, const R* __restrict__ T, const double* __restrict__ toff, int nslots,
-                            int nrc, const double* __restrict__ rcv, float* __restrict__ out)
+__global__ void k_receivers(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const double* __restrict__ ref,
+                            int nslots, int nrc, const double* __restrict__ rcv, float* __restrict__ out)
 {
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= nslots * nrc) return;
     int s = gid / nrc, i = gid % nrc;
     const size_t N = (size_t)g.nnr * g.nnx * g.nnz;
-    out[gid] = receiver_time<R>(g, T + (size_t)s * N, toff[s], rcv[3 * i], rcv[3 * i + 1], rcv[3 * i + 2]);
+    const double* rs = ref ? ref + (size_t)s * (g.nnr + 2) : nullptr;
+    out[gid] = receiver_time<R>(g, T + (size_t)s * N, toff[s], rs, rcv[3 * i], rcv[3 * i + 1], rcv[3 * i + 2]);
 }
 
 // Receiver times of many (slot, receiver) items at once: item i reads slot
 // islot[i] at receiver rcv[3i..3i+2].
 template <typename R>
-__global__ void k_receivers_items(Grid g, const R* __restrict__ T, const double* __restrict__ toff, int n,
-                                  const int* __restrict__ islot, const double* __restrict__ rcv, float* __restrict__ out)
+__global__ void k_receivers_items(Grid g, const R* __restrict__ T, const double* __restrict__ toff,
+                                  const double* __restrict__ ref, int n, const int* __restrict__ islot,
+                                  const double* __restrict__ rcv, float* __restrict__ out)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = islot[i];
     const size_t N = (size_t)g.nnr * g.nnx * g.nnz;
-    out[i] = receiver_time<R>(g, T + (size_t)s * N, toff[s], rcv[3 * i], rcv[3 * i + 1], rcv[3 * i + 2]);
+    const double* rs = ref ? ref + (size_t)s * (g.nnr + 2) : nullptr;
+    out[i] = receiver_time<R>(g, T + (size_t)s * N, toff[s], rs, rcv[3 * i], rcv[3 * i + 1], rcv[3 * i + 2]);
 }
 
 // ------------------------------------------------------------ rays + Frechet
@@ -1324,8 +1426,8 @@ struct RayPaths {
 // (k_ray_frechet then spreads each sample's 64 taps over 64 lanes).
 template <typename R>
 __global__ void __launch_bounds__(64)
-k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const double* __restrict__ veln,
-            int nrc, const double* __restrict__ rcv, int ray0, int nrays, RayPaths P,
+k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const double* __restrict__ ref,
+            const double* __restrict__ veln, int nrc, const double* __restrict__ rcv, int ray0, int nrays, RayPaths P,
             const double* __restrict__ srcpos)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1336,8 +1438,9 @@ k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, co
     const size_t N = (size_t)g.nnr * g.nnx * g.nnz;
     const R* F = T + (size_t)s * N;
     const double to = toff[s];
+    const double* rs = ref ? ref + (size_t)s * (g.nnr + 2) : nullptr;
     auto TT = [&](int r, int x, int z) -> double {   // 1-based node access
-        return field_at<R>(F, to, ((size_t)(r - 1) * g.nnx + (x - 1)) * g.nnz + (z - 1));
+        return field_at<R>(F, to, rs, g.nnr, r - 1, x - 1, z - 1, ((size_t)(r - 1) * g.nnx + (x - 1)) * g.nnz + (z - 1));
     };
     RaySample* out = P.rec + (int64_t)q * P.cap;
     int64_t ns = 0;
@@ -1675,6 +1778,10 @@ struct Slots {
     int* lastchg = nullptr;
     int* on = nullptr;
     double* toff = nullptr;
+    // float32 fields: per slot nnr + 2 values -- the vertical reference
+    // A[0..nnr-1] and the lateral steps gx, gz -- so that a node (ir, ix, iz)
+    // stores T - toff - (A[ir] + gx*ix + gz*iz); all zero: a plain offset
+    double* ref = nullptr;
     int* valchg = nullptr;    // a sweep changed a value
     int pcap = 0;             // slots P holds (allocated by the brick strategy only)
     int* prog = nullptr;      // column walk: per slot per patch layer progress
@@ -1737,6 +1844,10 @@ struct fmm3d_ctx {
     // on 50 plane waves at 201^3)
     int verify_patches = -1;
     int autotune = 0;              // time candidate patch shapes / pass caps at first use
+    int tele_ref = 1;              // float32 teleseismic fields relative to a 1-D plane-wave reference
+    double tol_floor = 64.0;       // those fields: a change counts above tol32 * max(|value|, tol_floor)
+    int ref_qbits = 22;            // those fields: updates rounded to multiples of 2^-ref_qbits s (0: not)
+    std::vector<double> layer_slo; // mean slowness of each layer of the current model
     std::vector<std::array<int, 8>> tuned;   // key (prec, order, walk, batch class) -> (px, pz, passes)
     int* d_iter = nullptr;         // device sweep counter (graph replays)
     hipGraph_t graph = nullptr;
@@ -1769,6 +1880,7 @@ struct fmm3d_ctx {
     size_t st_offs_n = 0, st_n = 0;
     double* d_velv = nullptr;      // the vertex model on the device (gridder input)
     size_t d_velv_n = 0;
+    double* d_layer = nullptr;     // per-layer mean slowness (nnr)
     double* d_gather = nullptr;    // refined contexts: the coarse nodes of one block
     size_t d_gather_n = 0;
     float* d_warm = nullptr;       // refined contexts: float32 fields of a warm start
@@ -2026,6 +2138,12 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     if (sp) c->spin_limit = std::max(0, atoi(sp));
     const char* at = getenv("FMM3D_AUTOTUNE");
     if (at) c->autotune = atoi(at) != 0;
+    const char* trf = getenv("FMM3D_TELE_REF");
+    if (trf) c->tele_ref = atoi(trf) != 0;
+    const char* tfl = getenv("FMM3D_TOL_FLOOR");
+    if (tfl) c->tol_floor = std::max(0.0, atof(tfl));
+    const char* rqb = getenv("FMM3D_REF_QBITS");
+    if (rqb) c->ref_qbits = std::min(30, std::max(0, atoi(rqb)));
     const char* gv = getenv("FMM3D_GRAPH");
     if (gv) c->use_graph = atoi(gv) != 0;
     const char* vf = getenv("FMM3D_VERIFY");
@@ -2056,7 +2174,7 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
 static void free_slots(Slots& s)
 {
     (void)hipFree(s.T); (void)hipFree(s.P); (void)hipFree(s.fix); (void)hipFree(s.stamp);
-    (void)hipFree(s.lastchg); (void)hipFree(s.on); (void)hipFree(s.toff);
+    (void)hipFree(s.lastchg); (void)hipFree(s.on); (void)hipFree(s.toff); (void)hipFree(s.ref);
     (void)hipFree(s.valchg); (void)hipFree(s.prog); (void)hipFree(s.flow); (void)hipFree(s.srcpos); (void)hipFree(s.walkalt);
     (void)hipFree(s.pend); (void)hipFree(s.slotpend); (void)hipFree(s.born); (void)hipFree(s.lrng);
     s = Slots();
@@ -2080,6 +2198,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->d_iter);
     (void)hipFree(c->d_faces);
     (void)hipFree(c->st_offs); (void)hipFree(c->st_node); (void)hipFree(c->st_tv); (void)hipFree(c->d_velv); (void)hipFree(c->d_bad);
+    (void)hipFree(c->d_layer);
     (void)hipFree(c->d_sides); (void)hipFree(c->d_side_of);
     (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv); (void)hipFree(c->eng_faces);
     (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out); (void)hipFree(c->eng_islot);
@@ -2129,6 +2248,14 @@ extern "C" int fmm3d_set_model_vtx(fmm3d_ctx* c, const double* velv)
         k_gridder<<<blocks, 256, 0, c->stream>>>(g, d, c->veln, c->slo64, c->slo32, g.nnx, g.nnz);
         e = hipGetLastError();
     }
+    // the per-layer mean slowness
+    if (e == hipSuccess && !c->d_layer) e = dalloc(&c->d_layer, (size_t)g.nnr * sizeof(double));
+    if (e == hipSuccess) {
+        k_layer_mean<<<g.nnr, 256, 0, c->stream>>>(c->slo64, (size_t)g.nnx * g.nnz, c->d_layer);
+        e = hipGetLastError();
+        c->layer_slo.assign(g.nnr, 0.0);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->layer_slo.data(), c->d_layer, g.nnr * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "gridder: %s", hipGetErrorString(e));
     c->have_model = true;
@@ -2174,6 +2301,7 @@ static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
     HIPCHK(c, dalloc(&s.lastchg, nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.on, nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.toff, nsrc * sizeof(double)));
+    HIPCHK(c, dalloc(&s.ref, (size_t)nsrc * (c->g.nnr + 2) * sizeof(double)));
     HIPCHK(c, dalloc(&s.valchg, nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.prog, (size_t)nsrc * nbmax * sizeof(int)));
     HIPCHK(c, dalloc(&s.flow, (size_t)nsrc * 2 * sizeof(int)));
@@ -2195,7 +2323,7 @@ static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
 // a positive boundary time hold it and are fixed; every other node unknown.
 template <typename R>
 __global__ void k_init_tele(int nnr, int nnx, int nnz, int nslots, const double* __restrict__ faces, R* __restrict__ T,
-                            const double* __restrict__ toff)
+                            const double* __restrict__ toff, const double* __restrict__ ref)
 {
     // one slot per grid row; the bottom layer is the index range from base
     const size_t NL = (size_t)nnx * nnz, N = NL * nnr, base = N - NL;
@@ -2203,11 +2331,13 @@ __global__ void k_init_tele(int nnr, int nnx, int nnz, int nslots, const double*
         R* Ts = T + s * N;
         const double* fs = faces + s * NL;
         const double off = toff[s];
+        const double* rs = ref ? ref + s * (nnr + 2) : nullptr;
         for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
             R v = Lim<R>::inf();
             if (c >= base) {
                 const double fv = fs[c - base];
-                if (fv > 0.0) v = std::is_same<R, double>::value ? (R)fv : (R)(fv - off);
+                const size_t ix = (c - base) / nnz, iz = (c - base) % nnz;
+                if (fv > 0.0) v = std::is_same<R, double>::value ? (R)fv : (R)(fv - off - ref_at(rs, nnr, nnr - 1, ix, iz));
             }
             Ts[c] = v;
         }
@@ -2280,13 +2410,15 @@ __global__ void k_fixbits_tele(int nnr, int nnx, int nnz, int nslots, const doub
 // Side-face injection after the bottom face: values > 0 on a slot's theta
 // face (ix = 0 or nnx-1, every ir and iz; (nnr, nnz) phi fastest) and phi
 // face (iz = 0 or nnz-1, every ir and ix; (nnr, nnx) theta fastest) become
-// fixed nodes, like the bottom face's (float32: minus the slot's offset).  A
+// fixed nodes, like the bottom face's (float32: minus the slot's offset and
+// reference).  A
 // side of -1 injects nothing.  Bits of one word come from several threads,
 // hence atomicOr.
 template <typename R>
 __global__ void k_init_sides(int nnr, int nnx, int nnz, int nslots, const double* __restrict__ xf,
                              const double* __restrict__ zf, const int* __restrict__ side,
-                             const double* __restrict__ toff, R* __restrict__ T, uint32_t* __restrict__ fix)
+                             const double* __restrict__ toff, const double* __restrict__ ref, R* __restrict__ T,
+                             uint32_t* __restrict__ fix)
 {
     const size_t N = (size_t)nnr * nnx * nnz, NW = (N + 31) / 32;
     const size_t XF = (size_t)nnr * nnz, ZF = (size_t)nnr * nnx, per = XF + ZF;
@@ -2308,7 +2440,8 @@ __global__ void k_init_sides(int nnr, int nnx, int nnz, int nslots, const double
         }
         if (!(v > 0.0)) continue;
         const size_t c = (ir * nnx + ix) * nnz + iz;
-        T[s * N + c] = std::is_same<R, double>::value ? (R)v : (R)(v - toff[s]);
+        const double rv = ref_at(ref ? ref + s * (nnr + 2) : nullptr, nnr, ir, ix, iz);
+        T[s * N + c] = std::is_same<R, double>::value ? (R)v : (R)(v - toff[s] - rv);
         atomicOr(fix + s * NW + (c >> 5), 1u << (c & 31));
     }
 }
@@ -2664,6 +2797,9 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     // changes leave sub-tolerance drift behind (up to 2.4 ms at receivers
     // of a 257^3 coarse solve)
     a.lrng = (c->point_window && (std::is_same<R, double>::value || c->geo_cap > 0)) ? s.lrng : nullptr;
+    a.ref = std::is_same<R, float>::value ? s.ref : nullptr;
+    a.tfloor = c->tol_floor;
+    a.qscale = c->ref_qbits > 0 ? std::ldexp(1.0f, c->ref_qbits) : 0.0f;
     a.geo_n = a.geo_hx = a.geo_hz = 0;
     a.sdims = nullptr;
     if (c->geo_cap > 0) {
@@ -2974,10 +3110,12 @@ static void flow_code(double gx, double gz, int out[2])
 // least-squares plane through the bottom face's known times (+1 when the
 // times grow with the index, the wave then travels that way).  Only the
 // order in which the relaxation visits patches depends on this.
-static void lateral_flow(const Grid& g, const double* face, int out[2])
+// Least-squares plane t = c0 + gx*ix + gz*iz through a bottom face's known
+// (positive) times, fitted on a sub-lattice of at most 64 x 64 nodes; false
+// when fewer than three nodes or a degenerate set is known.
+static bool face_plane(const Grid& g, const double* face, double* pgx, double* pgz, double* pc0)
 {
     const int nx = g.nnx, nz = g.nnz;
-    // only the signs are used: a sub-lattice of at most 64 x 64 nodes fits the plane
     const int sx_ = std::max(1, nx / 64), sz_ = std::max(1, nz / 64);
     double n = 0, sx = 0, sz = 0, sxx = 0, szz = 0, sxz = 0, st = 0, sxt = 0, szt = 0;
     for (int ix = 0; ix < nx; ix += sx_)
@@ -2987,14 +3125,68 @@ static void lateral_flow(const Grid& g, const double* face, int out[2])
             n += 1; sx += ix; sz += iz; sxx += (double)ix * ix; szz += (double)iz * iz; sxz += (double)ix * iz;
             st += v; sxt += ix * v; szt += iz * v;
         }
-    double gx = 0.0, gz = 0.0;
-    if (n >= 3) {
-        const double cxx = sxx - sx * sx / n, czz = szz - sz * sz / n, cxz = sxz - sx * sz / n;
-        const double cxt = sxt - sx * st / n, czt = szt - sz * st / n;
-        const double det = cxx * czz - cxz * cxz;
-        if (det > 0.0) { gx = (cxt * czz - czt * cxz) / det; gz = (czt * cxx - cxt * cxz) / det; }
-    }
+    *pgx = *pgz = 0.0;
+    *pc0 = n > 0 ? st / n : 0.0;
+    if (n < 3) return false;
+    const double cxx = sxx - sx * sx / n, czz = szz - sz * sz / n, cxz = sxz - sx * sz / n;
+    const double cxt = sxt - sx * st / n, czt = szt - sz * st / n;
+    const double det = cxx * czz - cxz * cxz;
+    if (!(det > 0.0)) return false;
+    *pgx = (cxt * czz - czt * cxz) / det;
+    *pgz = (czt * cxx - cxt * cxz) / det;
+    *pc0 = (st - *pgx * sx - *pgz * sz) / n;
+    return true;
+}
+
+static void lateral_flow(const Grid& g, const double* face, int out[2])
+{
+    double gx, gz, c0;
+    face_plane(g, face, &gx, &gz, &c0);
     flow_code(gx, gz, out);
+}
+
+// The reference a float32 teleseismic field is stored against: a plane wave
+// through the bottom face's fitted plane in the model's mean layer profile,
+// A(ir) + gx*ix + gz*iz above the offset c0, where A climbs from 0 at the
+// bottom layer by the vertical slowness sqrt(s(ir)^2 - px^2 - pz^2) of the
+// layer-mean slowness s and the plane's horizontal slownesses (gx over the
+// theta spacing, gz over the phi spacing of the middle column), trapezoid
+// rule over dnr.  Each step and gx, gz are multiples of 2^-20 s, so every
+// difference of the reference between two nodes of a stencil is exact in
+// float32 and A is exact in float64 (steps under 4 s, |gx|, |gz| under 4 s).
+// Writes ref[0..nnr+1] = A, gx, gz and *off = c0; false (ref zero, *off
+// untouched) without a fitted plane.
+static bool tele_reference(const fmm3d_ctx* c, const double* face, double* ref, double* off)
+{
+    const Grid& g = c->g;
+    std::fill(ref, ref + g.nnr + 2, 0.0);
+    if ((int)c->layer_slo.size() != g.nnr) return false;
+    double gx, gz, c0;
+    if (!face_plane(g, face, &gx, &gz, &c0)) return false;
+    auto quant = [](double v) { return std::ldexp(std::nearbyint(std::ldexp(v, 20)), -20); };
+    gx = quant(gx);
+    gz = quant(gz);
+    if (!(fabs(gx) < 4.0 && fabs(gz) < 4.0)) return false;
+    const double sth = sin(g.rgox + (g.nnx / 2) * g.dnx);
+    auto qv = [&](int ir) {
+        const double ri = g.rgor - ir * g.dnr + g.earth;
+        const double px = gx / (ri * g.dnx), pz = gz / (ri * sth * g.dnz), s = c->layer_slo[ir];
+        return sqrt(std::max(s * s - px * px - pz * pz, 0.0));
+    };
+    double q1 = qv(g.nnr - 1);
+    for (int ir = g.nnr - 2; ir >= 0; --ir) {
+        const double q0 = qv(ir), step = quant(0.5 * g.dnr * (q0 + q1));
+        if (!(step < 4.0)) {   // a stencil's two-layer difference would not be exact in float32
+            std::fill(ref, ref + g.nnr + 2, 0.0);
+            return false;
+        }
+        ref[ir] = ref[ir + 1] + step;
+        q1 = q0;
+    }
+    ref[g.nnr] = gx;
+    ref[g.nnr + 1] = gz;
+    *off = c0;
+    return true;
 }
 
 // Argument checks of teleseismic face arrays.  side[2s], side[2s+1]: -1
@@ -3058,18 +3250,32 @@ static int tele_upload(fmm3d_ctx* c, const char* fn, int nsrc, const double* fac
     k_tele_offsets<<<nsrc, 1024, 0, c->stream>>>(F, XF, ZF, dfaces, dx, dz, xf ? c->d_side_of : nullptr, s.toff, c->d_bad);
     int bad = nsrc;
     HIPCHK(c, hipMemcpyAsync(&bad, c->d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    // float32 fields on the column walk (the kernels that apply the
+    // reference) are stored against each slot's plane-wave reference
+    const bool useref = precision == FMM3D_PREC_FP32 && c->tele_ref && (c->strategy == 1 || (c->strategy == 0 && c->walk_alt));
+    const size_t RS = (size_t)g.nnr + 2;
+    std::vector<double> ref((size_t)nsrc * RS, 0.0), toff;
+    if (useref) {
+        toff.resize(nsrc);
+        HIPCHK(c, hipMemcpyAsync(toff.data(), s.toff, nsrc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (bad < nsrc) return fail(c, FMM3D_E_ARG, "%s: NaN boundary time (source %d)", fn, bad);
+    if (useref) {
+        for (int q = 0; q < nsrc; ++q) tele_reference(c, faces + (size_t)q * F, &ref[(size_t)q * RS], &toff[q]);
+        HIPCHK(c, hipMemcpyAsync(s.toff, toff.data(), nsrc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(s.ref, ref.data(), ref.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if (precision == FMM3D_PREC_FP64)
-        k_init_tele<double><<<dim3(256, std::min(nsrc, 65535)), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (double*)s.T, s.toff);
+        k_init_tele<double><<<dim3(256, std::min(nsrc, 65535)), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (double*)s.T, s.toff, nullptr);
     else
-        k_init_tele<float><<<dim3(256, std::min(nsrc, 65535)), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff);
+        k_init_tele<float><<<dim3(256, std::min(nsrc, 65535)), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff, s.ref);
     k_fixbits_tele<<<dim3(64, std::min(nsrc, 65535)), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, s.fix);
     if (xf) {
         if (precision == FMM3D_PREC_FP64)
-            k_init_sides<double><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dx, dz, c->d_side_of, s.toff, (double*)s.T, s.fix);
+            k_init_sides<double><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dx, dz, c->d_side_of, s.toff, nullptr, (double*)s.T, s.fix);
         else
-            k_init_sides<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dx, dz, c->d_side_of, s.toff, (float*)s.T, s.fix);
+            k_init_sides<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dx, dz, c->d_side_of, s.toff, s.ref, (float*)s.T, s.fix);
     }
     std::vector<int> flow((size_t)nsrc * 2);
     for (int q = 0; q < nsrc; ++q) lateral_flow(g, faces + (size_t)q * F, &flow[2 * q]);
@@ -3285,7 +3491,9 @@ static int fixed_upload(fmm3d_ctx* c, const char* fn, int slot0, int nsrc, const
     if (e == hipSuccess) e = hipMemcpyAsync(toff0, toff.data(), nsrc * sizeof(double), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s.flow + 2 * slot0, flow.data(), flow.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        const size_t n = (size_t)nsrc * N, nw = (size_t)nsrc * NW;
+        const size_t n = (size_t)nsrc * N, nw = (size_t)nsrc * NW, RS = (size_t)c->g.nnr + 2;
+        // fixed sets keep the plain offset (no reference)
+        k_fill_real<double><<<(int)std::min<size_t>(4096, (nsrc * RS + 255) / 256), 256, 0, c->stream>>>(s.ref + (size_t)slot0 * RS, nsrc * RS, 0.0);
         if (precision == FMM3D_PREC_FP64) k_fill_real<double><<<4096, 256, 0, c->stream>>>((double*)T0, n, Lim<double>::inf());
         else k_fill_real<float><<<4096, 256, 0, c->stream>>>((float*)T0, n, Lim<float>::inf());
         k_fill_u32<<<4096, 256, 0, c->stream>>>(fix0, nw);
@@ -3820,9 +4028,17 @@ extern "C" int fmm3d_get_field(fmm3d_ctx* c, int src, double* ttn)
         HIPCHK(c, hipMemcpy(d.data(), (double*)c->sl.T + (size_t)src * N, N * sizeof(double), hipMemcpyDeviceToHost));
     } else {
         std::vector<float> f(N);
+        std::vector<double> rs((size_t)g.nnr + 2);
         HIPCHK(c, hipMemcpy(f.data(), (float*)c->sl.T + (size_t)src * N, N * sizeof(float), hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(&to, c->sl.toff + src, sizeof(double), hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < N; ++i) d[i] = to + (double)f[i];
+        HIPCHK(c, hipMemcpy(rs.data(), c->sl.ref + (size_t)src * rs.size(), rs.size() * sizeof(double), hipMemcpyDeviceToHost));
+        // offset + reference + stored value, as the device readers form it
+        const double gx = rs[g.nnr], gz = rs[g.nnr + 1];
+        size_t i = 0;
+        for (int ir = 0; ir < g.nnr; ++ir)
+            for (int ix = 0; ix < g.nnx; ++ix)
+                for (int iz = 0; iz < g.nnz; ++iz, ++i)
+                    d[i] = to + (rs[ir] + gx * (double)ix + gz * (double)iz) + (double)f[i];
     }
     // same phi-fastest order as the host; unknown nodes come back as 0 (the reference's marker)
     for (size_t i = 0; i < N; ++i) ttn[i] = isfinite(d[i]) ? d[i] : 0.0;
@@ -3843,9 +4059,9 @@ extern "C" int fmm3d_receivers(fmm3d_ctx* c, int nrc, const double* rcv, float* 
     HIPCHK(c, hipMemcpyAsync(drcv, rcv, (size_t)nrc * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     int total = ns * nrc;
     if (c->sl.prec == FMM3D_PREC_FP64)
-        k_receivers<double><<<(total + 255) / 256, 256, 0, c->stream>>>(c->g, (const double*)c->sl.T, c->sl.toff, ns, nrc, drcv, dout);
+        k_receivers<double><<<(total + 255) / 256, 256, 0, c->stream>>>(c->g, (const double*)c->sl.T, c->sl.toff, nullptr, ns, nrc, drcv, dout);
     else
-        k_receivers<float><<<(total + 255) / 256, 256, 0, c->stream>>>(c->g, (const float*)c->sl.T, c->sl.toff, ns, nrc, drcv, dout);
+        k_receivers<float><<<(total + 255) / 256, 256, 0, c->stream>>>(c->g, (const float*)c->sl.T, c->sl.toff, c->sl.ref, ns, nrc, drcv, dout);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(t_out, dout, (size_t)total * sizeof(float), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -3876,9 +4092,9 @@ extern "C" int fmm3d_receivers_noalloc(fmm3d_ctx* c, int nrc, const double* rcv,
     const int total = ns * nrc;
     if (e == hipSuccess) {
         if (c->sl.prec == FMM3D_PREC_FP64)
-            k_receivers<double><<<(total + 255) / 256, 256, 0, c->stream>>>(c->g, (const double*)c->sl.T, c->sl.toff, ns, nrc, drcv, dout);
+            k_receivers<double><<<(total + 255) / 256, 256, 0, c->stream>>>(c->g, (const double*)c->sl.T, c->sl.toff, nullptr, ns, nrc, drcv, dout);
         else
-            k_receivers<float><<<(total + 255) / 256, 256, 0, c->stream>>>(c->g, (const float*)c->sl.T, c->sl.toff, ns, nrc, drcv, dout);
+            k_receivers<float><<<(total + 255) / 256, 256, 0, c->stream>>>(c->g, (const float*)c->sl.T, c->sl.toff, c->sl.ref, ns, nrc, drcv, dout);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(t_out, dout, (size_t)total * sizeof(float), hipMemcpyDeviceToHost, c->stream);
@@ -3966,9 +4182,9 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
         RayPaths pp{ c->fr_path, c->fr_pn, c->fr_pbox, pcap };
         const int pgrid = (nr + 63) / 64;
         if (c->sl.prec == FMM3D_PREC_FP64)
-            k_ray_paths<double><<<pgrid, 64, 0, c->stream>>>(g, (const double*)c->sl.T, c->sl.toff, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos);
+            k_ray_paths<double><<<pgrid, 64, 0, c->stream>>>(g, (const double*)c->sl.T, c->sl.toff, nullptr, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos);
         else
-            k_ray_paths<float><<<pgrid, 64, 0, c->stream>>>(g, (const float*)c->sl.T, c->sl.toff, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos);
+            k_ray_paths<float><<<pgrid, 64, 0, c->stream>>>(g, (const float*)c->sl.T, c->sl.toff, c->sl.ref, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos);
         e = hipGetLastError();
         if (e == hipSuccess) {
             const int grid = (int)std::min<size_t>(workers, (size_t)nr);
@@ -4161,6 +4377,7 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
     if (src3) { sp[0] = src3[0]; sp[1] = src3[1]; sp[2] = src3[2]; }
     if (e == hipSuccess) e = hipMemcpyAsync(s.srcpos + 3 * slot, sp, sizeof sp, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
+        k_fill_real<double><<<1, 256, 0, c->stream>>>(s.ref + (size_t)slot * (g.nnr + 2), (size_t)g.nnr + 2, 0.0);
         if (s.prec == FMM3D_PREC_FP64) {
             double* T = (double*)s.T + (size_t)slot * N;
             k_fill_real<double><<<1024, 256, 0, c->stream>>>(T, N, Lim<double>::inf());
@@ -4243,10 +4460,12 @@ int fmm3d_engine_admit_face(fmm3d_ctx* c, int slot, const double* face)
     if (e == hipSuccess) {
         // NaN faces are refused at submission; the flag is not read here
         k_tele_offsets<<<1, 1024, 0, c->stream>>>(NL, 0, 0, df, nullptr, nullptr, nullptr, s.toff + slot, c->d_bad);
+        // served slots keep the plain offset (no reference)
+        k_fill_real<double><<<1, 256, 0, c->stream>>>(s.ref + (size_t)slot * (g.nnr + 2), (size_t)g.nnr + 2, 0.0);
         if (s.prec == FMM3D_PREC_FP64)
-            k_init_tele<double><<<dim3(256, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, (double*)s.T + (size_t)slot * N, s.toff + slot);
+            k_init_tele<double><<<dim3(256, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, (double*)s.T + (size_t)slot * N, s.toff + slot, nullptr);
         else
-            k_init_tele<float><<<dim3(256, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, (float*)s.T + (size_t)slot * N, s.toff + slot);
+            k_init_tele<float><<<dim3(256, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, (float*)s.T + (size_t)slot * N, s.toff + slot, nullptr);
         k_fixbits_tele<<<dim3(64, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, s.fix + (size_t)slot * NW);
         // the slot's per-patch and per-slot state in one launch; every patch
         // column of the new slot runs in the next sweep
@@ -4346,10 +4565,10 @@ int fmm3d_engine_receivers_many(fmm3d_ctx* c, int n, const int* slots, const int
     if (e == hipSuccess) {
         const int nb = (int)((total + 255) / 256);
         if (c->sl.prec == FMM3D_PREC_FP64)
-            k_receivers_items<double><<<nb, 256, 0, c->stream>>>(c->g, (const double*)c->sl.T, c->sl.toff, (int)total,
+            k_receivers_items<double><<<nb, 256, 0, c->stream>>>(c->g, (const double*)c->sl.T, c->sl.toff, nullptr, (int)total,
                                                                  c->eng_islot, c->eng_rcv, c->eng_out);
         else
-            k_receivers_items<float><<<nb, 256, 0, c->stream>>>(c->g, (const float*)c->sl.T, c->sl.toff, (int)total,
+            k_receivers_items<float><<<nb, 256, 0, c->stream>>>(c->g, (const float*)c->sl.T, c->sl.toff, c->sl.ref, (int)total,
                                                                 c->eng_islot, c->eng_rcv, c->eng_out);
         e = hipGetLastError();
     }
@@ -4375,10 +4594,11 @@ int fmm3d_engine_receivers(fmm3d_ctx* c, int slot, int nrc, const double* rcv, f
     if (e == hipSuccess) {
         if (c->sl.prec == FMM3D_PREC_FP64)
             k_receivers<double><<<(nrc + 255) / 256, 256, 0, c->stream>>>(c->g, (const double*)c->sl.T + (size_t)slot * c->N,
-                                                                        c->sl.toff + slot, 1, nrc, drcv, dout);
+                                                                        c->sl.toff + slot, nullptr, 1, nrc, drcv, dout);
         else
             k_receivers<float><<<(nrc + 255) / 256, 256, 0, c->stream>>>(c->g, (const float*)c->sl.T + (size_t)slot * c->N,
-                                                                       c->sl.toff + slot, 1, nrc, drcv, dout);
+                                                                       c->sl.toff + slot,
+                                                                       c->sl.ref + (size_t)slot * (c->g.nnr + 2), 1, nrc, drcv, dout);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, dout, (size_t)nrc * sizeof(float), hipMemcpyDeviceToHost, c->stream);
@@ -4542,6 +4762,15 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "autotune must be 0 or 1");
         c->autotune = iv;
         if (!iv) c->tuned.clear();
+    } else if (!strcmp(key, "ref_qbits")) {
+        if (iv != 0 && (iv < 20 || iv > 30)) return fail(c, FMM3D_E_ARG, "ref_qbits must be 0 or 20..30");
+        c->ref_qbits = iv;
+    } else if (!strcmp(key, "tol_floor")) {
+        if (!(v >= 0.0 && v <= 1e4)) return fail(c, FMM3D_E_ARG, "tol_floor %g out of range", v);
+        c->tol_floor = v;
+    } else if (!strcmp(key, "tele_reference")) {
+        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "tele_reference must be 0 or 1");
+        c->tele_ref = iv;
     } else if (!strcmp(key, "graph")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "graph must be 0 or 1");
         c->use_graph = iv;

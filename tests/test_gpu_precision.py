@@ -11,13 +11,14 @@ injected on the bottom face and the two side faces they enter through,
   Frechet rows of the first 40 receivers with the same columns and values
   within 1e-4 relative, and every row's entry count, column sum and value
   2-norm (1e-4 relative);
-* float32 offset-time updates against float64 on the device, all 64 x 500
-  receivers: max |dt| <= 1 ms.  Rays traced on the float32 field follow its
-  gradient, so their rows differ from the float64 rows by more than the
-  float64 ones differ from the oracle's; the per-row relative 2-norm error
-  is printed and held to a median of 1e-3 and a maximum of 5e-2 (measured:
-  median 7e-5, p99 4.9e-3, maximum 1.9e-2; 88 % of rows keep the same
-  columns).
+* float32 updates against float64 on the device, all 64 x 500 receivers:
+  max |dt| <= 1 ms.  float32 fields are stored relative to each source's
+  plane-wave reference (offset + 1-D vertical profile + lateral plane).
+  Rays traced on the float32 field follow its gradient; the per-row
+  relative 2-norm error against the float64 rows is printed and held to a
+  median of 1e-5, at most 1 % of rows above 1e-4 and a maximum of 5e-2
+  (measured: median 3.5e-6, p99 3.3e-5, 0.4 % above 1e-4, maximum 2.6e-2 --
+  a ray that crosses a cell edge where the two fields order differently).
 """
 import hashlib
 import os
@@ -143,4 +144,7 @@ def test_fp32_frechet_error_is_stated(study):
     print(f"float32 vs float64 Frechet rows: {len(rel)} rows, same columns in {int(same.sum())}, "
           f"relative 2-norm error median {np.median(rel):.2e} p99 {np.percentile(rel, 99):.2e} "
           f"max {rel.max():.2e}, rows above 1e-4: {(rel > 1e-4).mean():.3f}")
-    assert np.median(rel) <= 1e-3 and rel.max() <= 5e-2
+    # float32 fields are stored against each source's plane-wave reference:
+    # 99 % of rows within 1e-4; a ray whose path crosses a cell edge where the
+    # two fields order differently takes another path (the rare large rows)
+    assert np.median(rel) <= 1e-5 and (rel > 1e-4).mean() <= 0.01 and rel.max() <= 5e-2
