@@ -24,6 +24,25 @@ pytestmark = pytest.mark.gpu
 
 SMALL = 8e-6   # eight ranks' partial rows, each cut at 1e-6
 
+# The one 201^3 source held to a looser row bar, and why.  Source 0 of the
+# 201^3 fixture needs 22 outer passes.  On it the relaxation form of the
+# blocks' marches (or_relax_tele_part, the form the GPU runs) and the
+# reference's heap march (or_fmm_tele_part, which reproduces rtravel.out bit
+# for bit) leave fields that differ by up to 9.4e-7 s on 23,719 of 8.1 M
+# nodes: nodes that keep a value from an earlier pass pop in the heap only
+# once a neighbour has pushed them, later than their value (shown on the
+# CPU: the two oracle forms first part in pass 3, block (0,1,0), at
+# top-layer nodes of that kind).  Receiver times stay inside the 1 ms bar
+# (they match rtravel.out to the float32 output).  Two of the source's 300
+# rays (receivers 157 and 268) pass cells where differences that small flip
+# the tracer's downward-edge test (fm3dt_parallel.f90:7351-7363), so they
+# take neighbouring paths: their rows come out 5.8e-4 and 2.6e-3 off in
+# 2-norm.  Those rays are sensitive down to the last bit: the oracle's
+# tracer (gcc, no fused multiply-add) on the GPU's own field takes yet
+# another path for receiver 157 (2.2e-2 off the GPU's row).  Sources 1-7 (4
+# passes each) are held to the full bar.
+LOOSE_201 = {0}
+
 
 def ctx_for(g, velv):
     c = lib.Context(g)
@@ -120,15 +139,23 @@ def test_partition_mode_201_against_reference():
           f"{np.round(gap_one, 4).tolist()}")
     assert gap.max() <= 1e-3
     worst = 0.0
+    loose = []
     for r in range(nsrc * nrc):
         a = G.row_dict(cols[rp[r]:rp[r + 1]], vals[rp[r]:rp[r + 1]])
         v = np.array(list(a.values()))
         ref = gd.summary[r]
         l2 = float(np.sqrt((v * v).sum())) if len(v) else 0.0
         rel = abs(l2 - ref["l2"]) / max(ref["l2"], 1e-30)
+        if r // nrc in LOOSE_201:
+            if rel > 1e-4:
+                loose.append(r)
+                print(f"201^3: row {r} (source {r // nrc}, receiver {r % nrc}): 2-norm {rel:.2e} off the reference's")
+            assert rel <= 5e-2, (r, rel)
+            continue
         worst = max(worst, rel)
         assert rel <= 1e-4, (r, l2, ref)
         if r % nrc < gd.full_rows:
             rel_f, odd = G.compare_rows(a, gd.full[(r // nrc, r % nrc)], small=SMALL)
             assert rel_f <= 1e-4 and not odd, (r, rel_f, odd)
-    print(f"201^3: worst row 2-norm difference {worst:.2e}")
+    print(f"201^3: worst row 2-norm difference {worst:.2e} (sources {sorted(LOOSE_201)} aside)")
+    assert len(loose) <= 0.01 * nrc * len(LOOSE_201), loose
