@@ -2875,7 +2875,9 @@ static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k, bo
                 unsigned long long v[2];
                 HIPCHK(c, hipMemcpyAsync(v, c->d_visits, sizeof v, hipMemcpyDeviceToHost, c->stream));
                 HIPCHK(c, hipStreamSynchronize(c->stream));
-                fprintf(stderr, "fmm3d trace: sweep %d visits %llu updates %llu\n", k + j, v[0], v[1]);
+                float ms = 0.0f;
+                if (c->timing) (void)hipEventElapsedTime(&ms, c->kev[2 * j], c->kev[2 * j + 1]);
+                fprintf(stderr, "fmm3d trace: sweep %d visits %llu updates %llu kernel %.3f ms\n", k + j, v[0], v[1], ms);
             }
         }
         if (c->timing) {
