@@ -50,13 +50,15 @@ BYTES_PER_NODE_VISIT = 12
 # rocprofv3 FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000), each scaled by a
 # 1 GiB copy, over the launches of one batch of 50
 # (profiles/k_column_tele201_batch50_traffic.md)
-MEASURED_BYTES_PER_NODE_VISIT = 41.42 + 7.62
-# VALU instructions per float32 node update in the column walk's pass loop
-# (counted in the kernel's gfx950 assembly), and the non-packed wave64 VALU
-# issue rate of the device: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
-# instruction per 4 cycles of a 16-lane SIMD
-VALU_PER_UPDATE = 240
-VALU_WAVE_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
+MEASURED_BYTES_PER_NODE_VISIT = 42.03 + 7.80
+# wave64 VALU instructions per 64 float32 node updates of the column walk,
+# from the rocprofv3 SQ_INSTS_VALU pass over one batch of 50, and the
+# device's wave64 VALU issue rate at the clock that pass measured
+# (GRBM_GUI_ACTIVE per XCD over the launch time, 2.33 GHz): 256 CUs x 4
+# SIMDs, one wave64 instruction per 4 cycles of a 16-lane SIMD
+# (profiles/k_column_tele201_batch50_valu.md)
+VALU_PER_64_UPDATES = 285.2
+VALU_WAVE_ISSUE_PER_S = 256 * 4 * 2.334e9 / 4
 
 
 def parse():
@@ -79,6 +81,11 @@ def parse():
                         "sources in one batch at 201^3; serving-mix: Poisson requests over 65-257^3 grids "
                         "with teleseismic and local sources")
     p.add_argument("--serve-rate", type=float, default=1.0, help="serving-mix: requests per second offered")
+    p.add_argument("--no-mixed", action="store_true",
+                   help="skip the 256-source mixed batch (192 teleseismic + 64 local) reported beside the headline")
+    p.add_argument("--only-batch", action="store_true",
+                   help="profiling: one warm-up float32 batch of the measured workload, then --steps timed ones "
+                        "(hipGraph replays), nothing else; prints their times and the solver's counts")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="first run one device-to-device copy of this many MiB (the reference kernel "
                         "that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -155,19 +162,15 @@ def row_errors(ctx, work, rcv, nv):
                           "fields of the timed batch"}
 
 
-def serving(g, velv, faces, rcv, seconds, t_src):
+def serving(g, velv, faces, rcv, seconds):
     """The serving loop under Poisson arrivals: requests of 4 float32 plane-wave
     sources (bottom faces) on the resident 201^3 model, each with receiver
     times and Frechet rows (the source-solve of the headline), offered at 70 %
-    of the batched bottom-face throughput; p50/p99 request latency and the
-    throughput the server sustains.  Results are checked against the batched
-    solve of the same sources."""
+    of the server's own capacity (a burst of 32 requests in flight together);
+    p50/p99 request latency and the throughput the server sustains.  Results
+    are checked against the batched solve of the same sources."""
     rng = np.random.default_rng(5)
     per = 4
-    rate = 0.7 * (1.0 / t_src) / per              # requests per second
-    n = max(8, int(rate * seconds))
-    gaps = rng.exponential(1.0 / rate, n)
-    picks = [rng.integers(0, faces.shape[0], per) for _ in range(n)]
     tickets = []
     with lib.Server(hbm_budget=64e9, max_slots=64) as S:
         # warm the model: the first request creates the resident context
@@ -175,6 +178,19 @@ def serving(g, velv, faces, rcv, seconds, t_src):
         S.wait(t, 600.0)
         S.rows(t)
         S.result(t)
+        # capacity: a burst of 32 requests submitted at once
+        b0 = time.perf_counter()
+        burst = [S.submit(g, velv, 11, faces=faces[rng.integers(0, faces.shape[0], per)], rcv=rcv, rows=True)
+                 for _ in range(32)]
+        for t in burst:
+            S.wait(t, 600.0)
+            S.rows(t)
+            S.result(t)
+        cap = 32 * per / (time.perf_counter() - b0)   # source-solves per second
+        rate = 0.7 * cap / per                         # requests per second
+        n = max(8, int(rate * seconds))
+        gaps = rng.exponential(1.0 / rate, n)
+        picks = [rng.integers(0, faces.shape[0], per) for _ in range(n)]
 
         t0 = time.perf_counter()
         next_t = t0
@@ -193,7 +209,8 @@ def serving(g, velv, faces, rcv, seconds, t_src):
             errs.append(out)
         span = time.perf_counter() - t0
         st = S.stats()
-    return {"requests": n, "sources_per_request": per, "offered_req_per_s": round(rate, 2),
+    return {"requests": n, "sources_per_request": per, "capacity_source_solves_per_s": round(cap, 2),
+            "offered_req_per_s": round(rate, 2),
             "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
             "source_solves_per_s": round(n * per / span, 2), "server_hbm_gb": round(st["hbm_bytes"] / 1e9, 2),
             "frechet_entries": nnz, "includes_rays": True}, tickets, errs
@@ -288,6 +305,19 @@ def mixed256(a):
         runs = [step(lib.PREC_FP32) for _ in range(a.steps)]
         tele_only = step(lib.PREC_FP32, point_on=False)
         point_only = step(lib.PREC_FP32, tele_on=False)
+        # the plain float64 GPU solver, one source per call (same pipeline:
+        # receivers and rows), on the first teleseismic and the first local
+        # source, weighted 192 : 64
+        def plain(f):
+            f()
+            s0 = time.perf_counter()
+            f()
+            c.receivers(rcv)
+            c.frechet(rcv, pinned=True)
+            return time.perf_counter() - s0
+        t_tele = plain(lambda: c.solve_tele_sides(F[:1], X[:1], Z[:1], S[:1], lib.ORDER_MIXED, lib.PREC_FP64))
+        t_point = plain(lambda: c.solve_point(src[:1], 5, int(rad[0]), lib.ORDER_MIXED, lib.PREC_FP64))
+    plain_rate = 256.0 / (192 * t_tele + 64 * t_point)
     el = sum(r["step_s"] for r in runs)
     err = float(np.nanmax(np.abs(runs[-1]["t"].astype(np.float64) - r64["t"])))
     med = lambda k: round(float(np.median([r[k] for r in runs])), 3)
@@ -300,15 +330,10 @@ def mixed256(a):
                                    f"{a.nrc} receivers; receiver times + Frechet rows"},
             "solve_wall_s": med("solve_wall_s"), "device_solve_ms": med("device_solve_ms"),
             "sweeps": runs[-1]["sweeps"], "frechet_entries": runs[-1]["entries"],
-            "max_receiver_err_s_vs_fp64": err,
-        "frechet_row_err_vs_fp64": rows_err,
-        "graph_timed": True,
-        "ms_per_step_direct_launches": round(1e3 * side["direct_elapsed"] / a.steps, 2),
-        "reference_partition_fp64": {
-            "source_solves_per_s": round(npart / part_s, 3), "sources": npart,
-            "passes": part_passes.tolist(),
-            "what": "the reference's own 2x2x2-partitioned float64 solve (FMM3D_REFERENCE_PARTITION, bottom "
-                    "faces), receivers and rows; reproduces the reference program's rtravel.out"}, "fp64_step_s": round(r64["step_s"], 3),
+            "max_receiver_err_s_vs_fp64": err, "fp64_step_s": round(r64["step_s"], 3),
+            "plain_fp64_source_solves_per_s": round(plain_rate, 3),
+            "ratio_vs_plain_fp64": round(256 * a.steps / el / plain_rate, 2),
+            "plain_fp64_s_per_source": {"teleseismic": round(t_tele, 4), "local": round(t_point, 4)},
             "tele_only": {k: round(tele_only[k], 3) for k in ("solve_wall_s", "device_solve_ms", "step_s")} | {"sweeps": tele_only["sweeps"]},
             "point_only": {k: round(point_only[k], 3) for k in ("solve_wall_s", "device_solve_ms", "step_s")} | {"sweeps": point_only["sweeps"]}}
 
@@ -417,6 +442,18 @@ def main():
     rcv = synth.receivers_rad(rcv_deg)
     ctx = lib.Context(g)
     ctx.set_model(velv)
+    if a.only_batch:
+        work = Workload(g, srcs, True)
+        one_step(ctx, work, rcv, lib.PREC_FP32)
+        out = []
+        for _ in range(a.steps):
+            s0 = time.perf_counter()
+            _, st, n = one_step(ctx, work, rcv, lib.PREC_FP32)
+            out.append({"ms": round(1e3 * (time.perf_counter() - s0), 2), "solve_ms": round(st["solve_ms"], 2),
+                        "sweeps": st["sweeps"], "node_updates": st["node_updates"], "node_visits": st["node_visits"],
+                        "frechet_entries": n})
+        print(json.dumps({"only_batch": out}), flush=True)
+        return
 
     def measure(work, steps, warmup, plain_n):
         """float32 error against float64, the timed float32 batches (sweeps
@@ -527,9 +564,13 @@ def main():
                      # one source-solve's algorithmic bytes (every node once) at the measured rate
                      "useful_work_gbs": round(g.nnr * g.nnx * g.nnz * BYTES_PER_NODE_VISIT * value / 1e9, 1),
                      "node_updates_per_s": round(upd / (kms * 1e-3), 1) if kms > 0 else None,
-                     # the kernel's actual bound: VALU issue (an estimate, not the contract's roofline)
-                     "valu_issue_frac_estimate": round(upd / (kms * 1e-3) * VALU_PER_UPDATE / 64
-                                                       / VALU_WAVE_ISSUE_PER_S, 3) if kms > 0 else None},
+                     "updates_per_node": round(upd / (a.steps * a.batch * g.nnr * g.nnx * g.nnz), 2),
+                     # the kernel's actual bound: VALU issue (this run's updates/s times the counter
+                     # pass's VALU instructions per update; not the contract's roofline)
+                     "valu_issue_frac": round(upd / (kms * 1e-3) / 64 * VALU_PER_64_UPDATES
+                                              / VALU_WAVE_ISSUE_PER_S, 3) if kms > 0 else None,
+                     "valu_source": "rocprofv3 SQ_INSTS_VALU pass in profiles/: 285.2 wave64 VALU instructions "
+                                    "per 64 node updates; issue rate 1024 SIMDs x 2.334 GHz / 4"},
         "bottom_faces_only": {"value": round(bottom["value"], 3),
                               "ms_per_step": round(1e3 * bottom["elapsed"] / a.steps, 2),
                               "plain_fp64_source_solves_per_s": round(bottom["plain"], 3) if bottom["plain"] else None,
@@ -537,8 +578,13 @@ def main():
                               "max_receiver_err_s_vs_fp64": bottom["err"]},
     }
     ctx.close()
+    if not a.no_mixed:
+        m = mixed256(argparse.Namespace(nrc=a.nrc, steps=1, warmup=1))
+        out["mixed256"] = {k: m[k] for k in ("value", "unit", "ms_per_step", "plain_fp64_source_solves_per_s",
+                                             "ratio_vs_plain_fp64", "max_receiver_err_s_vs_fp64", "sweeps")}
+        out["mixed256"]["workload"] = m["config"]["workload"] + " (float32; 1 timed step after 1 warm-up)"
     if a.serve_s > 0:
-        sv, tk, outs = serving(g, velv, faces, rcv, a.serve_s, bottom["elapsed"] / a.steps / a.batch)
+        sv, tk, outs = serving(g, velv, faces, rcv, a.serve_s)
         # every served request against the batched float32 solve of its sources
         err = 0.0
         for (t, idx), o in zip(tk, outs):

@@ -1938,6 +1938,10 @@ struct fmm3d_ctx {
     size_t fr_path_n = 0;          // samples fr_path holds
     size_t fr_path_rays = 0;       // rays fr_pn / fr_pbox hold
     int64_t fr_pcap = 0;           // path samples per ray (grows on overflow)
+    double* st_rcv = nullptr;      // receiver coordinates staged for receivers / rays (grown, kept:
+    size_t st_rcv_n = 0;           //   a hipFree waits for every stream of the device)
+    float* st_rout = nullptr;      // receiver times staged (grown, kept)
+    size_t st_rout_n = 0;
 };
 
 static thread_local std::string g_create_err;   // errors of calls without a context (per thread)
@@ -1976,6 +1980,20 @@ static int fail_to(std::string& err, int code, const char* fmt, ...)
             return fail((c), e_ == hipErrorOutOfMemory ? FMM3D_E_NOMEM : FMM3D_E_DEVICE,   \
                         "%s: %s", #expr, hipGetErrorString(e_));                           \
     } while (0)
+
+template <typename P> static hipError_t dalloc(P** p, size_t bytes);
+
+// A device staging buffer of at least n elements, grown and kept.
+template <typename P> static hipError_t staging(P** p, size_t* have, size_t n)
+{
+    if (*have >= n) return hipSuccess;
+    (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    hipError_t e = dalloc(p, n * sizeof(P));
+    if (e == hipSuccess) *have = n;
+    return e;
+}
 
 template <typename P> static hipError_t dalloc(P** p, size_t bytes)
 {
@@ -2198,7 +2216,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->d_iter);
     (void)hipFree(c->d_faces);
     (void)hipFree(c->st_offs); (void)hipFree(c->st_node); (void)hipFree(c->st_tv); (void)hipFree(c->d_velv); (void)hipFree(c->d_bad);
-    (void)hipFree(c->d_layer);
+    (void)hipFree(c->d_layer); (void)hipFree(c->st_rcv); (void)hipFree(c->st_rout);
     (void)hipFree(c->d_sides); (void)hipFree(c->d_side_of);
     (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv); (void)hipFree(c->eng_faces);
     (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out); (void)hipFree(c->eng_islot);
@@ -4054,10 +4072,10 @@ extern "C" int fmm3d_receivers(fmm3d_ctx* c, int nrc, const double* rcv, float* 
     if (nrc == 0) return FMM3D_OK;
     HIPCHK(c, hipSetDevice(c->device));
     const int ns = c->sl.n;
-    double* drcv = nullptr;
-    float* dout = nullptr;
-    HIPCHK(c, dalloc(&drcv, (size_t)nrc * 3 * sizeof(double)));
-    HIPCHK(c, dalloc(&dout, (size_t)ns * nrc * sizeof(float)));
+    HIPCHK(c, staging(&c->st_rcv, &c->st_rcv_n, (size_t)nrc * 3));
+    HIPCHK(c, staging(&c->st_rout, &c->st_rout_n, (size_t)ns * nrc));
+    double* drcv = c->st_rcv;
+    float* dout = c->st_rout;
     HIPCHK(c, hipMemcpyAsync(drcv, rcv, (size_t)nrc * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
     int total = ns * nrc;
     if (c->sl.prec == FMM3D_PREC_FP64)
@@ -4067,7 +4085,6 @@ extern "C" int fmm3d_receivers(fmm3d_ctx* c, int nrc, const double* rcv, float* 
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(t_out, dout, (size_t)total * sizeof(float), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    (void)hipFree(drcv); (void)hipFree(dout);
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "receivers: %s", hipGetErrorString(e));
     return FMM3D_OK;
 }
@@ -4136,8 +4153,8 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
     const size_t budget = (size_t)512 << 20;   // bytes of staging
     const size_t pbudget = (size_t)2 << 30;    // bytes of path samples
     if (c->fr_pcap <= 0) c->fr_pcap = 2 * ((int64_t)g.nnr + g.nnx + g.nnz) + 64;
-    double* drcv = nullptr;
-    HIPCHK(c, dalloc(&drcv, (size_t)nrc * 3 * sizeof(double)));
+    HIPCHK(c, staging(&c->st_rcv, &c->st_rcv_n, (size_t)nrc * 3));
+    double* drcv = c->st_rcv;
     hipError_t e = hipMemcpyAsync(drcv, rcv, (size_t)nrc * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
     std::vector<int32_t> cnt;
     std::vector<int64_t> off;
@@ -4242,7 +4259,6 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
         total += ctot;
         r0 += nr;
     }
-    (void)hipFree(drcv);
     if (e == hipErrorUnknown) return fail(c, FMM3D_E_DEVICE, "frechet: row staging or path overflow");
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "frechet: %s", hipGetErrorString(e));
     if (short_cap) return fail(c, FMM3D_E_CAPACITY, "fmm3d_frechet: capacity %lld < %lld entries", (long long)cap, (long long)row_ptr[nrays]);
