@@ -262,10 +262,21 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *   "graph"         1 (default): replay each chunk of check_every sweeps
  *                   from a captured hipGraph (batch solves; the serving
  *                   engine launches directly)
- *   "autotune"      1: the first solve of a kind (precision, order, walk,
- *                   batch size class) times patch shapes 8x8/8x16/16x8/16x16
- *                   with 8 or 16 passes on the real batch and keeps the
- *                   fastest (stats.tile_x/tile_z/tile_r report the choice)
+ *   "autotune"      1 (default): the first column-walk solve of a kind
+ *                   (device, grid, precision, order, walk mode, batch size
+ *                   class) takes its configuration from the tuning cache, or
+ *                   times candidates on the real batch -- the current
+ *                   configuration, patch shapes 8x8/8x16/16x8/16x16 with 16
+ *                   passes and 8x8 with 8, a 2-sweep check interval, the
+ *                   hyperplane walk -- and keeps one only when at least 5 %
+ *                   faster; the choice goes to the cache (FMM3D_TUNE_CACHE,
+ *                   else $XDG_CACHE_HOME/fmm3d/tune.txt, else
+ *                   ~/.cache/fmm3d/tune.txt; FMM3D_TUNE_CACHE="" for none).
+ *                   Batches under 8 sources or grids under 2 M nodes keep
+ *                   the current configuration untimed, and so does a context
+ *                   whose strategy, column_x/column_z, column_passes or
+ *                   check_every the caller set (stats.tile_x/tile_z/tile_r
+ *                   report the configuration used); 0: off
  *   "max_sweeps"    column walk: stop after this many sweeps (0 = at
  *                   convergence); a diagnostic, the field is then approximate
  *   "timing"        1: time every relaxation launch with events (stats.kernel_ms);

@@ -31,6 +31,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <array>
@@ -1796,6 +1797,11 @@ struct Slots {
 
 }  // namespace
 
+// A relaxation configuration the autotuner chooses between.
+struct TuneChoice {
+    int strategy, px, pz, passes, check;
+};
+
 struct PartState;                  // the reference-partition mode's device state (fmm3d_partition.inl)
 static void part_free(PartState* ps);
 static void part_model_changed(PartState* ps);
@@ -1843,12 +1849,14 @@ struct fmm3d_ctx {
     // only; in float64 the narrower sweep slowed propagation, 6 -> 17 sweeps
     // on 50 plane waves at 201^3)
     int verify_patches = -1;
-    int autotune = 0;              // time candidate patch shapes / pass caps at first use
+    int autotune = 1;              // time candidate configurations at the first solve of a kind
+    bool tune_fixed = false;       // the caller chose the strategy, patch, pass cap or check interval
+    std::string tune_arch;         // device identity for the tuning cache (arch name / compute units)
     int tele_ref = 1;              // float32 teleseismic fields relative to a 1-D plane-wave reference
     double tol_floor = 64.0;       // those fields: a change counts above tol32 * max(|value|, tol_floor)
     int ref_qbits = 22;            // those fields: updates rounded to multiples of 2^-ref_qbits s (0: not)
     std::vector<double> layer_slo; // mean slowness of each layer of the current model
-    std::vector<std::array<int, 8>> tuned;   // key (prec, order, walk, batch class) -> (px, pz, passes)
+    std::vector<std::pair<std::string, TuneChoice>> tuned;   // tuning key -> configuration (this context)
     int* d_iter = nullptr;         // device sweep counter (graph replays)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1905,6 +1913,7 @@ struct fmm3d_ctx {
     int ordered = 1;               // column walk: patches wait for their upstream neighbours
     int spin_limit = 20000;        // polls (~64 cycles each) before such a wait is given up
     int max_sweeps = 0;            // column walk: stop after this many sweeps (0: at convergence)
+    bool walk_settled = true;      // the last column-walk solve ended at its fixed point (not at max_sweeps)
     int point_order = 1;           // alternating walks: patches outward from the earliest fixed node, waiting on their inner neighbours
     int point_window = 1;          // alternating walks: each patch walks the layers near the last sweeps' changes only
     int point_warm = 1;            // refined blocks: a float32 solve first, then float64 sweeps from its field
@@ -2156,6 +2165,13 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     if (sp) c->spin_limit = std::max(0, atoi(sp));
     const char* at = getenv("FMM3D_AUTOTUNE");
     if (at) c->autotune = atoi(at) != 0;
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c->device) == hipSuccess)
+            c->tune_arch = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount) + "cu";
+        else
+            c->tune_arch = "unknown";
+    }
     const char* trf = getenv("FMM3D_TELE_REF");
     if (trf) c->tele_ref = atoi(trf) != 0;
     const char* tfl = getenv("FMM3D_TOL_FLOOR");
@@ -2170,6 +2186,7 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     if (tr) c->trace = atoi(tr);
     const char* sg = getenv("FMM3D_STRATEGY");   // "brick" | "column[:PXxPZ[:passes]]" | "plane[:chunk]"
     if (sg) {
+        c->tune_fixed = true;
         if (!strncmp(sg, "brick", 5)) c->strategy = 0;
         else if (!strncmp(sg, "plane", 5)) {
             c->strategy = 2;
@@ -2978,6 +2995,7 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     HIPCHK(c, hipGetLastError());
     int k = 1, last = 0;
     bool done = false;
+    c->walk_settled = true;
     std::vector<int> onv(nsrc, 1);
     while (!done) {
         const int chunk = c->max_sweeps > 0 ? std::min(c->check_every, c->max_sweeps - k + 1) : c->check_every;
@@ -3005,7 +3023,7 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
         }
         last = mx;
         if (mx < k - 1) done = true;
-        if (!done && c->max_sweeps > 0 && k > c->max_sweeps) break;
+        if (!done && c->max_sweeps > 0 && k > c->max_sweeps) { c->walk_settled = false; break; }
         if (!done && k > maxit) return fail(c, FMM3D_E_NOCONV, "solver: column walk did not converge in %d sweeps", maxit);
     }
     c->stats.phase_sweeps[0] = last + 1;
@@ -3017,53 +3035,165 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
 // and record the statistics.
 static int solve_run_once(fmm3d_ctx* c, int nsrc, int order, int precision);
 
-// Autotuner: the first solve of a kind (precision, order, walk mode, batch
-// size class) on a context times every candidate (patch shape x pass cap)
-// on the real batch, restarting from a copy of the initial fields, and keeps
-// the fastest for the rest of the context's life.
-static int autotune_column(fmm3d_ctx* c, int nsrc, int order, int precision)
+// ------------------------------------------------------------ autotuner
+
+// Tuning key: device, grid, precision, order, walk mode, batch size class.
+static std::string tune_key(const fmm3d_ctx* c, int nsrc, int order, int precision)
 {
     int cls = 0;
     while ((1 << cls) < nsrc) ++cls;
-    const std::array<int, 8> key = { precision, order, c->walk_alt, cls, 0, 0, 0, 0 };
+    char b[256];
+    snprintf(b, sizeof b, "%s %dx%dx%d p%d o%d w%d b%d", c->tune_arch.c_str(), c->g.nnr, c->g.nnx, c->g.nnz,
+             precision, order, c->walk_alt, cls);
+    return b;
+}
+
+// The tuning cache: FMM3D_TUNE_CACHE (empty: none), else
+// $XDG_CACHE_HOME/fmm3d/tune.txt, else ~/.cache/fmm3d/tune.txt.  One line per
+// choice, "<key> = <strategy> <patch theta> <patch phi> <pass cap> <check
+// interval>"; the last line of a key wins.
+static std::string tune_cache_path()
+{
+    const char* p = getenv("FMM3D_TUNE_CACHE");
+    if (p) return p;
+    const char* x = getenv("XDG_CACHE_HOME");
+    if (x && *x) return std::string(x) + "/fmm3d/tune.txt";
+    const char* h = getenv("HOME");
+    if (h && *h) return std::string(h) + "/.cache/fmm3d/tune.txt";
+    return "";
+}
+
+static bool tune_choice_ok(const TuneChoice& t)
+{
+    return t.strategy >= 1 && t.strategy <= 2 && column_supported(t.px, t.pz) && t.passes >= 1 &&
+           t.passes <= 16 && t.check >= 1 && t.check <= 8;
+}
+
+static bool tune_cache_get(const std::string& key, TuneChoice* out)
+{
+    const std::string path = tune_cache_path();
+    if (path.empty()) return false;
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[512];
+    bool found = false;
+    const std::string pre = key + " = ";
+    while (fgets(line, sizeof line, f)) {
+        if (strncmp(line, pre.c_str(), pre.size())) continue;
+        TuneChoice t;
+        if (sscanf(line + pre.size(), "%d %d %d %d %d", &t.strategy, &t.px, &t.pz, &t.passes, &t.check) == 5 &&
+            tune_choice_ok(t)) {
+            *out = t;
+            found = true;
+        }
+    }
+    fclose(f);
+    return found;
+}
+
+static void tune_cache_put(const std::string& key, const TuneChoice& t)
+{
+    const std::string path = tune_cache_path();
+    if (path.empty()) return;
+    // create the directories on the way (errors surface as a failed open)
+    for (size_t i = 1; i < path.size(); ++i)
+        if (path[i] == '/') (void)mkdir(path.substr(0, i).c_str(), 0755);
+    FILE* f = fopen(path.c_str(), "a");
+    if (!f) return;
+    fprintf(f, "%s = %d %d %d %d %d\n", key.c_str(), t.strategy, t.px, t.pz, t.passes, t.check);
+    fclose(f);
+}
+
+static TuneChoice tune_current(const fmm3d_ctx* c)
+{
+    return { c->strategy, c->col_px, c->col_pz, c->col_passes, c->check_every };
+}
+
+static void tune_apply(fmm3d_ctx* c, const TuneChoice& t)
+{
+    c->strategy = t.strategy; c->col_px = t.px; c->col_pz = t.pz; c->col_passes = t.passes; c->check_every = t.check;
+}
+
+// Autotuner.  At the first column-walk solve of a kind (the tuning key) on a
+// context whose configuration the caller left alone, the choice comes from
+// the tuning cache, or is timed on the real batch: the current configuration
+// first, then the column walk's other patch shapes and a pass cap of 8,
+// every-2-sweeps convergence checks, and the hyperplane walk (batches without
+// alternating walks); each restarts from a copy of the initial fields.  A
+// candidate replaces the current configuration only when at least 5 % faster;
+// the choice goes to the cache.  Batches under 8 sources or grids under 2 M
+// nodes keep the current configuration untimed.
+static int autotune(fmm3d_ctx* c, int nsrc, int order, int precision)
+{
+    const std::string key = tune_key(c, nsrc, order, precision);
     for (auto& e : c->tuned)
-        if (e[0] == key[0] && e[1] == key[1] && e[2] == key[2] && e[3] == key[3]) {
-            c->col_px = e[4]; c->col_pz = e[5]; c->col_passes = e[6];
+        if (e.first == key) {
+            tune_apply(c, e.second);
             return FMM3D_OK;
         }
-    static const int cand[][3] = { { 8, 8, 16 }, { 8, 8, 8 }, { 8, 16, 16 }, { 16, 8, 16 },
-                                   { 16, 16, 16 }, { 16, 16, 8 }, { 8, 16, 8 }, { 16, 8, 8 } };
+    TuneChoice ch = tune_current(c);
+    if (tune_cache_get(key, &ch)) {
+        tune_apply(c, ch);
+        c->tuned.push_back({ key, ch });
+        return FMM3D_OK;
+    }
+    const TuneChoice def = tune_current(c);
+    if (nsrc < 8 || c->N < ((size_t)1 << 21)) {
+        c->tuned.push_back({ key, def });
+        return FMM3D_OK;
+    }
+    std::vector<TuneChoice> cand = { def };
+    static const int shapes[][3] = { { 8, 8, 16 }, { 8, 8, 8 }, { 8, 16, 16 }, { 16, 8, 16 }, { 16, 16, 16 } };
+    for (auto& q : shapes) {
+        const TuneChoice t = { 1, q[0], q[1], q[2], def.check };
+        if (t.px != def.px || t.pz != def.pz || t.passes != def.passes || def.strategy != 1) cand.push_back(t);
+    }
+    cand.push_back({ 1, def.px, def.pz, def.passes, def.check == 2 ? 4 : 2 });
+    if (c->walk_alt == 0 && c->geo_cap == 0) cand.push_back({ 2, 8, 8, def.passes, def.check });
     Slots& s = c->sl;
     const size_t bytes = (size_t)nsrc * c->N * (precision == FMM3D_PREC_FP64 ? 8 : 4);
     void* save = nullptr;
     HIPCHK(c, dalloc(&save, bytes));
     HIPCHK(c, hipMemcpyAsync(save, s.T, bytes, hipMemcpyDeviceToDevice, c->stream));
-    double best = 1e300;
-    int bi = 0, rc = FMM3D_OK;
-    for (int i = 0; i < (int)(sizeof cand / sizeof cand[0]); ++i) {
-        c->col_px = cand[i][0]; c->col_pz = cand[i][1]; c->col_passes = cand[i][2];
+    std::vector<double> ms(cand.size(), 1e300);
+    int rc = FMM3D_OK;
+    const int max_sweeps0 = c->max_sweeps;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        tune_apply(c, cand[i]);
         hipError_t e = hipMemcpyAsync(s.T, save, bytes, hipMemcpyDeviceToDevice, c->stream);
         if (e != hipSuccess) { rc = fail(c, FMM3D_E_DEVICE, "autotune: %s", hipGetErrorString(e)); break; }
-        rc = solve_run_once(c, nsrc, order, precision);
-        if (rc) break;
-        if (c->stats.solve_ms < best) { best = c->stats.solve_ms; bi = i; }
+        const int r = solve_run_once(c, nsrc, order, precision);
+        // a candidate that does not settle within the sweep cap is out
+        if (r == FMM3D_E_NOCONV || (r == FMM3D_OK && !c->walk_settled)) continue;
+        if (r) { rc = r; break; }
+        ms[i] = c->stats.solve_ms;
+        // the others get at most three times the current configuration's sweeps
+        if (i == 0) c->max_sweeps = 3 * c->stats.sweeps + 4;
     }
+    c->max_sweeps = max_sweeps0;
     if (rc == FMM3D_OK) {
         hipError_t e = hipMemcpyAsync(s.T, save, bytes, hipMemcpyDeviceToDevice, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = fail(c, FMM3D_E_DEVICE, "autotune: %s", hipGetErrorString(e));
     }
     (void)hipFree(save);
+    tune_apply(c, def);
     if (rc) return rc;
-    c->col_px = cand[bi][0]; c->col_pz = cand[bi][1]; c->col_passes = cand[bi][2];
-    c->tuned.push_back({ key[0], key[1], key[2], key[3], c->col_px, c->col_pz, c->col_passes, 0 });
+    size_t bi = 0;
+    for (size_t i = 1; i < cand.size(); ++i)
+        if (ms[i] < ms[bi]) bi = i;
+    ch = ms[bi] < 0.95 * ms[0] ? cand[bi] : def;
+    tune_apply(c, ch);
+    c->tuned.push_back({ key, ch });
+    tune_cache_put(key, ch);
+    if (c->trace) fprintf(stderr, "fmm3d autotune: %s -> %d %d %d %d %d\n", key.c_str(), ch.strategy, ch.px, ch.pz, ch.passes, ch.check);
     return FMM3D_OK;
 }
 
 static int solve_run(fmm3d_ctx* c, int nsrc, int order, int precision)
 {
-    if (c->autotune && (c->strategy == 1 || (c->strategy == 0 && c->walk_alt))) {
-        int rc = autotune_column(c, nsrc, order, precision);
+    if (c->autotune && !c->tune_fixed && c->strategy >= 1) {
+        int rc = autotune(c, nsrc, order, precision);
         if (rc) return rc;
     }
     int rc = solve_run_once(c, nsrc, order, precision);
@@ -3738,8 +3868,9 @@ static int point_group(const fmm3d_ctx* c, fmm3d_ctx** finep, std::string& err, 
         if (rc) return fail_to(err, rc, "refined grid: %s", fmm3d_last_error(nullptr));
         fc = *finep;
         // direct launches (no graph capture while other refined solves run
-        // on other threads)
+        // on other threads); the refined solves keep the default configuration
         fc->use_graph = 0;
+        fc->autotune = 0;
     }
     fc->g = cg;
     fc->point_order = c->point_order;
@@ -4748,6 +4879,7 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "strategy")) {
         if (iv < 0 || iv > 2) return fail(c, FMM3D_E_ARG, "strategy must be 0, 1 or 2");
         c->strategy = iv;
+        c->tune_fixed = true;
     } else if (!strcmp(key, "plane_sweeps")) {
         if (iv < 0 || iv > 1000) return fail(c, FMM3D_E_ARG, "plane_sweeps must be in 0..1000");
         c->plane_sweeps = iv;
@@ -4757,6 +4889,7 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "column_x") || !strcmp(key, "column_z")) {
         if (iv != 8 && iv != 16 && iv != 32 && iv != 64) return fail(c, FMM3D_E_ARG, "%s must be 8, 16, 32 or 64", key);
         (key[7] == 'x' ? c->col_px : c->col_pz) = iv;
+        c->tune_fixed = true;
     } else if (!strcmp(key, "ordered")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "ordered must be 0 or 1");
         c->ordered = iv;
@@ -4773,6 +4906,7 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
         // at most 8: the per-launch timing events hold one chunk of 8 launches
         if (iv < 1 || iv > 8) return fail(c, FMM3D_E_ARG, "check_every must be in 1..8");
         c->check_every = iv;
+        c->tune_fixed = true;
     } else if (!strcmp(key, "spin_limit")) {
         if (iv < 0 || iv > 10000000) return fail(c, FMM3D_E_ARG, "spin_limit out of range");
         c->spin_limit = iv;
@@ -4812,6 +4946,7 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
         c->max_sweeps = iv;
     } else if (!strcmp(key, "column_passes")) {
         if (iv < 1 || iv > 16) return fail(c, FMM3D_E_ARG, "column_passes must be in 1..16");
+        c->tune_fixed = true;
         c->col_passes = iv;
     } else if (!strcmp(key, "timing")) {
         c->timing = iv != 0;

@@ -1,6 +1,7 @@
 """Sweep chunks replayed from a captured hipGraph give the fields the direct
 launches give, solve after solve with other work in between; the autotuner
-picks one of its candidates and its solve matches the untuned one."""
+times its candidates at the first solve of a kind, caches the choice in its
+file and reuses it."""
 import numpy as np
 import pytest
 
@@ -28,24 +29,70 @@ def test_graph_replays_match_direct_launches():
                 c.receivers(gd.rcv)
 
 
-def test_autotuner_choice_and_result():
-    g, velv, srcs, rcv = synth.config_201(nsrc=8, nrc=50, seed=2)
-    g2, velv2, _, _ = synth.tele_case((61, 61, 61), 400.0, (10.0, 10.0), (12, 12, 12), 0.05, 1, 20, 3)
-    faces = np.stack([synth.tele_bottom_face(g2, s) for s in synth.tele_sources(g2, 6, 3)])
-    with lib.Context(g2) as c, lib.Context(g2) as d:
-        c.set_model(velv2)
-        d.set_model(velv2)
-        c.set_option("autotune", 1)
-        c.solve_tele(faces, lib.ORDER_MIXED, lib.PREC_FP64)
+def _tune_case(n=129, nsrc=8, seed=3):
+    g = synth.make_grid((n, n, n), 600.0, (15.0, 15.0), nv=(20, 20, 20))
+    velv = synth.make_velv(g, 0.05, seed)
+    faces = np.stack([synth.tele_bottom_face(g, s) for s in synth.tele_sources(g, nsrc, seed)])
+    return g, velv, faces
+
+
+def test_autotuner_times_caches_and_reuses(tmp_path, monkeypatch):
+    """The first float32 solve of a kind times the candidates and writes its
+    choice to the tuning cache; a new context reads it back (same
+    configuration, the same fields bit for bit); a choice written to the
+    cache by hand is applied as it stands."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("FMM3D_TUNE_CACHE", str(cache))
+    g, velv, faces = _tune_case()
+    cands = {(8, 8, 16), (8, 8, 8), (8, 16, 16), (16, 8, 16), (16, 16, 16)}
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        c.solve_tele(faces, lib.ORDER_MIXED, lib.PREC_FP32)
         st = c.stats()
-        assert (st["tile_x"], st["tile_z"], st["tile_r"]) in {(8, 8, 16), (8, 8, 8), (8, 16, 16), (16, 8, 16),
-                                                             (16, 16, 16), (16, 16, 8), (8, 16, 8), (16, 8, 8)}
-        d.solve_tele(faces, lib.ORDER_MIXED, lib.PREC_FP64)
+        fields = [c.field(s) for s in range(faces.shape[0])]
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and " = " in lines[0], lines
+    key, val = lines[0].split(" = ")
+    strategy, px, pz, passes, check = map(int, val.split())
+    print(f"tuned: {lines[0]}", flush=True)
+    assert f"{g.nnr}x{g.nnx}x{g.nnz}" in key
+    assert strategy in (1, 2) and check in (2, 4)
+    if strategy == 1:
+        assert (px, pz, passes) in cands and (st["tile_x"], st["tile_z"], st["tile_r"]) == (px, pz, passes)
+    with lib.Context(g) as d:
+        d.set_model(velv)
+        d.solve_tele(faces, lib.ORDER_MIXED, lib.PREC_FP32)
+        assert (d.stats()["tile_x"], d.stats()["tile_z"], d.stats()["tile_r"]) == (st["tile_x"], st["tile_z"], st["tile_r"])
         for s in range(faces.shape[0]):
-            assert float(np.abs(c.field(s) - d.field(s)).max()) <= 1e-7
-        # the cached choice is reused without re-tuning
+            assert np.array_equal(d.field(s), fields[s]), s
+    assert len(cache.read_text().splitlines()) == 1   # read, not re-tuned
+    # a hand-written choice for the same key (the last line of a key wins)
+    with open(cache, "a") as f:
+        f.write(f"{key} = 1 16 16 8 2\n")
+    with lib.Context(g) as e:
+        e.set_model(velv)
+        e.solve_tele(faces, lib.ORDER_MIXED, lib.PREC_FP32)
+        assert (e.stats()["tile_x"], e.stats()["tile_z"], e.stats()["tile_r"]) == (16, 16, 8)
+        for s in range(faces.shape[0]):
+            assert float(np.abs(e.field(s) - fields[s]).max()) <= 1e-3, s
+
+
+def test_caller_choice_is_not_tuned(tmp_path, monkeypatch):
+    """A configuration the caller sets is used as it stands: no timing, no
+    cache entry; small batches keep the default untimed."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("FMM3D_TUNE_CACHE", str(cache))
+    g, velv, faces = _tune_case()
+    with lib.Context(g) as c:
+        c.set_model(velv)
+        c.set_option("column_x", 16)
         c.solve_tele(faces, lib.ORDER_MIXED, lib.PREC_FP64)
-        assert c.stats()["tile_x"] == st["tile_x"] and c.stats()["tile_r"] == st["tile_r"]
+        assert c.stats()["tile_x"] == 16
+    with lib.Context(g) as d:
+        d.set_model(velv)
+        d.solve_tele(faces[:4], lib.ORDER_MIXED, lib.PREC_FP64)
+        assert (d.stats()["tile_x"], d.stats()["tile_z"], d.stats()["tile_r"]) == (8, 8, 16)
+    assert not cache.exists()
 
 
 @pytest.mark.parametrize("every", [1, 3, 8])
