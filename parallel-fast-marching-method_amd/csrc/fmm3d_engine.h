@@ -37,10 +37,11 @@ FMM3D_HIDDEN int fmm3d_engine_receivers(fmm3d_ctx* c, int slot, int nrc, const d
 // its nrc[k] receivers rcv[k] into out[k]
 FMM3D_HIDDEN int fmm3d_engine_receivers_many(fmm3d_ctx* c, int n, const int* slots, const int* nrc,
                                              const double* const* rcv, float* const* out);
-// one slot's rays and Frechet rows (fmm3d_frechet's rows for that slot):
-// rp gets nrc+1 offsets into cols / vals
-FMM3D_HIDDEN int fmm3d_engine_frechet(fmm3d_ctx* c, int slot, int nrc, const double* rcv, std::vector<int64_t>& rp,
-                                      std::vector<int32_t>& cols, std::vector<float>& vals);
+// the rays and Frechet rows of n slots in one pass (slot slots[k] at its
+// nrc[k] receivers rcv[k]): rows slot after slot, rp gets sum(nrc)+1 offsets
+FMM3D_HIDDEN int fmm3d_engine_frechet_many(fmm3d_ctx* c, int n, const int* slots, const int* nrc,
+                                           const double* const* rcv, std::vector<int64_t>& rp,
+                                           std::vector<int32_t>& cols, std::vector<float>& vals);
 FMM3D_HIDDEN int fmm3d_engine_retire(fmm3d_ctx* c, int slot);
 FMM3D_HIDDEN int fmm3d_engine_active(const fmm3d_ctx* c);
 // A host-only view of a grid and vertex model (no device memory, no

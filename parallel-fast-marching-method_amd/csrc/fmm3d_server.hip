@@ -288,6 +288,11 @@ void fmm3d_server::run()
     std::vector<int> rslots, rnrc;
     std::vector<const double*> rrcv;
     std::vector<float*> rout;
+    std::vector<int> fslots, fnrc;        // slots whose rows are traced together
+    std::vector<const double*> frcv;
+    std::vector<int64_t> frp;
+    std::vector<int32_t> fcols;
+    std::vector<float> fvals;
     for (;;) {
         std::vector<Request*> waiting;
         std::vector<int> ready;
@@ -379,16 +384,35 @@ void fmm3d_server::run()
             }
             int rst = fmm3d_engine_receivers_many(m->ctx, (int)rslots.size(), rslots.data(), rnrc.data(),
                                                   rrcv.data(), rout.data());
-            // rays and Frechet rows of the converged slots whose request asked for them
+            // rays and Frechet rows of the converged slots whose request asked
+            // for them, all in one pass, then split per slot
+            fslots.clear(); fnrc.clear(); frcv.clear();
             for (int slot : done) {
                 Request* r = m->owner[slot].first;
                 if (!r || !r->want_rows || rst != FMM3D_OK) continue;
-                const int q = m->owner[slot].second;
-                const int frc = fmm3d_engine_frechet(m->ctx, slot, r->nrc, r->rcv.data(), r->rrp[q], r->rcols[q], r->rvals[q]);
-                if (frc != FMM3D_OK) {
-                    trace_failure(r, frc, fmm3d_last_error(m->ctx));
-                    std::lock_guard<std::mutex> lk(mu);
-                    if (r->status == FMM3D_OK) r->status = frc;
+                fslots.push_back(slot);
+                fnrc.push_back(r->nrc);
+                frcv.push_back(r->rcv.data());
+            }
+            if (!fslots.empty()) {
+                const int frc = fmm3d_engine_frechet_many(m->ctx, (int)fslots.size(), fslots.data(), fnrc.data(),
+                                                          frcv.data(), frp, fcols, fvals);
+                int64_t ray = 0;
+                for (size_t k = 0; k < fslots.size(); ++k) {
+                    Request* r = m->owner[fslots[k]].first;
+                    const int q = m->owner[fslots[k]].second;
+                    if (frc == FMM3D_OK) {
+                        const int64_t b0 = frp[ray], b1 = frp[ray + fnrc[k]];
+                        r->rrp[q].resize((size_t)fnrc[k] + 1);
+                        for (int i = 0; i <= fnrc[k]; ++i) r->rrp[q][i] = frp[ray + i] - b0;
+                        r->rcols[q].assign(fcols.begin() + b0, fcols.begin() + b1);
+                        r->rvals[q].assign(fvals.begin() + b0, fvals.begin() + b1);
+                    } else {
+                        trace_failure(r, frc, fmm3d_last_error(m->ctx));
+                        std::lock_guard<std::mutex> lk(mu);
+                        if (r->status == FMM3D_OK) r->status = frc;
+                    }
+                    ray += fnrc[k];
                 }
             }
             for (int pass = 0; pass < 2; ++pass)

@@ -1429,13 +1429,15 @@ template <typename R>
 __global__ void __launch_bounds__(64)
 k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const double* __restrict__ ref,
             const double* __restrict__ veln, int nrc, const double* __restrict__ rcv, int ray0, int nrays, RayPaths P,
-            const double* __restrict__ srcpos)
+            const double* __restrict__ srcpos, const int* __restrict__ islot)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nrays) return;
     const int rid = ray0 + q;
     const int NVR = g.nvr + 2, NVX = g.nvx + 2, NVZ = g.nvz + 2;
-    const int s = rid / nrc, i = rid % nrc;
+    // ray rid: receiver rid % nrc of slot rid / nrc, or (items) the slot
+    // islot[rid] at receiver rcv[3 rid .. 3 rid + 2]
+    const int s = islot ? islot[rid] : rid / nrc, i = islot ? rid : rid % nrc;
     const size_t N = (size_t)g.nnr * g.nnx * g.nnz;
     const R* F = T + (size_t)s * N;
     const double to = toff[s];
@@ -1951,6 +1953,8 @@ struct fmm3d_ctx {
     size_t st_rcv_n = 0;           //   a hipFree waits for every stream of the device)
     float* st_rout = nullptr;      // receiver times staged (grown, kept)
     size_t st_rout_n = 0;
+    int* st_islot = nullptr;       // per-ray slot indices staged (grown, kept)
+    size_t st_islot_n = 0;
 };
 
 static thread_local std::string g_create_err;   // errors of calls without a context (per thread)
@@ -2233,7 +2237,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->d_iter);
     (void)hipFree(c->d_faces);
     (void)hipFree(c->st_offs); (void)hipFree(c->st_node); (void)hipFree(c->st_tv); (void)hipFree(c->d_velv); (void)hipFree(c->d_bad);
-    (void)hipFree(c->d_layer); (void)hipFree(c->st_rcv); (void)hipFree(c->st_rout);
+    (void)hipFree(c->d_layer); (void)hipFree(c->st_rcv); (void)hipFree(c->st_rout); (void)hipFree(c->st_islot);
     (void)hipFree(c->d_sides); (void)hipFree(c->d_side_of);
     (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv); (void)hipFree(c->eng_faces);
     (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out); (void)hipFree(c->eng_islot);
@@ -2927,11 +2931,13 @@ static int launch_chunk(fmm3d_ctx* c, ColArgs<R> a, int blocks, int n, int k, bo
     }
     a.iter = 0;
     a.iterp = c->d_iter;
-    std::vector<char> sig(sizeof a + 3 * sizeof(int));
+    std::vector<char> sig(sizeof a + 4 * sizeof(int));
     memcpy(sig.data(), &a, sizeof a);
-    // the walk mode decides whether the graph holds the fixed-point check
-    const int tag[3] = { blocks, (int)sizeof(R) * 100 + kind * 10000000 + c->strategy * 1000000 +
-                                     c->plane_ch * 100000 + c->col_px * 10000 + c->col_pz, c->walk_alt };
+    // the walk mode decides whether the graph holds the fixed-point check;
+    // n is the sweeps the graph holds (a graph of fewer sweeps replayed for
+    // a longer chunk would skip sweeps)
+    const int tag[4] = { blocks, (int)sizeof(R) * 100 + kind * 10000000 + c->strategy * 1000000 +
+                                     c->plane_ch * 100000 + c->col_px * 10000 + c->col_pz, c->walk_alt, n };
     memcpy(sig.data() + sizeof a, tag, sizeof tag);
     if (!c->gexec || sig != c->g_sig) {
         if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
@@ -4256,7 +4262,7 @@ extern "C" int fmm3d_receivers_noalloc(fmm3d_ctx* c, int nrc, const double* rcv,
 // Rays ray_begin .. ray_end-1 (ray slot*nrc + i: slot's field, receiver i)
 // with their Frechet rows; row_ptr has ray_end-ray_begin+1 entries.
 static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin, int ray_end, int64_t* row_ptr,
-                         int32_t* cols, float* vals, int64_t cap)
+                         int32_t* cols, float* vals, int64_t cap, const int* islot = nullptr)
 {
     HIPCHK(c, hipSetDevice(c->device));
     const Grid& g = c->g;
@@ -4284,9 +4290,17 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
     const size_t budget = (size_t)512 << 20;   // bytes of staging
     const size_t pbudget = (size_t)2 << 30;    // bytes of path samples
     if (c->fr_pcap <= 0) c->fr_pcap = 2 * ((int64_t)g.nnr + g.nnx + g.nnz) + 64;
-    HIPCHK(c, staging(&c->st_rcv, &c->st_rcv_n, (size_t)nrc * 3));
+    // items (islot): one slot index and one receiver per ray, rays 0 .. nrays-1
+    const size_t nco = islot ? (size_t)nrays : (size_t)nrc;
+    HIPCHK(c, staging(&c->st_rcv, &c->st_rcv_n, nco * 3));
     double* drcv = c->st_rcv;
-    hipError_t e = hipMemcpyAsync(drcv, rcv, (size_t)nrc * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    int* dslot = nullptr;
+    hipError_t e = hipMemcpyAsync(drcv, rcv, nco * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (islot) {
+        HIPCHK(c, staging(&c->st_islot, &c->st_islot_n, (size_t)nrays));
+        dslot = c->st_islot;
+        if (e == hipSuccess) e = hipMemcpyAsync(dslot, islot, (size_t)nrays * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    }
     std::vector<int32_t> cnt;
     std::vector<int64_t> off;
     int64_t total = 0;
@@ -4332,9 +4346,9 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
         RayPaths pp{ c->fr_path, c->fr_pn, c->fr_pbox, pcap };
         const int pgrid = (nr + 63) / 64;
         if (c->sl.prec == FMM3D_PREC_FP64)
-            k_ray_paths<double><<<pgrid, 64, 0, c->stream>>>(g, (const double*)c->sl.T, c->sl.toff, nullptr, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos);
+            k_ray_paths<double><<<pgrid, 64, 0, c->stream>>>(g, (const double*)c->sl.T, c->sl.toff, nullptr, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos, dslot);
         else
-            k_ray_paths<float><<<pgrid, 64, 0, c->stream>>>(g, (const float*)c->sl.T, c->sl.toff, c->sl.ref, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos);
+            k_ray_paths<float><<<pgrid, 64, 0, c->stream>>>(g, (const float*)c->sl.T, c->sl.toff, c->sl.ref, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos, dslot);
         e = hipGetLastError();
         if (e == hipSuccess) {
             const int grid = (int)std::min<size_t>(workers, (size_t)nr);
@@ -4758,23 +4772,32 @@ int fmm3d_engine_receivers(fmm3d_ctx* c, int slot, int nrc, const double* rcv, f
 
 // One slot's rays and Frechet rows for nrc receivers, into host vectors
 // (rp: nrc+1 offsets); capacity grows to the exact size on a second try.
-int fmm3d_engine_frechet(fmm3d_ctx* c, int slot, int nrc, const double* rcv, std::vector<int64_t>& rp,
-                         std::vector<int32_t>& cols, std::vector<float>& vals)
+int fmm3d_engine_frechet_many(fmm3d_ctx* c, int n, const int* slots, const int* nrc, const double* const* rcv,
+                              std::vector<int64_t>& rp, std::vector<int32_t>& cols, std::vector<float>& vals)
 {
-    if (!c || slot < 0 || slot >= c->sl.cap || nrc < 0) return fail(c, FMM3D_E_ARG, "engine frechet: bad arguments");
-    rp.assign((size_t)nrc + 1, 0);
-    int64_t cap = std::max<int64_t>((int64_t)cols.capacity(), (int64_t)nrc * 512 + 1024);
+    if (!c || n < 0) return fail(c, FMM3D_E_ARG, "engine frechet: bad arguments");
+    std::vector<int> islot;
+    std::vector<double> rc;
+    for (int k = 0; k < n; ++k) {
+        if (slots[k] < 0 || slots[k] >= c->sl.cap || nrc[k] < 0) return fail(c, FMM3D_E_ARG, "engine frechet: bad slot");
+        islot.insert(islot.end(), (size_t)nrc[k], slots[k]);
+        rc.insert(rc.end(), rcv[k], rcv[k] + 3 * (size_t)nrc[k]);
+    }
+    const int nrays = (int)islot.size();
+    rp.assign((size_t)nrays + 1, 0);
+    if (nrays == 0) { cols.clear(); vals.clear(); return FMM3D_OK; }
+    int64_t cap = std::max<int64_t>((int64_t)cols.capacity(), (int64_t)nrays * 512 + 1024);
     for (int attempt = 0; attempt < 2; ++attempt) {
         cols.resize(cap);
         vals.resize(cap);
-        const int rc = frechet_range(c, nrc, rcv, slot * nrc, (slot + 1) * nrc, rp.data(), cols.data(), vals.data(), cap);
-        if (rc == FMM3D_OK) {
-            cols.resize(rp[nrc]);
-            vals.resize(rp[nrc]);
+        const int r = frechet_range(c, nrays, rc.data(), 0, nrays, rp.data(), cols.data(), vals.data(), cap, islot.data());
+        if (r == FMM3D_OK) {
+            cols.resize(rp[nrays]);
+            vals.resize(rp[nrays]);
             return FMM3D_OK;
         }
-        if (rc != FMM3D_E_CAPACITY) return rc;
-        cap = rp[nrc];
+        if (r != FMM3D_E_CAPACITY) return r;
+        cap = rp[nrays];
     }
     return fail(c, FMM3D_E_CAPACITY, "engine frechet: capacity retry failed");
 }

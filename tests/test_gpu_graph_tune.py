@@ -29,6 +29,27 @@ def test_graph_replays_match_direct_launches():
                 c.receivers(gd.rcv)
 
 
+def test_graph_follows_the_check_interval():
+    """One context changing its check interval between solves: each solve
+    replays a graph of its own chunk length (a graph captured for 2 sweeps
+    replayed for a 4-sweep chunk would skip sweeps); fields equal those of
+    direct launches."""
+    gd = G.load("tele_41x45x39_mixed")
+    with lib.Context(gd.g) as d, lib.Context(gd.g) as c:
+        d.set_model(gd.velv)
+        c.set_model(gd.velv)
+        d.set_option("graph", 0)
+        c.set_option("graph", 1)
+        for every in (2, 4, 2, 3):
+            for prec, tol in ((lib.PREC_FP64, 1e-7), (lib.PREC_FP32, 1e-3)):
+                c.set_option("check_every", every)
+                d.set_option("check_every", every)
+                d.solve_tele(gd.faces, gd.fom, prec)
+                c.solve_tele(gd.faces, gd.fom, prec)
+                for s in range(gd.faces.shape[0]):
+                    assert float(np.abs(c.field(s) - d.field(s)).max()) <= tol, (every, prec, s)
+
+
 def _tune_case(n=129, nsrc=8, seed=3):
     g = synth.make_grid((n, n, n), 600.0, (15.0, 15.0), nv=(20, 20, 20))
     velv = synth.make_velv(g, 0.05, seed)
