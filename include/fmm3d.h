@@ -289,6 +289,19 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *                   with such a node and their neighbours (teleseismic,
  *                   flow-ordered sources); 0: not; -1 (default): in float32
  *                   only
+ *   "tele_reference" float32 teleseismic solves: 1 (default) store each
+ *                   node against the source's plane-wave reference; 0 the
+ *                   plain offset from the boundary minimum (FMM3D_TELE_REF)
+ *   "tol_floor"     fields with a reference: a change counts for the sweep
+ *                   stop above tol_fp32 * max(|value|, tol_floor) (default
+ *                   64 s; FMM3D_TOL_FLOOR)
+ *   "ref_qbits"     fields with a reference: updated values rounded to
+ *                   multiples of 2^-ref_qbits s (20..30, default 22; 0 not;
+ *                   FMM3D_REF_QBITS)
+ *   "pass_tol"      fields with a reference: a layer's line passes stop
+ *                   after one that moves no node by more than this many
+ *                   seconds, and the fixed-point check counts only such
+ *                   moves (0..1e-3, default 1e-6; 0 exact; FMM3D_PASS_TOL)
  * Returns FMM3D_E_ARG for an unknown key or a value out of range. */
 int fmm3d_set_option(fmm3d_ctx *ctx, const char *key, double value);
 

@@ -434,6 +434,8 @@ template <typename R> struct ColArgs {
     const double* ref;      // float32 fields: per slot nnr + 2 reference values (A[ir], gx, gz), or null
     double tfloor;          // slots with a reference: a change counts above tol * max(|value|, tfloor)
     float qscale;           // slots with a reference: updated values are rounded to multiples of 1/qscale (0: not)
+    float ptol;             // slots with a reference: a layer's passes stop after one that moves no node by
+                            // more (s), and the fixed-point check counts only such moves (0: exact)
 };
 
 // Reference value of node (ir, ix, iz) from a slot's reference row
@@ -454,6 +456,14 @@ template <typename R> __device__ __forceinline__ R ref_round(R v, R qs)
     return qs != R(0) ? rint(v * qs) * (R(1) / qs) : v;
 }
 
+// A value every lane of a wave reads at the same address, from memory no
+// kernel writes while it runs: read through the constant address space, so
+// the load is a scalar one and the value takes no vector registers.
+template <typename P> __device__ __forceinline__ P uniform_load(const P* p, size_t i)
+{
+    return ((const __attribute__((address_space(4))) P*)p)[i];
+}
+
 // Steps of a slot's reference from node (ir, ix, iz) to its stencil
 // neighbours: a float32 neighbour value plus its step is the neighbour's
 // time relative to the centre node's reference.  Every step is a multiple
@@ -466,15 +476,30 @@ template <typename R> struct RefSteps {
     {
         if (std::is_same<R, float>::value && rs) { gx = (R)rs[nnr]; gz = (R)rs[nnr + 1]; }
     }
-    __device__ __forceinline__ void layer(const double* rs, int ir, int snr)
+    // layer ir in two steps, so that other loads can be issued between
+    // them: fetch issues five independent loads at indices inside the row
+    // (no branch between them: one memory round trip), settle forms the steps
+    double f0 = 0.0, fm1 = 0.0, fm2 = 0.0, fp1 = 0.0, fp2 = 0.0;
+    __device__ __forceinline__ void fetch(const double* rs, int ir, int snr)
     {
         if (std::is_same<R, float>::value && rs) {
-            const double a0 = rs[ir];
-            up1 = ir >= 1 ? (R)(rs[ir - 1] - a0) : R(0);
-            up2 = ir >= 2 ? (R)(rs[ir - 2] - a0) : R(0);
-            dn1 = ir + 1 < snr ? (R)(rs[ir + 1] - a0) : R(0);
-            dn2 = ir + 2 < snr ? (R)(rs[ir + 2] - a0) : R(0);
+            f0 = uniform_load(rs, ir); fm1 = uniform_load(rs, max(ir - 1, 0)); fm2 = uniform_load(rs, max(ir - 2, 0));
+            fp1 = uniform_load(rs, min(ir + 1, snr - 1)); fp2 = uniform_load(rs, min(ir + 2, snr - 1));
         }
+    }
+    __device__ __forceinline__ void settle(const double* rs, int ir, int snr)
+    {
+        if (std::is_same<R, float>::value && rs) {
+            up1 = ir >= 1 ? (R)(fm1 - f0) : R(0);
+            up2 = ir >= 2 ? (R)(fm2 - f0) : R(0);
+            dn1 = ir + 1 < snr ? (R)(fp1 - f0) : R(0);
+            dn2 = ir + 2 < snr ? (R)(fp2 - f0) : R(0);
+        }
+    }
+    __device__ __forceinline__ void layer(const double* rs, int ir, int snr)
+    {
+        fetch(rs, ir, snr);
+        settle(rs, ir, snr);
     }
     // v1/v2 in side order r-, r+, x-, x+, z-, z+
     __device__ __forceinline__ void apply(R v1[6], R v2[6]) const
@@ -600,6 +625,7 @@ k_column(ColArgs<R> a)
 {
     constexpr int NP = PX * PZ, S = NP / 64;
     static_assert(NP % 64 == 0 && PX % S == 0 && PZ % S == 0, "segments must tile the patch");
+    static_assert(PX <= 64, "one lane per theta row of the patch stages its spacing");
     constexpr int LX = PX + 4, LZ = (PZ + 4) | 1;   // odd row stride: phi passes spread over banks
     __shared__ R cur[LX * LZ];
     __shared__ R b1[NP], b2[NP], u1[NP], u2[NP], sl[NP];
@@ -711,6 +737,21 @@ k_column(ColArgs<R> a)
     const R inf = Lim<R>::inf();
     const int x0 = bx * PX, z0 = bz * PZ;
     const R hr = a.hr, ihr = R(1) / a.hr;
+    // the lateral halo ring (two nodes deep round the patch): this lane's
+    // entries as an LDS index and an offset within a layer (-1: outside the
+    // grid or beyond the ring).  Ring index i: two rows above the patch, two
+    // below, then the two columns each side of its PX rows.
+    constexpr int HW = PZ + 4, RING = 4 * (PX + PZ) + 16, NR = (RING + 63) / 64;
+    auto ring = [&](int k, int& idx, int& off) {
+        const int i = t + 64 * k;
+        int hx_ = 0, hz_ = 0;
+        if (i < 2 * HW) { hx_ = i / HW; hz_ = i % HW; }
+        else if (i < 4 * HW) { hx_ = PX + 2 + (i - 2 * HW) / HW; hz_ = (i - 2 * HW) % HW; }
+        else { const int j = i - 4 * HW, m = j & 3; hx_ = 2 + (j >> 2); hz_ = m < 2 ? m : PZ + m; }
+        const int gx = x0 + hx_ - 2, gz = z0 + hz_ - 2;
+        idx = i < RING ? hx_ * LZ + hz_ : -1;
+        off = (i < RING && gx >= 0 && gx < snx && gz >= 0 && gz < snz) ? gx * a.nnz + gz : -1;
+    };
     const double* rs = a.ref ? a.ref + (size_t)slot * (a.nnr + 2) : nullptr;
     RefSteps<R> rst;
     rst.lateral(rs, a.nnr);
@@ -720,8 +761,9 @@ k_column(ColArgs<R> a)
     const bool hasref = rs && rs[0] != 0.0;
     const double tfloor = hasref ? a.tfloor : 0.0;
     const R rq = hasref ? (R)a.qscale : R(0);
+    const R ptl = hasref ? (R)a.ptol : R(0);
     bool anyv = false;
-    unsigned nupd = 0, nvis = 0;
+    unsigned long long nupd = 0, nvis = 0;   // wave-uniform counts (scalar registers)
     R orig[S];
     // walk: bottom to top, or top to bottom when a.down; "prev" layers are the
     // two this block relaxed just before (b1, b2), "next" ones come from HBM (u1, u2)
@@ -763,7 +805,23 @@ k_column(ColArgs<R> a)
             __syncthreads();
             wx = s_act & 1; wz = s_act & 2;
         }
-        // stage: interior (own values kept for the write-back), above, slowness
+        // stage.  Every load of the layer is issued before the first LDS
+        // write needs one, so the stage costs one memory round trip: the halo
+        // ring, then the interior (own values kept for the write-back), the
+        // two layers ahead, fixed bits and slowness (read whether or not the
+        // node is fixed, at an index inside the layer).  Neighbours' values
+        // are device-coherent loads: another XCD may have written them this
+        // sweep.
+        R hv[NR];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            int idx, off;
+            ring(k, idx, off);
+            hv[k] = off >= 0 ? __hip_atomic_load(T + lay + off, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : inf;
+        }
+        const R hzv = t < PX ? hzs[(size_t)ir * a.nnx + min(x0 + t, snx - 1)] : R(1);
+        const R hxl = uniform_load(hxs, ir);
+        rst.fetch(rs, ir, snr);
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             const int e = t + 64 * k;
@@ -772,47 +830,48 @@ k_column(ColArgs<R> a)
             const bool in = gx < snx && gz < snz;
             const size_t c = lay + (size_t)gx * a.nnz + gz;
             const R v = in ? T[c] : inf;
-            orig[k] = v;
-            cur[(lx + 2) * LZ + lz + 2] = v;
             const int n1 = ir + dn, n2 = ir + 2 * dn;
-            u1[e] = (in && n1 >= 0 && n1 < snr) ? T[c + (ptrdiff_t)dn * (ptrdiff_t)NL] : inf;
-            u2[e] = (in && n2 >= 0 && n2 < snr) ? T[c + (ptrdiff_t)(2 * dn) * (ptrdiff_t)NL] : inf;
-            const bool fixed = !in || ((fb[c >> 5] >> (c & 31)) & 1u);
-            sl[e] = fixed ? R(-1) : slo[c];
-            nfree += fixed ? 0u : 1u;
+            const R w1 = (in && n1 >= 0 && n1 < snr) ? T[c + (ptrdiff_t)dn * (ptrdiff_t)NL] : inf;
+            const R w2 = (in && n2 >= 0 && n2 < snr) ? T[c + (ptrdiff_t)(2 * dn) * (ptrdiff_t)NL] : inf;
+            const uint32_t fw = in ? fb[c >> 5] : 1u;
+            const R sv = slo[in ? c : lay];
             // the walk's first layer: the two before it come from HBM (none at the edge)
             if (q == qs) {
                 b1[e] = (in && q >= 1) ? T[c - (ptrdiff_t)dn * (ptrdiff_t)NL] : inf;
                 b2[e] = (in && q >= 2) ? T[c - (ptrdiff_t)(2 * dn) * (ptrdiff_t)NL] : inf;
             }
+            orig[k] = v;
+            cur[(lx + 2) * LZ + lz + 2] = v;
+            u1[e] = w1;
+            u2[e] = w2;
+            const bool fixed = !in || ((fw >> (c & 31)) & 1u);
+            sl[e] = fixed ? R(-1) : sv;
+            nfree += (unsigned)__popcll(__ballot(!fixed));
         }
-        // lateral halo ring
-        for (int e = t; e < LX * (PZ + 4); e += 64) {
-            const int hx_ = e / (PZ + 4), hz_ = e % (PZ + 4);
-            if (hx_ >= 2 && hx_ < PX + 2 && hz_ >= 2 && hz_ < PZ + 2) continue;
-            const int gx = x0 + hx_ - 2, gz = z0 + hz_ - 2;
-            const bool in = gx >= 0 && gx < snx && gz >= 0 && gz < snz;
-            // neighbours' values: device-coherent loads (another XCD may have written them this sweep)
-            cur[hx_ * LZ + hz_] = in ? __hip_atomic_load(T + lay + (size_t)gx * a.nnz + gz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : inf;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            int idx, off;
+            ring(k, idx, off);
+            if (idx >= 0) cur[idx] = hv[k];
         }
-        for (int e = t; e < PX; e += 64) {
-            const int gx = min(x0 + e, snx - 1);
-            const R h = hzs[(size_t)ir * a.nnx + gx];
-            shz[e] = h;
-            sihz[e] = R(1) / h;
+        if (t < PX) {
+            shz[t] = hzv;
+            sihz[t] = R(1) / hzv;
         }
-        const R hxl = hxs[ir], ihx = R(1) / hxl;
-        rst.layer(rs, ir, snr);
+        const R ihx = R(1) / hxl;
+        rst.settle(rs, ir, snr);
         __syncthreads();
         // the passes stop after one that changes nothing: the layer is then
-        // at a fixpoint of the update and later passes would not move it
-        // (stopping at a tolerance instead leaves second-order transients
-        // that keep neighbouring patches oscillating)
+        // at a fixpoint of the update and later passes would not move it.
+        // Slots with a reference stop after one that moves no node by more
+        // than ptl; the fixed-point check after their first sweep counts
+        // only such moves too, or the residuals left below ptl would send
+        // nearly every patch into the next sweep.
         int npass = 0;
 #pragma unroll 1
         for (int pass = 0; pass < a.maxpass; ++pass) {
             const bool zpass = (pass & 3) < 2, rev = pass & 1;
-            bool moved = false;
+            bool moved = false;   // a node moved by more than the pass tolerance
             const int lane_l = zpass ? t / (PZ / S) : t % PZ;          // line (row or column) index
             const int seg0 = (zpass ? t % (PZ / S) : t / PZ) * S;     // segment start along the line
 #pragma unroll 1
@@ -846,7 +905,10 @@ k_column(ColArgs<R> a)
                     const R nv = ref_round(col_update<R, FAST, SHIFT>(a.fom, tc, v1, v2, in1, in2, h, ih, slown), rq);
                     // every lane has read its stencil before any lane writes (one wavefront)
                     __builtin_amdgcn_wave_barrier();
-                    if (nv != tc) { cur[own] = nv; moved = true; }
+                    if (nv != tc) {
+                        cur[own] = nv;
+                        if (!(fabs(nv - tc) <= ptl)) moved = true;
+                    }
                 }
                 __builtin_amdgcn_wave_barrier();
             }
@@ -893,10 +955,8 @@ k_column(ColArgs<R> a)
         __hip_atomic_store(a.lrng + ((size_t)slot * np + b) * 2 + (iter & 1), lrng_pack(iter, chg_lo, chg_hi),
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     {
-        unsigned long long w = nupd, wv = nvis;
-        for (int off = 32; off > 0; off >>= 1) { w += __shfl_down(w, off, 64); wv += __shfl_down(wv, off, 64); }
-        if (t == 0 && w) atomicAdd(a.updates, w);
-        if (t == 0 && wv) atomicAdd(a.nodes, wv);
+        if (t == 0 && nupd) atomicAdd(a.updates, nupd);
+        if (t == 0 && nvis) atomicAdd(a.nodes, nvis);
     }
     const bool blk = __syncthreads_or(anyv);
     if (t == 0 && blk) {
@@ -944,6 +1004,7 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
         RefSteps<R> rst;
         rst.lateral(rs, a.nnr);
         const R qs = (rs && rs[0] != 0.0) ? (R)a.qscale : R(0);
+        const R vtl = (rs && rs[0] != 0.0) ? (R)a.ptol : R(0);
         bool moved = false;
         // window w[0..4] = layers ir-2 .. ir+2 of this column
         R w[5];
@@ -976,7 +1037,7 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
                 const R h[3] = { hr, hxl, hzl };
                 const R ih[3] = { ihr, R(1) / hxl, R(1) / hzl };
                 const R nv = ref_round(col_update<R, FAST, SHIFT>(a.fom, w[2], v1, v2, in1, in2, h, ih, a.slo[slot * a.geo_n + c]), qs);
-                if (nv != w[2]) moved = true;
+                if (nv != w[2] && !(fabs(nv - w[2]) <= vtl)) moved = true;
             }
             w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
             w[4] = ir + 3 < snr ? T[c + 3 * NL] : inf;
@@ -1857,6 +1918,7 @@ struct fmm3d_ctx {
     int tele_ref = 1;              // float32 teleseismic fields relative to a 1-D plane-wave reference
     double tol_floor = 64.0;       // those fields: a change counts above tol32 * max(|value|, tol_floor)
     int ref_qbits = 22;            // those fields: updates rounded to multiples of 2^-ref_qbits s (0: not)
+    double pass_tol = 1e-6;        // those fields: a layer's passes stop after one that moves no node by more (s)
     std::vector<double> layer_slo; // mean slowness of each layer of the current model
     std::vector<std::pair<std::string, TuneChoice>> tuned;   // tuning key -> configuration (this context)
     int* d_iter = nullptr;         // device sweep counter (graph replays)
@@ -2182,6 +2244,8 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     if (tfl) c->tol_floor = std::max(0.0, atof(tfl));
     const char* rqb = getenv("FMM3D_REF_QBITS");
     if (rqb) c->ref_qbits = std::min(30, std::max(0, atoi(rqb)));
+    const char* ptl = getenv("FMM3D_PASS_TOL");
+    if (ptl) c->pass_tol = std::min(1e-3, std::max(0.0, atof(ptl)));
     const char* gv = getenv("FMM3D_GRAPH");
     if (gv) c->use_graph = atoi(gv) != 0;
     const char* vf = getenv("FMM3D_VERIFY");
@@ -2839,6 +2903,7 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     a.ref = std::is_same<R, float>::value ? s.ref : nullptr;
     a.tfloor = c->tol_floor;
     a.qscale = c->ref_qbits > 0 ? std::ldexp(1.0f, c->ref_qbits) : 0.0f;
+    a.ptol = (float)c->pass_tol;
     a.geo_n = a.geo_hx = a.geo_hz = 0;
     a.sdims = nullptr;
     if (c->geo_cap > 0) {
@@ -4940,6 +5005,9 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "ref_qbits")) {
         if (iv != 0 && (iv < 20 || iv > 30)) return fail(c, FMM3D_E_ARG, "ref_qbits must be 0 or 20..30");
         c->ref_qbits = iv;
+    } else if (!strcmp(key, "pass_tol")) {
+        if (!(v >= 0.0 && v <= 1e-3)) return fail(c, FMM3D_E_ARG, "pass_tol %g out of range", v);
+        c->pass_tol = v;
     } else if (!strcmp(key, "tol_floor")) {
         if (!(v >= 0.0 && v <= 1e4)) return fail(c, FMM3D_E_ARG, "tol_floor %g out of range", v);
         c->tol_floor = v;
