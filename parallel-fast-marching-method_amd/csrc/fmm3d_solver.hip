@@ -436,6 +436,9 @@ template <typename R> struct ColArgs {
     float qscale;           // slots with a reference: updated values are rounded to multiples of 1/qscale (0: not)
     float ptol;             // slots with a reference: a layer's passes stop after one that moves no node by
                             // more (s), and the fixed-point check counts only such moves (0: exact)
+    int* wlost;             // per slot: a wait on a neighbour was given up in a sweep
+    int vfilter;            // the check after a slot's first sweep evaluates only the nodes an input
+                            // relaxed after them could move (below)
 };
 
 // Reference value of node (ir, ix, iz) from a slot's reference row
@@ -763,6 +766,7 @@ k_column(ColArgs<R> a)
     const R rq = hasref ? (R)a.qscale : R(0);
     const R ptl = hasref ? (R)a.ptol : R(0);
     bool anyv = false;
+    bool capped = false;   // a layer's passes hit the cap with nodes still moving
     unsigned long long nupd = 0, nvis = 0;   // wave-uniform counts (scalar registers)
     R orig[S];
     // walk: bottom to top, or top to bottom when a.down; "prev" layers are the
@@ -796,6 +800,7 @@ k_column(ColArgs<R> a)
                     if (++spins > a.spin_limit) {
                         if (seen_x < need) wx = false;
                         if (seen_z < need) wz = false;
+                        if (a.wlost) __hip_atomic_store(a.wlost + slot, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(2);
@@ -914,6 +919,7 @@ k_column(ColArgs<R> a)
             }
             ++npass;
             if (!__ballot(moved)) break;
+            if (pass == a.maxpass - 1) capped = true;
         }
         nupd += nfree * (unsigned)npass;
         nvis += nfree;
@@ -951,6 +957,13 @@ k_column(ColArgs<R> a)
     }
     // layers the walk did not reach are finished too
     if (pg && t == 0) __hip_atomic_store(pg + b, pbase + a.nnr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // a layer left short of its fixpoint: the fixed-point check counts the
+    // patch as moving without evaluating it (its filter assumes every layer
+    // reached the fixpoint of the inputs it had)
+    if (capped && t == 0 && a.pend) {
+        a.pend[(size_t)slot * np + b] = 1;
+        a.slotpend[slot] = 1;
+    }
     if (win && t == 0 && chg_hi >= 0)
         __hip_atomic_store(a.lrng + ((size_t)slot * np + b) * 2 + (iter & 1), lrng_pack(iter, chg_lo, chg_hi),
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1005,6 +1018,39 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
         rst.lateral(rs, a.nnr);
         const R qs = (rs && rs[0] != 0.0) ? (R)a.qscale : R(0);
         const R vtl = (rs && rs[0] != 0.0) ? (R)a.ptol : R(0);
+        // After a slot's first sweep every patch walked every layer, each to
+        // the fixpoint of the inputs it had then (a layer that hit the pass
+        // cap marked its patch in the walk).  A node can then move only
+        // through an input relaxed after it -- the two layers walked after
+        // its own, and lateral neighbours outside its patch, except those in
+        // the patch it waited for on that axis -- and only if that input is
+        // not above it (the update counts no neighbour above its result).
+        // Nodes with no such input are not evaluated.  late_x/late_z: per
+        // offset -1, -2, +1, +2, whether that input can be one.
+        const bool vf = a.vfilter && (a.born || iter == 1);
+        const bool down = a.walkalt ? false : (a.down != 0);
+        bool late_x[4] = { false, false, false, false }, late_z[4] = { false, false, false, false };
+        if (vf) {
+            int fx = 0, fz = 0;   // the waited-for patch on each axis is at patch index - f (0: none)
+            const int pxo = ix / a.px, pzo = iz / a.pz;
+            if (a.prog && (a.ordered_until == 0 || iter <= a.ordered_until) && !a.wlost[slot]) {
+                const int f0 = a.flow ? a.flow[2 * slot] : 1, f1 = a.flow ? a.flow[2 * slot + 1] : 1;
+                if (f0 >= FLOW_CENTER) {
+                    const int sx = min((f0 - FLOW_CENTER) / a.px, a.npx - 1), sz = min((f1 - FLOW_CENTER) / a.pz, a.npz - 1);
+                    fx = pxo > sx ? 1 : (pxo < sx ? -1 : 0);
+                    fz = pzo > sz ? 1 : (pzo < sz ? -1 : 0);
+                } else {
+                    fx = f0 < 0 ? -1 : 1;
+                    fz = f1 < 0 ? -1 : 1;
+                }
+            }
+            const int off[4] = { -1, -2, 1, 2 };
+            for (int k = 0; k < 4; ++k) {
+                const int jx = ix + off[k], jz = iz + off[k];
+                late_x[k] = jx >= 0 && jx / a.px != pxo && !(fx != 0 && jx / a.px == pxo - fx);
+                late_z[k] = jz >= 0 && jz / a.pz != pzo && !(fz != 0 && jz / a.pz == pzo - fz);
+            }
+        }
         bool moved = false;
         // window w[0..4] = layers ir-2 .. ir+2 of this column
         R w[5];
@@ -1033,11 +1079,23 @@ __global__ void __launch_bounds__(256) k_verify_mark(ColArgs<R> a)
                 v1[5] = in1[5] ? T[c + 1] : inf; v2[5] = in2[5] ? T[c + 2] : inf;
                 rst.layer(rs, ir, snr);
                 rst.apply(v1, v2);
-                const R hxl = a.hx[slot * a.geo_hx + ir], hzl = a.hz[slot * a.geo_hz + (size_t)ir * a.nnx + ix];
-                const R h[3] = { hr, hxl, hzl };
-                const R ih[3] = { ihr, R(1) / hxl, R(1) / hzl };
-                const R nv = ref_round(col_update<R, FAST, SHIFT>(a.fom, w[2], v1, v2, in1, in2, h, ih, a.slo[slot * a.geo_n + c]), qs);
-                if (nv != w[2] && !(fabs(nv - w[2]) <= vtl)) moved = true;
+                // the lowest input relaxed after the node, in its frame
+                R lm = down ? fmin(v1[1], v2[1]) : fmin(v1[0], v2[0]);
+                if (late_x[0]) lm = fmin(lm, v1[2]);
+                if (late_x[1]) lm = fmin(lm, v2[2]);
+                if (late_x[2]) lm = fmin(lm, v1[3]);
+                if (late_x[3]) lm = fmin(lm, v2[3]);
+                if (late_z[0]) lm = fmin(lm, v1[4]);
+                if (late_z[1]) lm = fmin(lm, v2[4]);
+                if (late_z[2]) lm = fmin(lm, v1[5]);
+                if (late_z[3]) lm = fmin(lm, v2[5]);
+                if (!vf || lm <= w[2]) {
+                    const R hxl = a.hx[slot * a.geo_hx + ir], hzl = a.hz[slot * a.geo_hz + (size_t)ir * a.nnx + ix];
+                    const R h[3] = { hr, hxl, hzl };
+                    const R ih[3] = { ihr, R(1) / hxl, R(1) / hzl };
+                    const R nv = ref_round(col_update<R, FAST, SHIFT>(a.fom, w[2], v1, v2, in1, in2, h, ih, a.slo[slot * a.geo_n + c]), qs);
+                    if (nv != w[2] && !(fabs(nv - w[2]) <= vtl)) moved = true;
+                }
             }
             w[0] = w[1]; w[1] = w[2]; w[2] = w[3]; w[3] = w[4];
             w[4] = ir + 3 < snr ? T[c + 3 * NL] : inf;
@@ -1853,6 +1911,7 @@ struct Slots {
     double* srcpos = nullptr; // per slot point-source position (height, colat, lon), NaN if none
     int* walkalt = nullptr;   // per slot: alternate the column walk direction
     int* pend = nullptr;      // column walk: per slot per patch, a node would move (fixed-point check)
+    int* wlost = nullptr;     // column walk: per slot, a wait on a neighbour patch was given up
     unsigned long long* lrng = nullptr;   // column walk, alternating slots: per slot per patch, changed layers of two sweeps
     int* slotpend = nullptr;  // column walk: per slot, some node would move
     int* born = nullptr;      // serving engine: per slot, the sweep it joined at
@@ -1912,6 +1971,7 @@ struct fmm3d_ctx {
     // only; in float64 the narrower sweep slowed propagation, 6 -> 17 sweeps
     // on 50 plane waves at 201^3)
     int verify_patches = -1;
+    int verify_filter = 1;         // the check after a first sweep skips nodes no later input can move
     int autotune = 1;              // time candidate configurations at the first solve of a kind
     bool tune_fixed = false;       // the caller chose the strategy, patch, pass cap or check interval
     std::string tune_arch;         // device identity for the tuning cache (arch name / compute units)
@@ -2246,6 +2306,8 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     if (rqb) c->ref_qbits = std::min(30, std::max(0, atoi(rqb)));
     const char* ptl = getenv("FMM3D_PASS_TOL");
     if (ptl) c->pass_tol = std::min(1e-3, std::max(0.0, atof(ptl)));
+    const char* vfl = getenv("FMM3D_VERIFY_FILTER");
+    if (vfl) c->verify_filter = atoi(vfl) != 0;
     const char* gv = getenv("FMM3D_GRAPH");
     if (gv) c->use_graph = atoi(gv) != 0;
     const char* vf = getenv("FMM3D_VERIFY");
@@ -2279,7 +2341,7 @@ static void free_slots(Slots& s)
     (void)hipFree(s.T); (void)hipFree(s.P); (void)hipFree(s.fix); (void)hipFree(s.stamp);
     (void)hipFree(s.lastchg); (void)hipFree(s.on); (void)hipFree(s.toff); (void)hipFree(s.ref);
     (void)hipFree(s.valchg); (void)hipFree(s.prog); (void)hipFree(s.flow); (void)hipFree(s.srcpos); (void)hipFree(s.walkalt);
-    (void)hipFree(s.pend); (void)hipFree(s.slotpend); (void)hipFree(s.born); (void)hipFree(s.lrng);
+    (void)hipFree(s.pend); (void)hipFree(s.wlost); (void)hipFree(s.slotpend); (void)hipFree(s.born); (void)hipFree(s.lrng);
     s = Slots();
 }
 
@@ -2411,6 +2473,8 @@ static int ensure_slots(fmm3d_ctx* c, int nsrc, int prec)
     HIPCHK(c, dalloc(&s.srcpos, (size_t)nsrc * 3 * sizeof(double)));
     HIPCHK(c, dalloc(&s.walkalt, (size_t)nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.pend, (size_t)nsrc * nbmax * sizeof(int)));
+    HIPCHK(c, dalloc(&s.wlost, (size_t)nsrc * sizeof(int)));
+    HIPCHK(c, hipMemset(s.wlost, 0, (size_t)nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.lrng, (size_t)nsrc * nbmax * 2 * sizeof(unsigned long long)));
     HIPCHK(c, dalloc(&s.slotpend, (size_t)nsrc * sizeof(int)));
     HIPCHK(c, dalloc(&s.born, (size_t)nsrc * sizeof(int)));
@@ -2893,6 +2957,8 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     a.verify_patches = c->verify_patches < 0 ? !std::is_same<R, double>::value : c->verify_patches;
     a.pend = s.pend;
     a.slotpend = s.slotpend;
+    a.wlost = s.wlost;
+    a.vfilter = c->verify_filter;
     a.born = nullptr;
     // float64, and the float32 warm-up solves of refined blocks (float64
     // sweeps follow them): float32 walks count a change only above their
@@ -3060,6 +3126,7 @@ static int run_column(fmm3d_ctx* c, int nsrc, int order, int* sweeps_out)
     k_lrng_init<<<(nsrc * np + 255) / 256, 256, 0, c->stream>>>(s.lrng, (size_t)nsrc * np, 1);
     k_fill_int<<<(nsrc * np + 255) / 256, 256, 0, c->stream>>>(s.pend, nsrc * np, 0);
     k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.slotpend, nsrc, 0);
+    k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.wlost, nsrc, 0);
     k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.lastchg, nsrc, 0);
     k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.on, nsrc, 1);
     k_fill_int<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(s.valchg, nsrc, 0);
@@ -4628,6 +4695,7 @@ int fmm3d_engine_admit(fmm3d_ctx* c, int slot, const int64_t* node, const double
         k_fill_int<<<1, 1, 0, c->stream>>>(s.on + slot, 1, 1);
         k_fill_int<<<1, 1, 0, c->stream>>>(s.born + slot, 1, c->eng_k);
         k_fill_int<<<1, 1, 0, c->stream>>>(s.slotpend + slot, 1, 0);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.wlost + slot, 1, 0);
         k_fill_int<<<(np + 255) / 256, 256, 0, c->stream>>>(s.pend + (size_t)slot * np, np, 0);
         e = hipGetLastError();
     }
@@ -4699,6 +4767,7 @@ int fmm3d_engine_admit_face(fmm3d_ctx* c, int slot, const double* face)
         // column of the new slot runs in the next sweep
         k_slot_setup<<<(np + 255) / 256, 256, 0, c->stream>>>(slot, np, c->eng_k, 0, s.stamp, s.prog, s.pend, s.lastchg,
                                                              s.walkalt, s.born, s.slotpend, s.on, s.srcpos, s.lrng);
+        k_fill_int<<<1, 1, 0, c->stream>>>(s.wlost + slot, 1, 0);
         e = hipGetLastError();
     }
     if (e != hipSuccess) return fail(c, FMM3D_E_DEVICE, "engine admit: %s", hipGetErrorString(e));
@@ -4984,6 +5053,9 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "ordered_sweeps")) {
         if (iv < 0 || iv > 1000000) return fail(c, FMM3D_E_ARG, "ordered_sweeps out of range");
         c->ordered_until = iv;
+    } else if (!strcmp(key, "verify_filter")) {
+        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "verify_filter must be 0 or 1");
+        c->verify_filter = iv;
     } else if (!strcmp(key, "verify_patches")) {
         if (iv < -1 || iv > 1) return fail(c, FMM3D_E_ARG, "verify_patches must be -1, 0 or 1");
         c->verify_patches = iv;

@@ -139,13 +139,15 @@ def test_check_interval_and_per_source_exit(every):
             c.set_option("check_every", 9)
 
 
-@pytest.mark.parametrize("verify,narrow", [(1, 0), (1, 1), (2, -1)])
-def test_fixed_point_check_keeps_the_fields(verify, narrow):
+@pytest.mark.parametrize("verify,narrow,filt", [(1, 0, 1), (1, 1, 1), (1, 1, 0), (2, -1, 1)])
+def test_fixed_point_check_keeps_the_fields(verify, narrow, filt):
     """The read-only fixed-point check after sweep `verify` (sources with no
     node that would move leave the sweeps; with narrowing the next sweep
-    relaxes only the moving patches) reaches the fields of the sweeps
-    without it: float64 within 1e-9 s, float32 within 1 ms; plane waves with
-    side faces and a point source in one batch."""
+    relaxes only the moving patches; with the filter, after a first sweep,
+    only nodes an input relaxed after them could move are evaluated) reaches
+    the fields of the sweeps without it: float64 within 1e-9 s, float32
+    within 1 ms; plane waves with side faces and a point source in one
+    batch."""
     g, velv, tele, rcv = synth.tele_case((41, 45, 39), 500.0, (12.0, 11.0), (9, 10, 11), 0.08, 5, 20, 7)
     parts = [synth.tele_faces(g, s) for s in tele]
     F, X, Z, S = (np.stack([p[k] for p in parts]) for k in range(4))
@@ -156,10 +158,13 @@ def test_fixed_point_check_keeps_the_fields(verify, narrow):
         d.set_option("verify", 0)
         c.set_option("verify", verify)
         c.set_option("verify_patches", narrow)
+        c.set_option("verify_filter", filt)
         for prec, tol in ((lib.PREC_FP64, 1e-9), (lib.PREC_FP32, 1e-3)):
             d.solve_mixed(F, (X, Z, S), pts, np.array([4]), 5, lib.ORDER_MIXED, prec)
             c.solve_mixed(F, (X, Z, S), pts, np.array([4]), 5, lib.ORDER_MIXED, prec)
             for s in range(6):
-                assert float(np.abs(c.field(s) - d.field(s)).max()) <= tol, (verify, narrow, prec, s)
+                assert float(np.abs(c.field(s) - d.field(s)).max()) <= tol, (verify, narrow, filt, prec, s)
         with pytest.raises(lib.FmmError):
             c.set_option("verify_patches", 2)
+        with pytest.raises(lib.FmmError):
+            c.set_option("verify_filter", 2)
