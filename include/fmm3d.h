@@ -289,6 +289,12 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *                   with such a node and their neighbours (teleseismic,
  *                   flow-ordered sources); 0: not; -1 (default): in float32
  *                   only
+ *   "verify_filter" 1 (default): the check after a source's first sweep
+ *                   evaluates only nodes that an input relaxed after them
+ *                   (the layers walked later, lateral neighbours outside
+ *                   the patch not waited for) could move, i.e. where such
+ *                   an input is not above the node; 0 every node
+ *                   (FMM3D_VERIFY_FILTER)
  *   "tele_reference" float32 teleseismic solves: 1 (default) store each
  *                   node against the source's plane-wave reference; 0 the
  *                   plain offset from the boundary minimum (FMM3D_TELE_REF)

@@ -2002,6 +2002,10 @@ struct fmm3d_ctx {
     float* pin_vals = nullptr;
     size_t pin_rows = 0;           // entries pin_rp holds
     int64_t pin_cap = 0;           // entries pin_cols / pin_vals hold
+    int32_t* pin_dcols = nullptr;  // the device's addresses of pin_cols / pin_vals
+    float* pin_dvals = nullptr;
+    int32_t* pin_cnt = nullptr;    // Frechet chunks: row entry counts (pinned host)
+    size_t pin_cnt_n = 0;
     double* d_faces = nullptr;     // boundary faces of the last teleseismic solve
     size_t d_faces_n = 0;
     // staging of fixed node sets (grow only: a hipFree would wait for every
@@ -2358,6 +2362,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     if (c->pin_rp) (void)hipHostFree(c->pin_rp);
     if (c->pin_cols) (void)hipHostFree(c->pin_cols);
     if (c->pin_vals) (void)hipHostFree(c->pin_vals);
+    if (c->pin_cnt) (void)hipHostFree(c->pin_cnt);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     (void)hipFree(c->d_iter);
@@ -2696,16 +2701,42 @@ __global__ void k_stamp_init(int nslots, int nb, int nbxz, int nbr, int all, int
     }
 }
 
+// Offsets of a chunk's rows in the packed array: the exclusive prefix sum
+// of the entry counts (a negative count -- a row to redo -- takes none).
+// One block of 1024 threads, each summing a run of counts.
+__global__ void __launch_bounds__(1024) k_row_offsets(int nr, const int32_t* __restrict__ count, int64_t* __restrict__ off)
+{
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x, per = (nr + 1023) / 1024;
+    const int q0 = min(nr, t * per), q1 = min(nr, q0 + per);
+    int64_t run = 0;
+    for (int q = q0; q < q1; ++q) run += max(count[q], 0);
+    part[t] = run;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {   // inclusive scan of the runs
+        const int64_t v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t o = part[t] - run;
+    for (int q = q0; q < q1; ++q) {
+        off[q] = o;
+        o += max(count[q], 0);
+    }
+}
+
 // Frechet rows from the per-ray staging into one packed array: ray q's
 // count[q] entries move to off[q] (one block per ray, grid-stride).
 __global__ void k_pack_rows(int nr, int64_t rcap, const int32_t* __restrict__ count, const int64_t* __restrict__ off,
                             const int32_t* __restrict__ scols, const float* __restrict__ svals,
-                            int32_t* __restrict__ pcols, float* __restrict__ pvals)
+                            int32_t* __restrict__ pcols, float* __restrict__ pvals, int64_t room)
 {
     for (int q = blockIdx.x; q < nr; q += gridDim.x) {
         const int n = count[q];
         if (n <= 0) continue;
         const int64_t src = (int64_t)q * rcap, dst = off[q];
+        if (dst + n > room) continue;   // past the destination's end: the caller sees the total and retries
         for (int i = threadIdx.x; i < n; i += blockDim.x) {
             pcols[dst + i] = scols[src + i];
             pvals[dst + i] = svals[src + i];
@@ -4394,7 +4425,8 @@ extern "C" int fmm3d_receivers_noalloc(fmm3d_ctx* c, int nrc, const double* rcv,
 // Rays ray_begin .. ray_end-1 (ray slot*nrc + i: slot's field, receiver i)
 // with their Frechet rows; row_ptr has ray_end-ray_begin+1 entries.
 static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin, int ray_end, int64_t* row_ptr,
-                         int32_t* cols, float* vals, int64_t cap, const int* islot = nullptr)
+                         int32_t* cols, float* vals, int64_t cap, const int* islot = nullptr,
+                         int32_t* dcols = nullptr, float* dvals = nullptr)
 {
     HIPCHK(c, hipSetDevice(c->device));
     const Grid& g = c->g;
@@ -4414,11 +4446,14 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
     }
     // Rays go in chunks whose staging (fr_rcap entries per ray) and path
     // samples (fr_pcap per ray) fit fixed budgets.  Per chunk: paths, one
-    // lane per ray -> Frechet rows, one wavefront per ray -> counts to the
-    // host -> row offsets -> rows packed on the device -> two copies into
-    // the caller's arrays.  A chunk with a row longer than the staging runs
-    // again with room for the whole vertex grid, one with a path longer than
-    // its samples with four times the samples.
+    // lane per ray -> Frechet rows, one wavefront per ray -> row offsets on
+    // the device -> counts to the host.  With device-visible destination
+    // arrays (dcols, dvals: pinned host memory) the rows are packed straight
+    // into them before the counts come back; otherwise they are packed on
+    // the device and copied into the caller's arrays.  A chunk with a row
+    // longer than the staging runs again with room for the whole vertex
+    // grid, one with a path longer than its samples with four times the
+    // samples.
     const size_t budget = (size_t)512 << 20;   // bytes of staging
     const size_t pbudget = (size_t)2 << 30;    // bytes of path samples
     if (c->fr_pcap <= 0) c->fr_pcap = 2 * ((int64_t)g.nnr + g.nnx + g.nnz) + 64;
@@ -4433,8 +4468,6 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
         dslot = c->st_islot;
         if (e == hipSuccess) e = hipMemcpyAsync(dslot, islot, (size_t)nrays * sizeof(int), hipMemcpyHostToDevice, c->stream);
     }
-    std::vector<int32_t> cnt;
-    std::vector<int64_t> off;
     int64_t total = 0;
     bool short_cap = false;
     for (int r0 = ray_begin; r0 < ray_end && e == hipSuccess;) {
@@ -4487,10 +4520,25 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
             k_ray_frechet<<<grid, 64, 0, c->stream>>>(g, nr, pp, c->ray_scratch, o);
             e = hipGetLastError();
         }
-        cnt.resize(nr);
-        if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), c->fr_count, nr * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) {
+            k_row_offsets<<<1, 1024, 0, c->stream>>>(nr, c->fr_count, c->fr_off);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && dcols && !short_cap) {
+            k_pack_rows<<<std::min(nr, 8192), 256, 0, c->stream>>>(nr, rcap, c->fr_count, c->fr_off, c->fr_cols, c->fr_vals,
+                                                                  dcols + total, dvals + total, cap - total);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && c->pin_cnt_n < (size_t)nr) {
+            if (c->pin_cnt) (void)hipHostFree(c->pin_cnt);
+            c->pin_cnt = nullptr; c->pin_cnt_n = 0;
+            e = hipHostMalloc((void**)&c->pin_cnt, (size_t)nr * sizeof(int32_t), hipHostMallocDefault);
+            if (e == hipSuccess) c->pin_cnt_n = nr;
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(c->pin_cnt, c->fr_count, nr * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) break;
+        const int32_t* cnt = c->pin_cnt;
         bool over = false, pover = false;
         for (int q = 0; q < nr; ++q) { over |= cnt[q] == -2; pover |= cnt[q] == -3; }
         if (pover) {   // a path longer than its samples: four times the samples (a ray has at most 5e5)
@@ -4503,17 +4551,15 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
             c->fr_rcap = (int64_t)NV;
             continue;
         }
-        off.resize(nr);
         int64_t ctot = 0;
         for (int q = 0; q < nr; ++q) {
             const int64_t n = std::max(cnt[q], 0);
             row_ptr[r0 - ray_begin + q + 1] = row_ptr[r0 - ray_begin + q] + n;
-            off[q] = ctot;
             ctot += n;
         }
         if (short_cap || total + ctot > cap) {
             short_cap = true;
-        } else if (ctot > 0) {
+        } else if (!dcols && ctot > 0) {
             if (c->fr_packed < (size_t)ctot) {
                 (void)hipFree(c->fr_pcols); (void)hipFree(c->fr_pvals);
                 c->fr_pcols = nullptr; c->fr_pvals = nullptr; c->fr_packed = 0;
@@ -4522,12 +4568,9 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
                 if (e != hipSuccess) break;
                 c->fr_packed = ctot;
             }
-            e = hipMemcpyAsync(c->fr_off, off.data(), nr * sizeof(int64_t), hipMemcpyHostToDevice, c->stream);
-            if (e == hipSuccess) {
-                k_pack_rows<<<std::min(nr, 8192), 256, 0, c->stream>>>(nr, rcap, c->fr_count, c->fr_off, c->fr_cols, c->fr_vals,
-                                                                      c->fr_pcols, c->fr_pvals);
-                e = hipGetLastError();
-            }
+            k_pack_rows<<<std::min(nr, 8192), 256, 0, c->stream>>>(nr, rcap, c->fr_count, c->fr_off, c->fr_cols, c->fr_vals,
+                                                                  c->fr_pcols, c->fr_pvals, ctot);
+            e = hipGetLastError();
             if (e == hipSuccess) e = hipMemcpyAsync(cols + total, c->fr_pcols, ctot * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
             if (e == hipSuccess) e = hipMemcpyAsync(vals + total, c->fr_pvals, ctot * sizeof(float), hipMemcpyDeviceToHost, c->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -4578,8 +4621,12 @@ extern "C" int fmm3d_frechet_pinned(fmm3d_ctx* c, int nrc, const double* rcv, co
             HIPCHK(c, hipHostMalloc((void**)&c->pin_cols, (size_t)want * sizeof(int32_t), hipHostMallocDefault));
             HIPCHK(c, hipHostMalloc((void**)&c->pin_vals, (size_t)want * sizeof(float), hipHostMallocDefault));
             c->pin_cap = want;
+            HIPCHK(c, hipHostGetDevicePointer((void**)&c->pin_dcols, c->pin_cols, 0));
+            HIPCHK(c, hipHostGetDevicePointer((void**)&c->pin_dvals, c->pin_vals, 0));
         }
-        const int rc = fmm3d_frechet(c, nrc, rcv, c->pin_rp, c->pin_cols, c->pin_vals, c->pin_cap);
+        if (!c->have_model) return fail(c, FMM3D_E_STATE, "fmm3d_frechet_pinned: no velocity model");
+        const int rc = frechet_range(c, nrc, rcv, 0, c->sl.n * nrc, c->pin_rp, c->pin_cols, c->pin_vals, c->pin_cap, nullptr,
+                                     c->pin_dcols, c->pin_dvals);
         if (rc == FMM3D_E_CAPACITY) { want = c->pin_rp[rows - 1]; continue; }
         if (rc) return rc;
         *row_ptr = c->pin_rp; *cols = c->pin_cols; *vals = c->pin_vals; *nnz = c->pin_rp[rows - 1];
