@@ -437,6 +437,8 @@ template <typename R> struct ColArgs {
     float ptol;             // slots with a reference: a layer's passes stop after one that moves no node by
                             // more (s), and the fixed-point check counts only such moves (0: exact)
     int* wlost;             // per slot: a wait on a neighbour was given up in a sweep
+    int lguess;             // slots with a reference: a node not yet reached starts its layer's passes
+                            // from the value of the node walked just before it in its column
     int vfilter;            // the check after a slot's first sweep evaluates only the nodes an input
                             // relaxed after them could move (below)
 };
@@ -765,6 +767,7 @@ k_column(ColArgs<R> a)
     const double tfloor = hasref ? a.tfloor : 0.0;
     const R rq = hasref ? (R)a.qscale : R(0);
     const R ptl = hasref ? (R)a.ptol : R(0);
+    const bool lguess = hasref && a.lguess;
     bool anyv = false;
     bool capped = false;   // a layer's passes hit the cap with nodes still moving
     unsigned long long nupd = 0, nvis = 0;   // wave-uniform counts (scalar registers)
@@ -845,11 +848,11 @@ k_column(ColArgs<R> a)
                 b1[e] = (in && q >= 1) ? T[c - (ptrdiff_t)dn * (ptrdiff_t)NL] : inf;
                 b2[e] = (in && q >= 2) ? T[c - (ptrdiff_t)(2 * dn) * (ptrdiff_t)NL] : inf;
             }
+            const bool fixed = !in || ((fw >> (c & 31)) & 1u);
             orig[k] = v;
-            cur[(lx + 2) * LZ + lz + 2] = v;
+            cur[(lx + 2) * LZ + lz + 2] = (lguess && !fixed && !(v < inf) && b1[e] < inf) ? b1[e] : v;
             u1[e] = w1;
             u2[e] = w2;
-            const bool fixed = !in || ((fw >> (c & 31)) & 1u);
             sl[e] = fixed ? R(-1) : sv;
             nfree += (unsigned)__popcll(__ballot(!fixed));
         }
@@ -1972,6 +1975,7 @@ struct fmm3d_ctx {
     // on 50 plane waves at 201^3)
     int verify_patches = -1;
     int verify_filter = 1;         // the check after a first sweep skips nodes no later input can move
+    int layer_guess = 1;           // fields with a reference: unreached nodes start from the node walked before
     int autotune = 1;              // time candidate configurations at the first solve of a kind
     bool tune_fixed = false;       // the caller chose the strategy, patch, pass cap or check interval
     std::string tune_arch;         // device identity for the tuning cache (arch name / compute units)
@@ -2312,6 +2316,8 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     if (ptl) c->pass_tol = std::min(1e-3, std::max(0.0, atof(ptl)));
     const char* vfl = getenv("FMM3D_VERIFY_FILTER");
     if (vfl) c->verify_filter = atoi(vfl) != 0;
+    const char* lgs = getenv("FMM3D_LAYER_GUESS");
+    if (lgs) c->layer_guess = atoi(lgs) != 0;
     const char* gv = getenv("FMM3D_GRAPH");
     if (gv) c->use_graph = atoi(gv) != 0;
     const char* vf = getenv("FMM3D_VERIFY");
@@ -2989,6 +2995,7 @@ static int column_args(fmm3d_ctx* c, int nslots, int order, ColArgs<R>& a, int& 
     a.pend = s.pend;
     a.slotpend = s.slotpend;
     a.wlost = s.wlost;
+    a.lguess = c->layer_guess;
     a.vfilter = c->verify_filter;
     a.born = nullptr;
     // float64, and the float32 warm-up solves of refined blocks (float64
@@ -5100,6 +5107,9 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
     } else if (!strcmp(key, "ordered_sweeps")) {
         if (iv < 0 || iv > 1000000) return fail(c, FMM3D_E_ARG, "ordered_sweeps out of range");
         c->ordered_until = iv;
+    } else if (!strcmp(key, "layer_guess")) {
+        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "layer_guess must be 0 or 1");
+        c->layer_guess = iv;
     } else if (!strcmp(key, "verify_filter")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "verify_filter must be 0 or 1");
         c->verify_filter = iv;
