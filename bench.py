@@ -50,15 +50,15 @@ BYTES_PER_NODE_VISIT = 12
 # rocprofv3 FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000), each scaled by a
 # 1 GiB copy, over the launches of one batch of 50
 # (profiles/k_column_tele201_batch50_traffic.md)
-MEASURED_BYTES_PER_NODE_VISIT = 42.03 + 7.80
+MEASURED_BYTES_PER_NODE_VISIT = 42.54 + 8.01
 # wave64 VALU instructions per 64 float32 node updates of the column walk,
 # from the rocprofv3 SQ_INSTS_VALU pass over one batch of 50, and the
 # device's wave64 VALU issue rate at the clock that pass measured
-# (GRBM_GUI_ACTIVE per XCD over the launch time, 2.33 GHz): 256 CUs x 4
+# (GRBM_GUI_ACTIVE per XCD over the launch time, 2.29 GHz): 256 CUs x 4
 # SIMDs, one wave64 instruction per 4 cycles of a 16-lane SIMD
 # (profiles/k_column_tele201_batch50_valu.md)
-VALU_PER_64_UPDATES = 285.2
-VALU_WAVE_ISSUE_PER_S = 256 * 4 * 2.334e9 / 4
+VALU_PER_64_UPDATES = 299.2
+VALU_WAVE_ISSUE_PER_S = 256 * 4 * 2.289e9 / 4
 
 
 def parse():
@@ -569,8 +569,9 @@ def main():
                      # pass's VALU instructions per update; not the contract's roofline)
                      "valu_issue_frac": round(upd / (kms * 1e-3) / 64 * VALU_PER_64_UPDATES
                                               / VALU_WAVE_ISSUE_PER_S, 3) if kms > 0 else None,
-                     "valu_source": "rocprofv3 SQ_INSTS_VALU pass in profiles/: 285.2 wave64 VALU instructions "
-                                    "per 64 node updates; issue rate 1024 SIMDs x 2.334 GHz / 4"},
+                     "valu_source": f"rocprofv3 SQ_INSTS_VALU pass in profiles/: {VALU_PER_64_UPDATES} wave64 VALU "
+                                    f"instructions per 64 node updates; issue rate 1024 SIMDs x "
+                                    f"{VALU_WAVE_ISSUE_PER_S * 4 / 1024 / 1e9:.3f} GHz / 4"},
         "bottom_faces_only": {"value": round(bottom["value"], 3),
                               "ms_per_step": round(1e3 * bottom["elapsed"] / a.steps, 2),
                               "plain_fp64_source_solves_per_s": round(bottom["plain"], 3) if bottom["plain"] else None,
