@@ -307,7 +307,7 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *   "pass_tol"      fields with a reference: a layer's line passes stop
  *                   after one that moves no node by more than this many
  *                   seconds, and the fixed-point check counts only such
- *                   moves (0..1e-3, default 1e-6; 0 exact; FMM3D_PASS_TOL)
+ *                   moves (0..1e-3, default 5e-7; 0 exact; FMM3D_PASS_TOL)
  *   "layer_guess"   fields with a reference: 1 (default) a node the walk
  *                   reaches for the first time starts its layer's passes
  *                   from the value of the node walked just before it in its

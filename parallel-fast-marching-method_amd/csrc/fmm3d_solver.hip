@@ -1982,7 +1982,7 @@ struct fmm3d_ctx {
     int tele_ref = 1;              // float32 teleseismic fields relative to a 1-D plane-wave reference
     double tol_floor = 64.0;       // those fields: a change counts above tol32 * max(|value|, tol_floor)
     int ref_qbits = 22;            // those fields: updates rounded to multiples of 2^-ref_qbits s (0: not)
-    double pass_tol = 1e-6;        // those fields: a layer's passes stop after one that moves no node by more (s)
+    double pass_tol = 5e-7;        // those fields: a layer's passes stop after one that moves no node by more (s)
     std::vector<double> layer_slo; // mean slowness of each layer of the current model
     std::vector<std::pair<std::string, TuneChoice>> tuned;   // tuning key -> configuration (this context)
     int* d_iter = nullptr;         // device sweep counter (graph replays)
