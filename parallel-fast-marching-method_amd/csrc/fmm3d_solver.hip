@@ -2035,6 +2035,8 @@ struct fmm3d_ctx {
     size_t eng_stage = 0;
     double* eng_faces = nullptr;   // per slot bottom face of a teleseismic admission
     std::vector<int> eng_flow;     // per slot lateral flow (host copy the uploads read from)
+    std::vector<double> eng_ref;   // per slot reference row and offset (host copies the uploads read from)
+    std::vector<double> eng_off;
     double* eng_rcv = nullptr;     // receiver staging (grows; no free on the serving path)
     float* eng_out = nullptr;
     int* eng_islot = nullptr;
@@ -4670,6 +4672,8 @@ int fmm3d_engine_open(fmm3d_ctx* c, int cap, int precision, int order)
     c->eng_active.assign(cap, 0);
     c->eng_born.assign(cap, 0);
     c->eng_flow.assign((size_t)cap * 2, 1);
+    c->eng_ref.assign((size_t)cap * (c->g.nnr + 2), 0.0);
+    c->eng_off.assign((size_t)cap, 0.0);
     (void)hipFree(c->eng_faces);
     c->eng_faces = nullptr;
     HIPCHK(c, dalloc(&c->eng_faces, (size_t)cap * c->g.nnx * c->g.nnz * sizeof(double)));
@@ -4807,15 +4811,24 @@ int fmm3d_engine_admit_face(fmm3d_ctx* c, int slot, const double* face)
     hipError_t e = hipMemcpyAsync(df, face, NL * sizeof(double), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(s.flow + 2 * slot, &c->eng_flow[2 * (size_t)slot], 2 * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    // a float32 slot is stored against the face's plane-wave reference, as
+    // in the batched solves (offset: the plane's; else the face minimum)
+    const size_t RS = (size_t)g.nnr + 2;
+    double* rrow = c->eng_ref.data() + (size_t)slot * RS;
+    const bool useref = s.prec == FMM3D_PREC_FP32 && c->tele_ref && c->strategy == 1 &&
+                        tele_reference(c, face, rrow, &c->eng_off[slot]);
+    if (!useref) std::fill(rrow, rrow + RS, 0.0);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.ref + (size_t)slot * RS, rrow, RS * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && useref)
+        e = hipMemcpyAsync(s.toff + slot, &c->eng_off[slot], sizeof(double), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
         // NaN faces are refused at submission; the flag is not read here
-        k_tele_offsets<<<1, 1024, 0, c->stream>>>(NL, 0, 0, df, nullptr, nullptr, nullptr, s.toff + slot, c->d_bad);
-        // served slots keep the plain offset (no reference)
-        k_fill_real<double><<<1, 256, 0, c->stream>>>(s.ref + (size_t)slot * (g.nnr + 2), (size_t)g.nnr + 2, 0.0);
+        if (!useref) k_tele_offsets<<<1, 1024, 0, c->stream>>>(NL, 0, 0, df, nullptr, nullptr, nullptr, s.toff + slot, c->d_bad);
         if (s.prec == FMM3D_PREC_FP64)
             k_init_tele<double><<<dim3(256, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, (double*)s.T + (size_t)slot * N, s.toff + slot, nullptr);
         else
-            k_init_tele<float><<<dim3(256, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, (float*)s.T + (size_t)slot * N, s.toff + slot, nullptr);
+            k_init_tele<float><<<dim3(256, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, (float*)s.T + (size_t)slot * N, s.toff + slot,
+                                                                    useref ? s.ref + (size_t)slot * RS : nullptr);
         k_fixbits_tele<<<dim3(64, 1), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, 1, df, s.fix + (size_t)slot * NW);
         // the slot's per-patch and per-slot state in one launch; every patch
         // column of the new slot runs in the next sweep
