@@ -308,11 +308,12 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *                   after one that moves no node by more than this many
  *                   seconds, and the fixed-point check counts only such
  *                   moves (0..1e-3, default 5e-7; 0 exact; FMM3D_PASS_TOL)
- *   "layer_guess"   fields with a reference: 1 (default) a node the walk
- *                   reaches for the first time starts its layer's passes
- *                   from the value of the node walked just before it in its
- *                   column (the reference carries the vertical travel time,
- *                   so that value is close); 0 from infinity
+ *   "layer_guess"   fields with a reference: a node the walk reaches for
+ *                   the first time starts its layer's passes from the stored
+ *                   values of the nodes walked just before it in its column
+ *                   (the reference carries the vertical travel time, so they
+ *                   are close): 2 (default) the line through the two before
+ *                   it, 1 the one before it, 0 from infinity
  *                   (FMM3D_LAYER_GUESS)
  * Returns FMM3D_E_ARG for an unknown key or a value out of range. */
 int fmm3d_set_option(fmm3d_ctx *ctx, const char *key, double value);

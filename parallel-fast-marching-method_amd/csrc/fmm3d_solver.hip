@@ -438,7 +438,8 @@ template <typename R> struct ColArgs {
                             // more (s), and the fixed-point check counts only such moves (0: exact)
     int* wlost;             // per slot: a wait on a neighbour was given up in a sweep
     int lguess;             // slots with a reference: a node not yet reached starts its layer's passes
-                            // from the value of the node walked just before it in its column
+                            // from the node walked just before it in its column (1), or from the line
+                            // through the two walked before it (2)
     int vfilter;            // the check after a slot's first sweep evaluates only the nodes an input
                             // relaxed after them could move (below)
 };
@@ -768,6 +769,7 @@ k_column(ColArgs<R> a)
     const R rq = hasref ? (R)a.qscale : R(0);
     const R ptl = hasref ? (R)a.ptol : R(0);
     const bool lguess = hasref && a.lguess;
+    const bool lguess2 = a.lguess == 2;
     bool anyv = false;
     bool capped = false;   // a layer's passes hit the cap with nodes still moving
     unsigned long long nupd = 0, nvis = 0;   // wave-uniform counts (scalar registers)
@@ -850,7 +852,10 @@ k_column(ColArgs<R> a)
             }
             const bool fixed = !in || ((fw >> (c & 31)) & 1u);
             orig[k] = v;
-            cur[(lx + 2) * LZ + lz + 2] = (lguess && !fixed && !(v < inf) && b1[e] < inf) ? b1[e] : v;
+            R g0 = v;
+            if (lguess && !fixed && !(v < inf) && b1[e] < inf)
+                g0 = (lguess2 && b2[e] < inf) ? b1[e] + (b1[e] - b2[e]) : b1[e];
+            cur[(lx + 2) * LZ + lz + 2] = g0;
             u1[e] = w1;
             u2[e] = w2;
             sl[e] = fixed ? R(-1) : sv;
@@ -1975,7 +1980,7 @@ struct fmm3d_ctx {
     // on 50 plane waves at 201^3)
     int verify_patches = -1;
     int verify_filter = 1;         // the check after a first sweep skips nodes no later input can move
-    int layer_guess = 1;           // fields with a reference: unreached nodes start from the node walked before
+    int layer_guess = 2;           // fields with a reference: unreached nodes start from the two nodes walked before
     int autotune = 1;              // time candidate configurations at the first solve of a kind
     bool tune_fixed = false;       // the caller chose the strategy, patch, pass cap or check interval
     std::string tune_arch;         // device identity for the tuning cache (arch name / compute units)
@@ -2319,7 +2324,7 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     const char* vfl = getenv("FMM3D_VERIFY_FILTER");
     if (vfl) c->verify_filter = atoi(vfl) != 0;
     const char* lgs = getenv("FMM3D_LAYER_GUESS");
-    if (lgs) c->layer_guess = atoi(lgs) != 0;
+    if (lgs) c->layer_guess = std::min(2, std::max(0, atoi(lgs)));
     const char* gv = getenv("FMM3D_GRAPH");
     if (gv) c->use_graph = atoi(gv) != 0;
     const char* vf = getenv("FMM3D_VERIFY");
@@ -5121,7 +5126,7 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
         if (iv < 0 || iv > 1000000) return fail(c, FMM3D_E_ARG, "ordered_sweeps out of range");
         c->ordered_until = iv;
     } else if (!strcmp(key, "layer_guess")) {
-        if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "layer_guess must be 0 or 1");
+        if (iv < 0 || iv > 2) return fail(c, FMM3D_E_ARG, "layer_guess must be 0, 1 or 2");
         c->layer_guess = iv;
     } else if (!strcmp(key, "verify_filter")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "verify_filter must be 0 or 1");
