@@ -312,8 +312,9 @@ int fmm3d_get_stats(const fmm3d_ctx *ctx, fmm3d_stats *out);
  *                   the first time starts its layer's passes from the stored
  *                   values of the nodes walked just before it in its column
  *                   (the reference carries the vertical travel time, so they
- *                   are close): 2 (default) the line through the two before
- *                   it, 1 the one before it, 0 from infinity
+ *                   are close), extrapolated by the polynomial through the
+ *                   last n of them: 4 (default, the cubic), 3, 2 (the
+ *                   line), 1 (the node before), 0 from infinity
  *                   (FMM3D_LAYER_GUESS)
  * Returns FMM3D_E_ARG for an unknown key or a value out of range. */
 int fmm3d_set_option(fmm3d_ctx *ctx, const char *key, double value);

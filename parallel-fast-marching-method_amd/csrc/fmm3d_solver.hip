@@ -438,8 +438,8 @@ template <typename R> struct ColArgs {
                             // more (s), and the fixed-point check counts only such moves (0: exact)
     int* wlost;             // per slot: a wait on a neighbour was given up in a sweep
     int lguess;             // slots with a reference: a node not yet reached starts its layer's passes
-                            // from the node walked just before it in its column (1), or from the line
-                            // through the two walked before it (2)
+                            // from the polynomial through the lguess nodes walked just before it in its
+                            // column (1: the node before; up to 4: the cubic), 0: from infinity
     int vfilter;            // the check after a slot's first sweep evaluates only the nodes an input
                             // relaxed after them could move (below)
 };
@@ -635,6 +635,7 @@ k_column(ColArgs<R> a)
     constexpr int LX = PX + 4, LZ = (PZ + 4) | 1;   // odd row stride: phi passes spread over banks
     __shared__ R cur[LX * LZ];
     __shared__ R b1[NP], b2[NP], u1[NP], u2[NP], sl[NP];
+    __shared__ R b3[NP], b4[NP];   // the third and fourth layers walked before (layer start only)
     __shared__ R shz[PX], sihz[PX];
     __shared__ int s_act;
     __shared__ int s_win[2];
@@ -769,7 +770,6 @@ k_column(ColArgs<R> a)
     const R rq = hasref ? (R)a.qscale : R(0);
     const R ptl = hasref ? (R)a.ptol : R(0);
     const bool lguess = hasref && a.lguess;
-    const bool lguess2 = a.lguess == 2;
     bool anyv = false;
     bool capped = false;   // a layer's passes hit the cap with nodes still moving
     unsigned long long nupd = 0, nvis = 0;   // wave-uniform counts (scalar registers)
@@ -849,13 +849,12 @@ k_column(ColArgs<R> a)
             if (q == qs) {
                 b1[e] = (in && q >= 1) ? T[c - (ptrdiff_t)dn * (ptrdiff_t)NL] : inf;
                 b2[e] = (in && q >= 2) ? T[c - (ptrdiff_t)(2 * dn) * (ptrdiff_t)NL] : inf;
+                b3[e] = (in && q >= 3) ? T[c - (ptrdiff_t)(3 * dn) * (ptrdiff_t)NL] : inf;
+                b4[e] = (in && q >= 4) ? T[c - (ptrdiff_t)(4 * dn) * (ptrdiff_t)NL] : inf;
             }
             const bool fixed = !in || ((fw >> (c & 31)) & 1u);
             orig[k] = v;
-            R g0 = v;
-            if (lguess && !fixed && !(v < inf) && b1[e] < inf)
-                g0 = (lguess2 && b2[e] < inf) ? b1[e] + (b1[e] - b2[e]) : b1[e];
-            cur[(lx + 2) * LZ + lz + 2] = g0;
+            cur[(lx + 2) * LZ + lz + 2] = v;
             u1[e] = w1;
             u2[e] = w2;
             sl[e] = fixed ? R(-1) : sv;
@@ -866,6 +865,24 @@ k_column(ColArgs<R> a)
             int idx, off;
             ring(k, idx, off);
             if (idx >= 0) cur[idx] = hv[k];
+        }
+        // layer start: a free node not reached yet begins from the stored
+        // values of the nodes walked before it in its column (this lane
+        // staged all of them), extrapolated by the polynomial through up to
+        // a.lguess of them (fewer near the walk's first layers)
+        if (lguess) {
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                const int e = t + 64 * k;
+                const R p1 = b1[e];
+                if (sl[e] >= R(0) && !(orig[k] < inf) && p1 < inf) {
+                    const R p2 = b2[e], p3 = b3[e], p4 = b4[e];
+                    const int m = a.lguess < 2 || !(p2 < inf) ? 1 : a.lguess < 3 || !(p3 < inf) ? 2 : a.lguess < 4 || !(p4 < inf) ? 3 : 4;
+                    cur[(e / PZ + 2) * LZ + e % PZ + 2] = m == 4 ? R(4) * (p1 - p2) - R(2) * (p2 - p3) + (p3 - p4) + p3
+                                                        : m == 3 ? R(3) * (p1 - p2) + p3
+                                                        : m == 2 ? p1 + (p1 - p2) : p1;
+                }
+            }
         }
         if (t < PX) {
             shz[t] = hzv;
@@ -947,6 +964,8 @@ k_column(ColArgs<R> a)
                     !(fabs((double)nv - (double)tc) <= a.tol * fmax(fmax(fabs((double)nv), fabs((double)tc)), tfloor)))
                     lay_chg = true;
             }
+            b4[e] = b3[e];
+            b3[e] = b2[e];
             b2[e] = b1[e];
             b1[e] = nv;
         }
@@ -1980,7 +1999,7 @@ struct fmm3d_ctx {
     // on 50 plane waves at 201^3)
     int verify_patches = -1;
     int verify_filter = 1;         // the check after a first sweep skips nodes no later input can move
-    int layer_guess = 2;           // fields with a reference: unreached nodes start from the two nodes walked before
+    int layer_guess = 4;           // fields with a reference: unreached nodes start from the cubic through the four nodes walked before
     int autotune = 1;              // time candidate configurations at the first solve of a kind
     bool tune_fixed = false;       // the caller chose the strategy, patch, pass cap or check interval
     std::string tune_arch;         // device identity for the tuning cache (arch name / compute units)
@@ -2324,7 +2343,7 @@ static int ctx_setup(fmm3d_ctx* c, int device, fmm3d_ctx** out)
     const char* vfl = getenv("FMM3D_VERIFY_FILTER");
     if (vfl) c->verify_filter = atoi(vfl) != 0;
     const char* lgs = getenv("FMM3D_LAYER_GUESS");
-    if (lgs) c->layer_guess = std::min(2, std::max(0, atoi(lgs)));
+    if (lgs) c->layer_guess = std::min(4, std::max(0, atoi(lgs)));
     const char* gv = getenv("FMM3D_GRAPH");
     if (gv) c->use_graph = atoi(gv) != 0;
     const char* vf = getenv("FMM3D_VERIFY");
@@ -5126,7 +5145,7 @@ extern "C" int fmm3d_set_option(fmm3d_ctx* c, const char* key, double v)
         if (iv < 0 || iv > 1000000) return fail(c, FMM3D_E_ARG, "ordered_sweeps out of range");
         c->ordered_until = iv;
     } else if (!strcmp(key, "layer_guess")) {
-        if (iv < 0 || iv > 2) return fail(c, FMM3D_E_ARG, "layer_guess must be 0, 1 or 2");
+        if (iv < 0 || iv > 4) return fail(c, FMM3D_E_ARG, "layer_guess must be 0 .. 4");
         c->layer_guess = iv;
     } else if (!strcmp(key, "verify_filter")) {
         if (iv != 0 && iv != 1) return fail(c, FMM3D_E_ARG, "verify_filter must be 0 or 1");
