@@ -122,6 +122,22 @@ int fmm3d_solve_tele(fmm3d_ctx *ctx, int nsrc, const double *faces, int order,
 int fmm3d_solve_tele_sides(fmm3d_ctx *ctx, int nsrc, const double *faces, const double *xface,
                            const double *zface, const int *side, int order, int precision, int flags);
 
+/* The same solves in two steps, for callers that solve the same sources
+ * more than once (several models of an inversion, repeated precisions) or
+ * keep their inputs resident: fmm3d_stage_tele_sides copies the faces (as
+ * fmm3d_solve_tele_sides takes them; xface = NULL for bottom faces only)
+ * into device memory the context owns and fits each source's plane-wave
+ * reference and lateral flow there and then; fmm3d_solve_tele_staged
+ * solves the staged sources, with the fields and results of
+ * fmm3d_solve_tele_sides on the same faces.  No counterpart in the
+ * reference (its faces come from itimes.out on every run,
+ * fm3dt_parallel.f90:6894-6961).  A solve of any kind on the context
+ * stages its own sources in place of them; the partitioned solve leaves
+ * none staged (FMM3D_E_STATE). */
+int fmm3d_stage_tele_sides(fmm3d_ctx *ctx, int nsrc, const double *faces, const double *xface,
+                           const double *zface, const int *side);
+int fmm3d_solve_tele_staged(fmm3d_ctx *ctx, int order, int precision);
+
 /* Solves from general fixed (Dirichlet) node sets, one per source: the
  * generalisation of the bottom-face injection above to any nodes, e.g.
  * bottom and side faces of a teleseismic plane wave or the refined

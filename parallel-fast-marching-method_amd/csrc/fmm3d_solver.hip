@@ -2054,6 +2054,14 @@ struct fmm3d_ctx {
     double* d_sides = nullptr;     // side faces of the last teleseismic solve: theta faces, then phi faces
     int* d_side_of = nullptr;      // per source: theta side, phi side (-1 none, 0 low index, 1 high)
     size_t d_sides_n = 0, d_side_of_n = 0;
+    // teleseismic sources staged in d_faces (and d_sides): by fmm3d_stage_tele_sides or by a solve
+    int stg_n = 0;
+    bool stg_sides = false;
+    double* d_stg_toff = nullptr;  // per source: the boundary minimum
+    double* d_stg_ref = nullptr;   // per source: the plane-wave reference row (zero without a fitted plane)
+    double* d_stg_off = nullptr;   // per source: the reference's offset (NaN without a fitted plane)
+    int* d_stg_flow = nullptr;     // per source: the lateral flow
+    size_t stg_cap = 0;            // sources those arrays hold
     int64_t* eng_node = nullptr;   // device staging of an admission's fixed set
     double* eng_tv = nullptr;
     size_t eng_stage = 0;
@@ -2402,6 +2410,7 @@ extern "C" int fmm3d_destroy(fmm3d_ctx* c)
     (void)hipFree(c->st_offs); (void)hipFree(c->st_node); (void)hipFree(c->st_tv); (void)hipFree(c->d_velv); (void)hipFree(c->d_bad);
     (void)hipFree(c->d_layer); (void)hipFree(c->st_rcv); (void)hipFree(c->st_rout); (void)hipFree(c->st_islot);
     (void)hipFree(c->d_sides); (void)hipFree(c->d_side_of);
+    (void)hipFree(c->d_stg_toff); (void)hipFree(c->d_stg_ref); (void)hipFree(c->d_stg_off); (void)hipFree(c->d_stg_flow);
     (void)hipFree(c->eng_node); (void)hipFree(c->eng_tv); (void)hipFree(c->eng_faces);
     (void)hipFree(c->eng_rcv); (void)hipFree(c->eng_out); (void)hipFree(c->eng_islot);
     free_slots(c->sl);
@@ -3555,14 +3564,26 @@ static int tele_check(fmm3d_ctx* c, const char* fn, int nsrc, const double* face
     return FMM3D_OK;
 }
 
-// Initial fields, fixed bits, offsets and lateral flows of slots
-// 0 .. nsrc-1 from teleseismic faces (slots must be allocated).
-static int tele_upload(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, const double* xf,
-                       const double* zf, const int* side, int precision)
+// A staged source's offset: the reference's where a plane was fitted,
+// else the boundary minimum.
+__global__ void k_pick_offsets(int n, const double* __restrict__ off, double* __restrict__ toff)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && off[i] == off[i]) toff[i] = off[i];
+}
+
+// Stage teleseismic sources: their faces in device memory (d_faces,
+// d_sides), and per source the boundary minimum and NaN check (on the
+// device), the lateral flow and the plane-wave reference row and offset
+// (fitted on the host from the face).  Solves start from the staged data
+// (tele_init); fmm3d_stage_tele_sides stages them ahead of the solves.
+static int tele_stage(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, const double* xf,
+                      const double* zf, const int* side)
 {
     const Grid& g = c->g;
-    Slots& s = c->sl;
     const size_t F = (size_t)g.nnx * g.nnz, XF = (size_t)g.nnr * g.nnz, ZF = (size_t)g.nnr * g.nnx;
+    const size_t RS = (size_t)g.nnr + 2;
+    c->stg_n = 0;
     if (c->d_faces_n < (size_t)nsrc * F) {
         (void)hipFree(c->d_faces);
         c->d_faces = nullptr;
@@ -3570,8 +3591,7 @@ static int tele_upload(fmm3d_ctx* c, const char* fn, int nsrc, const double* fac
         HIPCHK(c, dalloc(&c->d_faces, (size_t)nsrc * F * sizeof(double)));
         c->d_faces_n = (size_t)nsrc * F;
     }
-    double* dfaces = c->d_faces;
-    HIPCHK(c, hipMemcpyAsync(dfaces, faces, (size_t)nsrc * F * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_faces, faces, (size_t)nsrc * F * sizeof(double), hipMemcpyHostToDevice, c->stream));
     double* dx = nullptr;
     double* dz = nullptr;
     if (xf) {
@@ -3596,47 +3616,88 @@ static int tele_upload(fmm3d_ctx* c, const char* fn, int nsrc, const double* fac
         HIPCHK(c, hipMemcpyAsync(dz, zf, (size_t)nsrc * ZF * sizeof(double), hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, hipMemcpyAsync(c->d_side_of, side, (size_t)nsrc * 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
     }
-    // the float32 offsets (boundary minima) and the NaN check, on the device
+    if (c->stg_cap < (size_t)nsrc) {
+        (void)hipFree(c->d_stg_toff); (void)hipFree(c->d_stg_ref); (void)hipFree(c->d_stg_off); (void)hipFree(c->d_stg_flow);
+        c->d_stg_toff = c->d_stg_ref = c->d_stg_off = nullptr;
+        c->d_stg_flow = nullptr;
+        c->stg_cap = 0;
+        HIPCHK(c, dalloc(&c->d_stg_toff, (size_t)nsrc * sizeof(double)));
+        HIPCHK(c, dalloc(&c->d_stg_ref, (size_t)nsrc * RS * sizeof(double)));
+        HIPCHK(c, dalloc(&c->d_stg_off, (size_t)nsrc * sizeof(double)));
+        HIPCHK(c, dalloc(&c->d_stg_flow, (size_t)nsrc * 2 * sizeof(int)));
+        c->stg_cap = nsrc;
+    }
+    // the boundary minima and the NaN check, on the device
     if (!c->d_bad) HIPCHK(c, dalloc(&c->d_bad, sizeof(int)));
     k_fill_int<<<1, 1, 0, c->stream>>>(c->d_bad, 1, nsrc);
-    k_tele_offsets<<<nsrc, 1024, 0, c->stream>>>(F, XF, ZF, dfaces, dx, dz, xf ? c->d_side_of : nullptr, s.toff, c->d_bad);
+    k_tele_offsets<<<nsrc, 1024, 0, c->stream>>>(F, XF, ZF, c->d_faces, dx, dz, xf ? c->d_side_of : nullptr, c->d_stg_toff, c->d_bad);
     int bad = nsrc;
     HIPCHK(c, hipMemcpyAsync(&bad, c->d_bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    // flows and reference rows from the faces, on the host, while those run
+    std::vector<int> flow((size_t)nsrc * 2);
+    std::vector<double> ref((size_t)nsrc * RS, 0.0), off((size_t)nsrc, __builtin_nan(""));
+    for (int q = 0; q < nsrc; ++q) {
+        lateral_flow(g, faces + (size_t)q * F, &flow[2 * (size_t)q]);
+        if (c->tele_ref) tele_reference(c, faces + (size_t)q * F, &ref[(size_t)q * RS], &off[q]);
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_stg_flow, flow.data(), flow.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_stg_ref, ref.data(), ref.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_stg_off, off.data(), off.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // the host vectors are pageable memory: the copies finish before they go
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (bad < nsrc) return fail(c, FMM3D_E_ARG, "%s: NaN boundary time (source %d)", fn, bad);
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) return fail(c, FMM3D_E_DEVICE, "stage: %s", hipGetErrorString(le));
+    c->stg_n = nsrc;
+    c->stg_sides = xf != nullptr;
+    return FMM3D_OK;
+}
+
+// Initial fields, fixed bits, offsets, reference rows and lateral flows of
+// slots 0 .. nsrc-1 from the staged sources (slots must be allocated); all
+// on the device.
+static int tele_init(fmm3d_ctx* c, int nsrc, int precision)
+{
+    const Grid& g = c->g;
+    Slots& s = c->sl;
+    const size_t RS = (size_t)g.nnr + 2;
     // float32 fields on the column walk (the kernels that apply the
     // reference) are stored against each slot's plane-wave reference
     const bool useref = precision == FMM3D_PREC_FP32 && c->tele_ref && (c->strategy == 1 || (c->strategy == 0 && c->walk_alt));
-    const size_t RS = (size_t)g.nnr + 2;
-    std::vector<double> ref((size_t)nsrc * RS, 0.0), toff;
+    HIPCHK(c, hipMemcpyAsync(s.toff, c->d_stg_toff, nsrc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     if (useref) {
-        toff.resize(nsrc);
-        HIPCHK(c, hipMemcpyAsync(toff.data(), s.toff, nsrc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        k_pick_offsets<<<(nsrc + 255) / 256, 256, 0, c->stream>>>(nsrc, c->d_stg_off, s.toff);
+        HIPCHK(c, hipMemcpyAsync(s.ref, c->d_stg_ref, (size_t)nsrc * RS * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        k_fill_real<double><<<(int)std::min<size_t>(1024, ((size_t)nsrc * RS + 255) / 256), 256, 0, c->stream>>>(s.ref, (size_t)nsrc * RS, 0.0);
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (bad < nsrc) return fail(c, FMM3D_E_ARG, "%s: NaN boundary time (source %d)", fn, bad);
-    if (useref) {
-        for (int q = 0; q < nsrc; ++q) tele_reference(c, faces + (size_t)q * F, &ref[(size_t)q * RS], &toff[q]);
-        HIPCHK(c, hipMemcpyAsync(s.toff, toff.data(), nsrc * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    }
-    HIPCHK(c, hipMemcpyAsync(s.ref, ref.data(), ref.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.flow, c->d_stg_flow, (size_t)nsrc * 2 * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    const double* dfaces = c->d_faces;
     if (precision == FMM3D_PREC_FP64)
         k_init_tele<double><<<dim3(256, std::min(nsrc, 65535)), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (double*)s.T, s.toff, nullptr);
     else
         k_init_tele<float><<<dim3(256, std::min(nsrc, 65535)), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, (float*)s.T, s.toff, s.ref);
     k_fixbits_tele<<<dim3(64, std::min(nsrc, 65535)), 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dfaces, s.fix);
-    if (xf) {
+    if (c->stg_sides) {
+        const double* dx = c->d_sides;
+        const double* dz = c->d_sides + (size_t)nsrc * g.nnr * g.nnz;
         if (precision == FMM3D_PREC_FP64)
             k_init_sides<double><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dx, dz, c->d_side_of, s.toff, nullptr, (double*)s.T, s.fix);
         else
             k_init_sides<float><<<4096, 256, 0, c->stream>>>(g.nnr, g.nnx, g.nnz, nsrc, dx, dz, c->d_side_of, s.toff, s.ref, (float*)s.T, s.fix);
     }
-    std::vector<int> flow((size_t)nsrc * 2);
-    for (int q = 0; q < nsrc; ++q) lateral_flow(g, faces + (size_t)q * F, &flow[2 * q]);
-    HIPCHK(c, hipMemcpyAsync(s.flow, flow.data(), flow.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    // the flow vector is pageable host memory: the copy must finish before it goes
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) return fail(c, FMM3D_E_DEVICE, "init: %s", hipGetErrorString(le));
     return FMM3D_OK;
+}
+
+// Stage slots 0 .. nsrc-1 from teleseismic faces and initialize them.
+static int tele_upload(fmm3d_ctx* c, const char* fn, int nsrc, const double* faces, const double* xf,
+                       const double* zf, const int* side, int precision)
+{
+    int rc = tele_stage(c, fn, nsrc, faces, xf, zf, side);
+    if (rc) return rc;
+    return tele_init(c, nsrc, precision);
 }
 
 #include "fmm3d_partition.inl"
@@ -3658,6 +3719,7 @@ static int solve_tele_part(fmm3d_ctx* c, const char* fn, int nsrc, const double*
         HIPCHK(c, dalloc(&c->d_faces, (size_t)nsrc * F * sizeof(double)));
         c->d_faces_n = (size_t)nsrc * F;
     }
+    c->stg_n = 0;   // the staged faces are replaced
     HIPCHK(c, hipMemcpyAsync(c->d_faces, faces, (size_t)nsrc * F * sizeof(double), hipMemcpyHostToDevice, c->stream));
     c->stats = fmm3d_stats{};
     HIPCHK(c, hipMemsetAsync(c->d_visits, 0, 3 * sizeof(unsigned long long), c->stream));
@@ -3717,6 +3779,37 @@ extern "C" int fmm3d_solve_tele_sides(fmm3d_ctx* c, int nsrc, const double* face
                                       const double* zface, const int* side, int order, int precision, int flags)
 {
     return solve_tele_impl(c, "fmm3d_solve_tele_sides", nsrc, faces, xface, zface, side, order, precision, flags);
+}
+
+extern "C" int fmm3d_stage_tele_sides(fmm3d_ctx* c, int nsrc, const double* faces, const double* xface,
+                                      const double* zface, const int* side)
+{
+    const char* fn = "fmm3d_stage_tele_sides";
+    if (!c) return fail(c, FMM3D_E_ARG, "%s: null context", fn);
+    if (!c->have_model) return fail(c, FMM3D_E_STATE, "%s: no velocity model (call fmm3d_set_model_vtx)", fn);
+    int rc = tele_check(c, fn, nsrc, faces, xface, zface, side);
+    if (rc) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    return tele_stage(c, fn, nsrc, faces, xface, zface, side);
+}
+
+extern "C" int fmm3d_solve_tele_staged(fmm3d_ctx* c, int order, int precision)
+{
+    const char* fn = "fmm3d_solve_tele_staged";
+    if (!c) return fail(c, FMM3D_E_ARG, "%s: null context", fn);
+    if (order != FMM3D_ORDER_FIRST && order != FMM3D_ORDER_MIXED) return fail(c, FMM3D_E_ARG, "%s: bad order %d", fn, order);
+    if (precision != FMM3D_PREC_FP64 && precision != FMM3D_PREC_FP32) return fail(c, FMM3D_E_ARG, "%s: bad precision %d", fn, precision);
+    if (!c->have_model) return fail(c, FMM3D_E_STATE, "%s: no velocity model (call fmm3d_set_model_vtx)", fn);
+    if (c->stg_n <= 0) return fail(c, FMM3D_E_STATE, "%s: no staged sources (call fmm3d_stage_tele_sides)", fn);
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nsrc = c->stg_n;
+    int rc = ensure_slots(c, nsrc, precision);
+    if (rc) return rc;
+    c->sl.n = 0;
+    c->walk_alt = 0;
+    rc = tele_init(c, nsrc, precision);
+    if (rc) return rc;
+    return solve_run(c, nsrc, order, precision);
 }
 
 // Lateral flow signs for a general fixed set: the (theta, phi) slopes of a

@@ -26,6 +26,7 @@ REFERENCE_PARTITION = 2
 
 EXPORTS = ("fmm3d_create", "fmm3d_destroy", "fmm3d_last_error", "fmm3d_set_model_vtx",
            "fmm3d_solve_fixed", "fmm3d_solve_point", "fmm3d_solve_tele_sides", "fmm3d_solve_mixed",
+           "fmm3d_stage_tele_sides", "fmm3d_solve_tele_staged",
            "fmm3d_server_create", "fmm3d_server_submit", "fmm3d_server_wait", "fmm3d_server_result",
            "fmm3d_server_stats", "fmm3d_server_destroy", "fmm3d_server_rows",
            "fmm3d_solve_tele", "fmm3d_receivers", "fmm3d_receivers_noalloc", "fmm3d_frechet", "fmm3d_frechet_pinned", "fmm3d_get_field", "fmm3d_get_passes",
@@ -95,6 +96,8 @@ def load(path: Optional[str] = None):
     L.fmm3d_solve_tele.argtypes = [vp, C.c_int, dp, C.c_int, C.c_int, C.c_int]
     L.fmm3d_solve_tele_sides.argtypes = [vp, C.c_int, dp, dp, dp, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS"),
                                          C.c_int, C.c_int, C.c_int]
+    L.fmm3d_stage_tele_sides.argtypes = [vp, C.c_int, dp, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.fmm3d_solve_tele_staged.argtypes = [vp, C.c_int, C.c_int]
     L.fmm3d_server_create.argtypes = [C.c_int, C.c_double, C.c_int, C.POINTER(vp)]
     L.fmm3d_server_submit.argtypes = [vp, C.POINTER(RequestC), C.POINTER(C.c_int64)]
     L.fmm3d_server_wait.argtypes = [vp, C.c_int64, C.c_double]
@@ -152,6 +155,7 @@ class Context:
         if rc != OK:
             raise FmmError(rc, self.L.fmm3d_last_error(None).decode())
         self.nsrc = 0
+        self._staged = 0
 
     def _check(self, rc: int):
         if rc != OK:
@@ -189,6 +193,7 @@ class Context:
         flags = REFERENCE_PARTITION if partition else 0
         self._check(self.L.fmm3d_solve_tele(self._h, faces.shape[0], faces, order, precision, flags))
         self.nsrc = faces.shape[0]
+        self._staged = 0 if partition else self.nsrc   # a solve stages its own sources
 
     def passes(self) -> np.ndarray:
         """Outer passes per source of the last partitioned solve."""
@@ -212,6 +217,32 @@ class Context:
             raise ValueError("solve_tele_sides: face arrays of inconsistent shapes")
         self._check(self.L.fmm3d_solve_tele_sides(self._h, n, faces, xface, zface, side, order, precision, 0))
         self.nsrc = n
+        self._staged = n
+
+    def stage_tele(self, faces: np.ndarray, xface: Optional[np.ndarray] = None, zface: Optional[np.ndarray] = None,
+                   side: Optional[np.ndarray] = None) -> None:
+        """Stage teleseismic sources in device memory (fmm3d_stage_tele_sides):
+        bottom faces (nsrc, nnx, nnz), and optionally the side faces and sides
+        as solve_tele_sides takes them; solve_tele_staged solves them."""
+        faces = np.ascontiguousarray(faces, dtype=np.float64)
+        n = faces.shape[0]
+        g = self.g
+        if faces.shape != (n, g.nnx, g.nnz):
+            raise ValueError("stage_tele: faces must be (nsrc, nnx, nnz)")
+        ptr = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
+        if xface is not None:
+            xface = np.ascontiguousarray(xface, dtype=np.float64)
+            zface = np.ascontiguousarray(zface, dtype=np.float64)
+            side = np.ascontiguousarray(side, dtype=np.int32)
+            if xface.shape != (n, g.nnr, g.nnz) or zface.shape != (n, g.nnr, g.nnx) or side.shape != (n, 2):
+                raise ValueError("stage_tele: face arrays of inconsistent shapes")
+        self._check(self.L.fmm3d_stage_tele_sides(self._h, n, faces, ptr(xface), ptr(zface), ptr(side)))
+        self._staged = n
+
+    def solve_tele_staged(self, order: int = ORDER_MIXED, precision: int = PREC_FP64) -> None:
+        """Solve the sources staged by stage_tele (fmm3d_solve_tele_staged)."""
+        self._check(self.L.fmm3d_solve_tele_staged(self._h, order, precision))
+        self.nsrc = self._staged
 
     def solve_fixed(self, sets, order: int = ORDER_MIXED, precision: int = PREC_FP64, alternate: bool = False) -> None:
         """sets: one (node_index array, time array) pair per source."""
@@ -266,6 +297,8 @@ class Context:
             addr(None if src is None else np.atleast_2d(src), np.float64), refine,
             addr(radius, np.int32), order, precision, 0))
         self.nsrc = nt + npt
+        if nt > 0:
+            self._staged = nt
 
     def field(self, src: int) -> np.ndarray:
         out = np.zeros((self.g.nnr, self.g.nnx, self.g.nnz))
