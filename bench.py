@@ -50,15 +50,15 @@ BYTES_PER_NODE_VISIT = 12
 # rocprofv3 FETCH_SIZE (x2.000) and WRITE_SIZE (x1.000), each scaled by a
 # 1 GiB copy, over the launches of one batch of 50
 # (profiles/k_column_tele201_batch50_traffic.md)
-MEASURED_BYTES_PER_NODE_VISIT = 44.98 + 8.06
+MEASURED_BYTES_PER_NODE_VISIT = 44.94 + 8.06
 # wave64 VALU instructions per 64 float32 node updates of the column walk,
 # from the rocprofv3 SQ_INSTS_VALU pass over one batch of 50, and the
 # device's wave64 VALU issue rate at the clock that pass measured
-# (GRBM_GUI_ACTIVE per XCD over the launch time, 2.31 GHz): 256 CUs x 4
+# (GRBM_GUI_ACTIVE per XCD over the launch time, 2.28 GHz): 256 CUs x 4
 # SIMDs, one wave64 instruction per 4 cycles of a 16-lane SIMD
 # (profiles/k_column_tele201_batch50_valu.md)
-VALU_PER_64_UPDATES = 338.2
-VALU_WAVE_ISSUE_PER_S = 256 * 4 * 2.314e9 / 4
+VALU_PER_64_UPDATES = 338.8
+VALU_WAVE_ISSUE_PER_S = 256 * 4 * 2.281e9 / 4
 
 
 def parse():
