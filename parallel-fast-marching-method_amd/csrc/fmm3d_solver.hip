@@ -1542,23 +1542,22 @@ struct RayOut {
     int64_t cap;
 };
 
-// A sample of a ray path for the Frechet pass: a step's start point (kind 0:
-// its B-spline weights and velocity open the step's first sub-segment) or a
-// sub-segment's end (kind 1: the taps of the sub-segment from the previous
-// sample to this one, over length dinc).  The weights are taken in vertex
-// cell (ivr, ivx, ivz), at the sample's fractions (tr, tx, tz) of that
-// cell; ivel2 is 1 / velocity^2.  Both are formed once per sample here, not
-// on each of the 64 lanes that spread the sample's taps.
-struct RaySample {
-    double tr, tx, tz;
-    double ivel2;
-    double dinc;
-    int ivr, ivx, ivz;
-    int kind;
+// A ray path point as k_ray_paths records it: a step's start (kind 0) or a
+// sub-segment's end (kind 1, over length dinc), at (r, x, z) in vertex cell
+// (ivr, ivx, ivz).  A step's start also keeps the propagation-grid cell its
+// velocity is interpolated in (ipr, ipx, ipz: the ray's current cell, which
+// the start and source rules can clamp); a sub-segment end's cell is the
+// one its position falls in.  k_ray_frechet turns 64 points at a time, one
+// per lane, into samples (fractions, velocity), so that none of that float64
+// work sits on the ray's step-to-step chain.
+struct RayPoint {
+    double r, x, z, dinc;
+    int ivr, ivx, ivz, kind;
+    int ipr, ipx, ipz, pad;
 };
 
 struct RayPaths {
-    RaySample* rec;     // ray q's samples at rec + q * cap
+    RayPoint* rec;      // ray q's points at rec + q * cap
     int32_t* n;         // samples per ray; -1 start outside the grid, -3 more than cap
     int32_t* box;       // vertices touched per ray: rmin, rmax, xmin, xmax, zmin, zmax
     int64_t cap;
@@ -1567,14 +1566,14 @@ struct RayPaths {
 // Ray paths, one lane per ray: the reference's ray tracer (F:7347-7541) --
 // the field's gradient in the current cell, a step of dpl against it,
 // clipping at the first grid face crossed -- and its split of each step at
-// the vertex-cell faces (F:7542-7600), recorded as samples; the velocity of
-// every sample is interpolated here, once per ray.  The lanes of a
-// wavefront trace different rays, so none of this float64 work is repeated
-// (k_ray_frechet then spreads each sample's 64 taps over 64 lanes).
+// the vertex-cell faces (F:7542-7600), recorded as points.  The lanes of a
+// wavefront trace different rays; a ray's steps depend on each other, so
+// the per-point work (the velocity, the B-spline fractions) is left to
+// k_ray_frechet, which spreads it over the lanes.
 template <typename R>
 __global__ void __launch_bounds__(64)
 k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, const double* __restrict__ ref,
-            const double* __restrict__ veln, int nrc, const double* __restrict__ rcv, int ray0, int nrays, RayPaths P,
+            int nrc, const double* __restrict__ rcv, int ray0, int nrays, RayPaths P,
             const double* __restrict__ srcpos, const int* __restrict__ islot)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1591,17 +1590,15 @@ k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, co
     auto TT = [&](int r, int x, int z) -> double {   // 1-based node access
         return field_at<R>(F, to, rs, g.nnr, r - 1, x - 1, z - 1, ((size_t)(r - 1) * g.nnx + (x - 1)) * g.nnz + (z - 1));
     };
-    RaySample* out = P.rec + (int64_t)q * P.cap;
+    RayPoint* out = P.rec + (int64_t)q * P.cap;
     int64_t ns = 0;
     bool over = false;
-    auto put = [&](double r, double x, double z, double vel, double dinc, int a, int b, int c, int kind) {
+    auto put = [&](double r, double x, double z, int pa, int pb, int pc, double dinc, int a, int b, int c, int kind) {
         if (ns >= P.cap) { over = true; return; }
-        RaySample S;
-        S.tr = ((g.gor - r) - (a - 1) * g.dvr) / g.dvr;
-        S.tx = ((x - g.gox) - (b - 1) * g.dvx) / g.dvx;
-        S.tz = ((z - g.goz) - (c - 1) * g.dvz) / g.dvz;
-        S.ivel2 = 1.0 / (vel * vel);
-        S.dinc = dinc; S.ivr = a; S.ivx = b; S.ivz = c; S.kind = kind;
+        RayPoint S;
+        S.r = r; S.x = x; S.z = z; S.dinc = dinc;
+        S.ivr = a; S.ivx = b; S.ivz = c; S.kind = kind;
+        S.ipr = pa; S.ipx = pb; S.ipz = pc; S.pad = 0;
         out[ns++] = S;
     };
     double r0 = rcv[3 * i], x0 = rcv[3 * i + 1], z0 = rcv[3 * i + 2];
@@ -1627,6 +1624,11 @@ k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, co
         const double rr = g.earth + sr;
         xsrc[0] = rr * sin(sxs) * cos(szs); xsrc[1] = rr * sin(sxs) * sin(szs); xsrc[2] = rr * cos(sxs);
     }
+    // the vertex cell of the step's start: the previous step's end cell
+    // (the same expression of the same position)
+    int ivro = (int)floor((g.gor - pr) / g.dvr) + 1;
+    int ivxo = (int)floor((px - g.gox) / g.dvx) + 1;
+    int ivzo = (int)floor((pz - g.goz) / g.dvz) + 1;
     for (int step = 1; step <= 100000 && !over; ++step) {
         double dtr = 0.0;
         int cnt = 0;
@@ -1716,9 +1718,6 @@ k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, co
         int ivr = (int)floor((g.gor - nr_) / g.dvr) + 1;
         int ivx = (int)floor((nx_ - g.gox) / g.dvx) + 1;
         int ivz = (int)floor((nz_ - g.goz) / g.dvz) + 1;
-        int ivro = (int)floor((g.gor - pr) / g.dvr) + 1;
-        int ivxo = (int)floor((px - g.gox) / g.dvx) + 1;
-        int ivzo = (int)floor((pz - g.goz) / g.dvz) + 1;
         int nhp = 0;
         double vrat[4];
         int chp[4];
@@ -1741,10 +1740,7 @@ k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, co
             vrat[p] = qq; chp[p] = 3; nhp++;
         }
         vrat[nhp] = 1.0; chp[nhp] = 0; nhp++;
-        put(pr, px, pz,
-            tri_vel(g, veln, ipro, ipxo, ipzo, (g.rgor - pr) - (ipro - 1) * g.dnr,
-                    (px - g.rgox) - (ipxo - 1) * g.dnx, (pz - g.rgoz) - (ipzo - 1) * g.dnz),
-            0.0, ivro, ivxo, ivzo, 0);
+        put(pr, px, pz, ipro, ipxo, ipzo, 0.0, ivro, ivxo, ivzo, 0);
         int ivrt = ivro, ivxt = ivxo, ivzt = ivzo;
         for (int k = 0; k < nhp; ++k) {
             if (k > 0) {
@@ -1755,20 +1751,14 @@ k_ray_paths(Grid g, const R* __restrict__ T, const double* __restrict__ toff, co
             double sz_ = pz + vrat[k] * (nz_ - pz);
             double sx_ = px + vrat[k] * (nx_ - px);
             double sr_ = pr + vrat[k] * (nr_ - pr);
-            int iprt = (int)floor((g.rgor - sr_) / g.dnr) + 1;
-            int ipxt = (int)floor((sx_ - g.rgox) / g.dnx) + 1;
-            int ipzt = (int)floor((sz_ - g.rgoz) / g.dnz) + 1;
-            const double vel = tri_vel(g, veln, iprt, ipxt, ipzt,
-                                       (g.rgor - sr_) - (iprt - 1) * g.dnr,
-                                       (sx_ - g.rgox) - (ipxt - 1) * g.dnx,
-                                       (sz_ - g.rgoz) - (ipzt - 1) * g.dnz);
             const double dinc = (k == 0) ? vrat[0] * seg : (vrat[k] - vrat[k - 1]) * seg;
-            put(sr_, sx_, sz_, vel, dinc, ivrt, ivxt, ivzt, 1);
+            put(sr_, sx_, sz_, 0, 0, 0, dinc, ivrt, ivxt, ivzt, 1);
             bmin[0] = min(bmin[0], max(ivrt - 1, 0)); bmax[0] = max(bmax[0], min(ivrt + 2, NVR - 1));
             bmin[1] = min(bmin[1], max(ivxt - 1, 0)); bmax[1] = max(bmax[1], min(ivxt + 2, NVX - 1));
             bmin[2] = min(bmin[2], max(ivzt - 1, 0)); bmax[2] = max(bmax[2], min(ivzt + 2, NVZ - 1));
         }
         pr = nr_; px = nx_; pz = nz_;
+        ivro = ivr; ivxo = ivx; ivzo = ivz;
         if (done) break;
     }
     P.n[q] = over ? -3 : (int32_t)ns;
@@ -1795,11 +1785,11 @@ __device__ __forceinline__ double lane_double(double v, int j)
 }
 
 // Frechet rows from recorded ray paths, one wavefront per ray (F:7600-7764).
-// The samples come in 64 at a time, one per lane, and are broadcast one
-// after another.  A lane evaluates only the three weights its tap uses
-// (bspline4's arithmetic, weight by weight); the previous sample's weight
-// on an axis is reused while the lane's vertex offset on that axis stays
-// the same, and otherwise taken from the previous sample's fraction.
+// The points come in 64 at a time, one per lane: the lane turns its point
+// into a sample (fractions, velocity) and evaluates the sample's twelve
+// B-spline weights (four per axis, bspline4's arithmetic) into LDS, once.
+// The samples are then taken one after another: each lane reads the weights
+// its tap uses, of this sample and of the one before, from LDS.
 // Each lane owns the vertices whose (r, theta, phi) indices are congruent
 // to its (lane/16, lane/4 % 4, lane % 4) mod 4, so the 64 taps of every
 // sub-segment land on 64 different lanes and every vertex is only ever
@@ -1810,8 +1800,11 @@ __device__ __forceinline__ double lane_double(double v, int j)
 // reference's float(double(acc) + inc).  The scratch is then scanned over
 // the touched box in column order, keeping |v| >= 1e-6 (F:7744).
 __global__ void __launch_bounds__(64)
-k_ray_frechet(Grid g, int nrays, RayPaths P, float* __restrict__ scratch, RayOut out)
+k_ray_frechet(Grid g, const double* __restrict__ veln, int nrays, RayPaths P, float* __restrict__ scratch, RayOut out)
 {
+    // the weights of the batch's samples at slots 1..64, the previous
+    // batch's last sample at slot 0: [slot][axis * 4 + n]
+    __shared__ double Wt[65][12];
     const int lane = threadIdx.x;
     const int NVR = g.nvr + 2, NVX = g.nvx + 2, NVZ = g.nvz + 2;
     const size_t NV = (size_t)NVR * NVX * NVZ;
@@ -1820,49 +1813,67 @@ k_ray_frechet(Grid g, int nrays, RayPaths P, float* __restrict__ scratch, RayOut
     for (int q = blockIdx.x; q < nrays; q += gridDim.x) {
         const int ns = P.n[q];
         if (ns < 0) { if (lane == 0) out.count[q] = ns; continue; }
-        const RaySample* S = P.rec + (int64_t)q * P.cap;
+        const RayPoint* S = P.rec + (int64_t)q * P.cap;
         int64_t acc_at = -1;   // scratch index of the vertex held in acc, -1 none
         float acc = 0.0f;
         double ivel2 = 1.0;
-        double pt[3] = { 0.0, 0.0, 0.0 };   // previous sample's B-spline fractions (r, theta, phi)
-        double pw[3] = { 0.0, 0.0, 0.0 };   // its weights at offsets pn[], when pn[] >= 0
-        int pn[3] = { -1, -1, -1 };
-        RaySample mine;   // this lane's sample of the current batch of 64
-        mine.tr = mine.tx = mine.tz = mine.ivel2 = mine.dinc = 0.0;
-        mine.ivr = mine.ivx = mine.ivz = mine.kind = 0;
+        // this lane's sample of the current batch of 64 (its weights in Wt)
+        double m_ivel2 = 0.0, m_dinc = 0.0;
+        int m_ivr = 0, m_ivx = 0, m_ivz = 0, m_kind = 0;
         for (int j = 0; j < ns; ++j) {
             const int jb = j & 63;
             if (jb == 0) {
+                // this lane's point of the next 64: its velocity (in the
+                // cell the point keeps, or the one it falls in), its
+                // fractions of its vertex cell and their weights
                 const int k = j + lane;
-                if (k < ns) mine = S[k];
-                else { mine.tr = mine.tx = mine.tz = mine.ivel2 = mine.dinc = 0.0; mine.ivr = mine.ivx = mine.ivz = mine.kind = 0; }
+                double tc[3] = { 0.0, 0.0, 0.0 };
+                if (k < ns) {
+                    const RayPoint Q = S[k];
+                    int ipr = Q.ipr, ipx = Q.ipx, ipz = Q.ipz;
+                    if (Q.kind == 1) {
+                        ipr = (int)floor((g.rgor - Q.r) / g.dnr) + 1;
+                        ipx = (int)floor((Q.x - g.rgox) / g.dnx) + 1;
+                        ipz = (int)floor((Q.z - g.rgoz) / g.dnz) + 1;
+                    }
+                    const double vel = tri_vel(g, veln, ipr, ipx, ipz, (g.rgor - Q.r) - (ipr - 1) * g.dnr,
+                                               (Q.x - g.rgox) - (ipx - 1) * g.dnx, (Q.z - g.rgoz) - (ipz - 1) * g.dnz);
+                    tc[0] = ((g.gor - Q.r) - (Q.ivr - 1) * g.dvr) / g.dvr;
+                    tc[1] = ((Q.x - g.gox) - (Q.ivx - 1) * g.dvx) / g.dvx;
+                    tc[2] = ((Q.z - g.goz) - (Q.ivz - 1) * g.dvz) / g.dvz;
+                    m_ivel2 = 1.0 / (vel * vel);
+                    m_dinc = Q.dinc; m_ivr = Q.ivr; m_ivx = Q.ivx; m_ivz = Q.ivz; m_kind = Q.kind;
+                } else {
+                    m_ivel2 = m_dinc = 0.0; m_ivr = m_ivx = m_ivz = m_kind = 0;
+                }
+                __syncthreads();   // the previous batch's reads of Wt are done
+                // (before the first sample, the weights at fraction 0)
+                if (lane < 12) Wt[0][lane] = j > 0 ? Wt[64][lane] : bspline_w(0.0, lane & 3);
+                __syncthreads();
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int n = 0; n < 4; ++n) Wt[lane + 1][a * 4 + n] = bspline_w(tc[a], n);
+                __syncthreads();
             }
-            RaySample sm;
-            sm.tr = lane_double(mine.tr, jb); sm.tx = lane_double(mine.tx, jb); sm.tz = lane_double(mine.tz, jb);
-            sm.ivel2 = lane_double(mine.ivel2, jb); sm.dinc = lane_double(mine.dinc, jb);
-            sm.ivr = __builtin_amdgcn_readlane(mine.ivr, jb); sm.ivx = __builtin_amdgcn_readlane(mine.ivx, jb);
-            sm.ivz = __builtin_amdgcn_readlane(mine.ivz, jb); sm.kind = __builtin_amdgcn_readlane(mine.kind, jb);
             const double ivel2o = ivel2;
-            ivel2 = sm.ivel2;
-            const double tc[3] = { sm.tr, sm.tx, sm.tz };
-            if (sm.kind != 1) {
-                for (int a = 0; a < 3; ++a) { pt[a] = tc[a]; pn[a] = -1; }
-                continue;
-            }
+            ivel2 = lane_double(m_ivel2, jb);
+            if (__builtin_amdgcn_readlane(m_kind, jb) != 1) continue;
+            const double dinc = lane_double(m_dinc, jb);
+            const int ivr = __builtin_amdgcn_readlane(m_ivr, jb);
+            const int ivx = __builtin_amdgcn_readlane(m_ivx, jb);
+            const int ivz = __builtin_amdgcn_readlane(m_ivz, jb);
             // this lane's tap: vertex (iv - 1 + n) with n the lane's residue offset
-            const int rr = sm.ivr - 1 + (((own_r - (sm.ivr - 1)) % 4 + 4) % 4);
-            const int xx = sm.ivx - 1 + (((own_x - (sm.ivx - 1)) % 4 + 4) % 4);
-            const int zz = sm.ivz - 1 + (((own_z - (sm.ivz - 1)) % 4 + 4) % 4);
-            const int off[3] = { rr - (sm.ivr - 1), xx - (sm.ivx - 1), zz - (sm.ivz - 1) };
-            double wc[3], wp[3];
-            for (int a = 0; a < 3; ++a) {
-                wc[a] = bspline_w(tc[a], off[a]);
-                wp[a] = pn[a] == off[a] ? pw[a] : bspline_w(pt[a], off[a]);
-            }
-            const double e1 = wc[0] * wc[1] * wc[2] * ivel2;
-            const double e0 = wp[0] * wp[1] * wp[2] * ivel2o;
-            for (int a = 0; a < 3; ++a) { pt[a] = tc[a]; pw[a] = wc[a]; pn[a] = off[a]; }
-            const double inc = -(e1 + e0) * sm.dinc / 2.0;
+            const int rr = ivr - 1 + (((own_r - (ivr - 1)) % 4 + 4) % 4);
+            const int xx = ivx - 1 + (((own_x - (ivx - 1)) % 4 + 4) % 4);
+            const int zz = ivz - 1 + (((own_z - (ivz - 1)) % 4 + 4) % 4);
+            const int off[3] = { rr - (ivr - 1), xx - (ivx - 1), zz - (ivz - 1) };
+            // this sample's weights and the previous sample's, at the lane's offsets
+            const double* Wc = Wt[jb + 1];
+            const double* Wp = Wt[jb];
+            const double e1 = Wc[off[0]] * Wc[4 + off[1]] * Wc[8 + off[2]] * ivel2;
+            const double e0 = Wp[off[0]] * Wp[4 + off[1]] * Wp[8 + off[2]] * ivel2o;
+            const double inc = -(e1 + e0) * dinc / 2.0;
             if (zz >= 0 && zz < NVZ && xx >= 0 && xx < NVX && rr >= 0 && rr < NVR) {
                 const int64_t at = ((int64_t)zz * NVX + xx) * NVR + rr;
                 if (at != acc_at) {
@@ -2107,7 +2118,7 @@ struct fmm3d_ctx {
     float* fr_pvals = nullptr;
     size_t fr_packed = 0;          // entries the packed arrays hold
     int64_t fr_rcap = 2048;        // staging entries per ray (grows on overflow)
-    RaySample* fr_path = nullptr;  // ray path samples, fr_pcap per ray
+    RayPoint* fr_path = nullptr;   // ray path points, fr_pcap per ray
     int32_t* fr_pn = nullptr;      // samples per ray
     int32_t* fr_pbox = nullptr;    // touched vertex box per ray
     size_t fr_path_n = 0;          // samples fr_path holds
@@ -4600,11 +4611,11 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
         const int64_t rcap = std::min<int64_t>(c->fr_rcap, (int64_t)NV);
         const int64_t pcap = c->fr_pcap;
         const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>({ (int64_t)(ray_end - r0), (int64_t)(budget / (rcap * 8)),
-                                                                        (int64_t)(pbudget / (pcap * sizeof(RaySample))) }));
+                                                                        (int64_t)(pbudget / (pcap * sizeof(RayPoint))) }));
         if (c->fr_path_n < (size_t)chunk * pcap) {
             (void)hipFree(c->fr_path);
             c->fr_path = nullptr; c->fr_path_n = 0;
-            e = dalloc(&c->fr_path, (size_t)chunk * pcap * sizeof(RaySample));
+            e = dalloc(&c->fr_path, (size_t)chunk * pcap * sizeof(RayPoint));
             if (e != hipSuccess) break;
             c->fr_path_n = (size_t)chunk * pcap;
         }
@@ -4637,13 +4648,13 @@ static int frechet_range(fmm3d_ctx* c, int nrc, const double* rcv, int ray_begin
         RayPaths pp{ c->fr_path, c->fr_pn, c->fr_pbox, pcap };
         const int pgrid = (nr + 63) / 64;
         if (c->sl.prec == FMM3D_PREC_FP64)
-            k_ray_paths<double><<<pgrid, 64, 0, c->stream>>>(g, (const double*)c->sl.T, c->sl.toff, nullptr, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos, dslot);
+            k_ray_paths<double><<<pgrid, 64, 0, c->stream>>>(g, (const double*)c->sl.T, c->sl.toff, nullptr, nrc, drcv, r0, nr, pp, c->sl.srcpos, dslot);
         else
-            k_ray_paths<float><<<pgrid, 64, 0, c->stream>>>(g, (const float*)c->sl.T, c->sl.toff, c->sl.ref, c->veln, nrc, drcv, r0, nr, pp, c->sl.srcpos, dslot);
+            k_ray_paths<float><<<pgrid, 64, 0, c->stream>>>(g, (const float*)c->sl.T, c->sl.toff, c->sl.ref, nrc, drcv, r0, nr, pp, c->sl.srcpos, dslot);
         e = hipGetLastError();
         if (e == hipSuccess) {
             const int grid = (int)std::min<size_t>(workers, (size_t)nr);
-            k_ray_frechet<<<grid, 64, 0, c->stream>>>(g, nr, pp, c->ray_scratch, o);
+            k_ray_frechet<<<grid, 64, 0, c->stream>>>(g, c->veln, nr, pp, c->ray_scratch, o);
             e = hipGetLastError();
         }
         if (e == hipSuccess) {
