@@ -2551,18 +2551,31 @@ __global__ void k_init_tele(int nnr, int nnx, int nnz, int nslots, const double*
 {
     // one slot per grid row; the bottom layer is the index range from base
     const size_t NL = (size_t)nnx * nnz, N = NL * nnr, base = N - NL;
+    constexpr int V = 16 / sizeof(R);   // values per 16-byte store
+    typedef R RV __attribute__((ext_vector_type(V)));
+    const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, nth = (size_t)gridDim.x * blockDim.x;
+    const R inf = Lim<R>::inf();
     for (size_t s = blockIdx.y; s < (size_t)nslots; s += gridDim.y) {
         R* Ts = T + s * N;
+        // the layers above the bottom are unknown: 16-byte stores from the
+        // slot's first 16-byte aligned value, single values before and after
+        const size_t head = std::min(base, (size_t)((16 - ((uintptr_t)Ts & 15)) & 15) / sizeof(R));
+        const size_t nv = (base - head) / V;
+        RV* Tv = reinterpret_cast<RV*>(Ts + head);
+        RV iv;
+#pragma unroll
+        for (int k = 0; k < V; ++k) iv[k] = inf;
+        for (size_t i = tid; i < nv; i += nth) Tv[i] = iv;
+        if (tid < head) Ts[tid] = inf;
+        for (size_t c = head + nv * V + tid; c < base; c += nth) Ts[c] = inf;
         const double* fs = faces + s * NL;
         const double off = toff[s];
         const double* rs = ref ? ref + s * (nnr + 2) : nullptr;
-        for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < N; c += (size_t)gridDim.x * blockDim.x) {
-            R v = Lim<R>::inf();
-            if (c >= base) {
-                const double fv = fs[c - base];
-                const size_t ix = (c - base) / nnz, iz = (c - base) % nnz;
-                if (fv > 0.0) v = std::is_same<R, double>::value ? (R)fv : (R)(fv - off - ref_at(rs, nnr, nnr - 1, ix, iz));
-            }
+        for (size_t c = base + tid; c < N; c += nth) {
+            R v = inf;
+            const double fv = fs[c - base];
+            const size_t ix = (c - base) / nnz, iz = (c - base) % nnz;
+            if (fv > 0.0) v = std::is_same<R, double>::value ? (R)fv : (R)(fv - off - ref_at(rs, nnr, nnr - 1, ix, iz));
             Ts[c] = v;
         }
     }
