@@ -2624,22 +2624,35 @@ __global__ void __launch_bounds__(1024) k_tele_offsets(size_t F, size_t XF, size
 
 __global__ void k_fixbits_tele(int nnr, int nnx, int nnz, int nslots, const double* __restrict__ faces, uint32_t* __restrict__ fix)
 {
-    // one slot per grid row; only words reaching the bottom layer (nodes from
-    // base on) hold bits
+    // one slot per grid row; only words from wb on reach the bottom layer
+    // (nodes from base on) and hold bits
     const size_t NL = (size_t)nnx * nnz, N = NL * nnr, base = N - NL;
-    const size_t NW = (N + 31) / 32;
+    const size_t NW = (N + 31) / 32, wb = base / 32;
+    typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+    const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x, nth = (size_t)gridDim.x * blockDim.x;
+    const int lane = threadIdx.x & 63;
     for (size_t s = blockIdx.y; s < (size_t)nslots; s += gridDim.y) {
         const double* fs = faces + s * NL;
-        for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < NW; w += (size_t)gridDim.x * blockDim.x) {
-            uint32_t bits = 0;
-            if (w * 32 + 31 >= base) {
-                for (int b = 0; b < 32; ++b) {
-                    const size_t c = w * 32 + b;
-                    if (c >= N) break;
-                    if (c >= base && fs[c - base] > 0.0) bits |= 1u << b;
-                }
+        uint32_t* fw = fix + s * NW;
+        // words below wb: zero, in 16-byte stores from the first aligned word
+        const size_t head = std::min(wb, (size_t)((16 - ((uintptr_t)fw & 15)) & 15) / 4);
+        const size_t nv = (wb - head) / 4;
+        U4* fv = reinterpret_cast<U4*>(fw + head);
+        for (size_t i = tid; i < nv; i += nth) fv[i] = U4{ 0u, 0u, 0u, 0u };
+        if (tid < head) fw[tid] = 0u;
+        for (size_t w = head + nv * 4 + tid; w < wb; w += nth) fw[w] = 0u;
+        // words from wb: one node per lane; a wavefront's ballot is two
+        // words (every wavefront starts at a multiple of 64 nodes from wb)
+        const size_t nb = (NW - wb) * 32, nbr = (nb + 63) / 64 * 64;
+        for (size_t j = tid; j < nbr; j += nth) {
+            const size_t c = wb * 32 + j;
+            const bool f = c >= base && c < N && fs[c - base] > 0.0;
+            const uint64_t m = __ballot(f);
+            if (lane == 0) {
+                const size_t w = wb + j / 32;
+                fw[w] = (uint32_t)m;
+                if (w + 1 < NW) fw[w + 1] = (uint32_t)(m >> 32);
             }
-            fix[s * NW + w] = bits;
         }
     }
 }
